@@ -1,0 +1,677 @@
+// als.hip -- CP-ALS sweep kernels and driver for gfx950.
+//
+// One mode update (reference src/cp_als.cu:46-63, 66-82, 85-101; src/cp4_als.cu:67-151):
+//   V  = hadamard of the other modes' Grams           (FP64, R x R)
+//   M  = MTTKRP_n(X, other factors)                    (fused kernel, split-K FP32 partials)
+//   U_n = M V^{-1}                                      (FP64, written FP32 column-major
+//                                                        plus the panel layout the next
+//                                                        MTTKRP reads)
+//   G_n = U_n^T U_n                                     (FP64, fixed-order reduction)
+// The reference does the same with a KR batched GEMM, an SGEMM on an unfolded copy of X,
+// FP32 LU (getrf/getrs) with a cudaMalloc per call, and a transpose (Sgeam).
+//
+// Kernels per mode: k_vinv (one workgroup, Cholesky + triangular inverse, Jacobi
+// pseudo-inverse fallback) runs on a forked stream concurrently with the MTTKRP, then
+// k_solve (one workgroup per 64 rows) and k_gram_reduce (one workgroup, also the fit).
+// A sweep is captured once into a hipGraph and replayed.
+#include "mttkrp.hpp"
+#include "linalg.hpp"
+#include <math.h>
+#include <string.h>
+#include <vector>
+
+namespace tdx {
+
+static constexpr int kMaxR = 64;     // rank limit of the LDS-resident solve
+static constexpr int kSolveRows = 64;
+
+// V = hadamard_{m != mode} G_m; Vinv = V^{-1} by Cholesky (lower) and L^{-1}; on failure
+// a pseudo-inverse from the Jacobi eigendecomposition (eigenvalues below 1e-12 * max
+// are dropped) and status |= 1.
+__global__ __launch_bounds__(256) void k_vinv(const double *__restrict__ G0, const double *__restrict__ G1,
+                                              const double *__restrict__ G2, const double *__restrict__ G3,
+                                              int order, int mode, int R, double *__restrict__ Vinv,
+                                              int *__restrict__ status) {
+    __shared__ double V[kMaxR * kMaxR];
+    __shared__ double L[kMaxR * kMaxR];
+    __shared__ double W[kMaxR * kMaxR];
+    __shared__ double sh[256];
+    __shared__ double cs[4 * (kMaxR / 2 + 1)];
+    __shared__ int fail;
+    const int tid = threadIdx.x, nt = blockDim.x, RR = R * R;
+    const double *G[4] = {G0, G1, G2, G3};
+    for (int e = tid; e < RR; e += nt) {
+        double v = 1.0;
+        for (int m = 0; m < order; ++m)
+            if (m != mode) v *= G[m][e];
+        V[e] = v;
+        L[e] = v;
+    }
+    if (tid == 0) fail = 0;
+    __syncthreads();
+    // right-looking Cholesky on the lower triangle of L
+    for (int j = 0; j < R; ++j) {
+        if (tid == 0) {
+            double d = L[j + j * R];
+            if (!(d > 0.0) || !isfinite(d)) fail = 1;
+            L[j + j * R] = sqrt(fmax(d, 0.0));
+        }
+        __syncthreads();
+        if (fail) break;
+        const double d = L[j + j * R];
+        for (int i = j + 1 + tid; i < R; i += nt) L[i + j * R] /= d;
+        __syncthreads();
+        const int m = R - j - 1;
+        for (int e = tid; e < m * m; e += nt) {
+            int i = j + 1 + e % m, k = j + 1 + e / m;
+            if (k <= i) L[i + k * R] -= L[i + j * R] * L[k + j * R];
+        }
+        __syncthreads();
+    }
+    if (!fail) {
+        // W = L^{-1} (lower): column c by forward substitution, one thread per column
+        for (int c = tid; c < R; c += nt) {
+            for (int i = 0; i < c; ++i) W[i + c * R] = 0.0;
+            for (int i = c; i < R; ++i) {
+                double s = (i == c) ? 1.0 : 0.0;
+                for (int k = c; k < i; ++k) s -= L[i + k * R] * W[k + c * R];
+                W[i + c * R] = s / L[i + i * R];
+            }
+        }
+        __syncthreads();
+        // V^{-1} = W^T W
+        for (int e = tid; e < RR; e += nt) {
+            int a = e % R, b = e / R, k0 = a > b ? a : b;
+            double s = 0.0;
+            for (int k = k0; k < R; ++k) s += W[k + a * R] * W[k + b * R];
+            Vinv[e] = s;
+        }
+        return;
+    }
+    // pseudo-inverse fallback: V = Q diag(w) Q^T
+    for (int e = tid; e < RR; e += nt) L[e] = V[e];
+    __syncthreads();
+    jacobi_eig_block(L, W, R, cs, sh);
+    double wmax = 0.0;
+    for (int a = 0; a < R; ++a) wmax = fmax(wmax, fabs(L[a + a * R]));
+    for (int e = tid; e < RR; e += nt) {
+        int a = e % R, b = e / R;
+        double s = 0.0;
+        for (int k = 0; k < R; ++k) {
+            double lam = L[k + k * R];
+            if (fabs(lam) > 1e-12 * wmax) s += W[a + k * R] * W[b + k * R] / lam;
+        }
+        Vinv[e] = s;
+    }
+    if (tid == 0) atomicOr(status, 1);
+}
+
+// U_n rows = (sum of split partials) V^{-1}.  One workgroup per 64 rows, 256 threads,
+// each thread owns 16 (row, column) elements of the 64 x Rp block.
+// Also writes the panel layout (optionally row-scaled for FP8 storage), the per-block
+// inner product <M, U> (fit) and the per-block Gram partial U_blk^T U_blk.
+__global__ __launch_bounds__(256) void k_solve(const float *__restrict__ partial, int nsplit, long rows_pad, int Rp,
+                                               long In, int R, const double *__restrict__ Vinv,
+                                               float *__restrict__ U, float *__restrict__ panel,
+                                               const float *__restrict__ panel_scale,
+                                               double *__restrict__ ip_part, double *__restrict__ gram_part) {
+    __shared__ double Vs[kMaxR * kMaxR];
+    __shared__ double Ms[kSolveRows * kMaxR];
+    __shared__ double sh[256];
+    const int tid = threadIdx.x;
+    const long row0 = (long)blockIdx.x * kSolveRows;
+    const int RR = R * R;
+    for (int e = tid; e < RR; e += 256) Vs[e] = Vinv[e];
+    const int nel = kSolveRows * Rp;  // elements of the block (<= 64*64)
+    // 1. partial sums in fixed split order, FP64
+    double m[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) m[u] = 0.0;
+    for (int sp = 0; sp < nsplit; ++sp) {
+        const float *src = partial + ((long)sp * rows_pad + row0) * Rp;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int e = tid + 256 * u;
+            if (e < nel && row0 + e / Rp < rows_pad) m[u] += (double)src[e];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int e = tid + 256 * u;
+        if (e < nel) Ms[(e / Rp) * kMaxR + (e % Rp)] = m[u];
+    }
+    __syncthreads();
+    // 2. U = M V^{-1}, inner product with M
+    double uval[16];
+    double ip = 0.0;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int e = tid + 256 * u;
+        uval[u] = 0.0;
+        if (e < nel) {
+            const int r = e / Rp, b = e % Rp;
+            const long i = row0 + r;
+            float uf = 0.0f;
+            if (b < R && i < In) {
+                double s = 0.0;
+                for (int a = 0; a < R; ++a) s += Ms[r * kMaxR + a] * Vs[a + b * R];
+                uf = (float)s;
+                ip += Ms[r * kMaxR + b] * (double)uf;
+            }
+            uval[u] = (double)uf;
+        }
+    }
+    __syncthreads();
+    // 3. write outputs; stage U (as rounded FP32 values) for the Gram partial
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int e = tid + 256 * u;
+        if (e < nel) {
+            const int r = e / Rp, b = e % Rp;
+            const long i = row0 + r;
+            Ms[r * kMaxR + b] = uval[u];
+            if (b < R && i < In) U[i + (long)b * In] = (float)uval[u];
+            float pv = (float)uval[u];
+            if (panel_scale && i < In) pv *= panel_scale[i];
+            panel[((i >> 3) * Rp + b) * 8 + (i & 7)] = pv;
+        }
+    }
+    ip = block_sum(ip, sh);
+    if (tid == 0) ip_part[blockIdx.x] = ip;
+    // Ms now holds U block (sync inside block_sum)
+    double *gp = gram_part + (long)blockIdx.x * RR;
+    for (int e = tid; e < RR; e += 256) {
+        const int a = e % R, b = e / R;
+        double s = 0.0;
+        for (int r = 0; r < kSolveRows; ++r) s += Ms[r * kMaxR + a] * Ms[r * kMaxR + b];
+        gp[e] = s;
+    }
+}
+
+// G_mode = sum of the block Gram partials (fixed order).  With fit != 0 also computes
+// the fast-fit relative error of Tensorlab cpd_als.m:281-283,
+//   ||X - Xhat||^2 = ||X||^2 - 2 <M, U_n> + sum(hadamard of all Grams),
+// appends it to fit_hist[*counter] and increments *counter.
+__global__ __launch_bounds__(256) void k_gram_reduce(const double *__restrict__ gram_part, int nblk, int R,
+                                                     double *__restrict__ Gout, const double *__restrict__ ip_part,
+                                                     const double *G0, const double *G1, const double *G2,
+                                                     const double *G3, int order, int fit,
+                                                     const double *__restrict__ normx2, double *__restrict__ fit_hist,
+                                                     int *__restrict__ counter) {
+    __shared__ double sh[256];
+    const int tid = threadIdx.x, RR = R * R;
+    for (int e = tid; e < RR; e += 256) {
+        double s = 0.0;
+        for (int b = 0; b < nblk; ++b) s += gram_part[(long)b * RR + e];
+        Gout[e] = s;
+    }
+    if (!fit) return;
+    __syncthreads();
+    __threadfence_block();
+    const double *G[4] = {G0, G1, G2, G3};
+    double hx = 0.0;
+    for (int e = tid; e < RR; e += 256) {
+        double v = 1.0;
+        for (int m = 0; m < order; ++m) v *= G[m][e];
+        hx += v;
+    }
+    hx = block_sum(hx, sh);
+    double ip = 0.0;
+    for (int b = tid; b < nblk; b += 256) ip += ip_part[b];
+    ip = block_sum(ip, sh);
+    if (tid == 0) {
+        const double nx2 = *normx2;
+        const double r2 = nx2 - 2.0 * ip + hx;
+        const int c = *counter;
+        fit_hist[c] = sqrt(fabs(r2) / nx2);
+        *counter = c + 1;
+    }
+}
+
+// direct Gram U^T U (FP64) of a column-major factor; one workgroup
+__global__ __launch_bounds__(256) void k_gram_direct(const float *__restrict__ U, long I, int R, double *__restrict__ G) {
+    const int RR = R * R;
+    for (int e = threadIdx.x; e < RR; e += 256) {
+        const int a = e % R, b = e / R;
+        double s = 0.0;
+        for (long i = 0; i < I; ++i) s += (double)U[i + a * I] * (double)U[i + b * I];
+        G[e] = s;
+    }
+}
+
+// ||X||^2 in FP64 with Kahan compensation per thread; per-block partials then one block
+template <typename T>
+__global__ __launch_bounds__(256) void k_norm2_partial(const T *__restrict__ X, long n, const float *__restrict__ scales,
+                                                       long I, double *__restrict__ part) {
+    __shared__ double sh[256];
+    double s = 0.0, c = 0.0;
+    for (long e = blockIdx.x * 256L + threadIdx.x; e < n; e += (long)gridDim.x * 256) {
+        double x = (double)X[e];
+        if (scales) x *= (double)scales[e % I];
+        double y = x * x - c, t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void k_sum_partials(const double *__restrict__ part, int n, double *__restrict__ out) {
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) s += part[i];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) *out = s;
+}
+
+__global__ void k_set_int(int *p, int v) { *p = v; }
+
+// ---------------------------------------------------------------------------
+// host driver
+
+struct CpPlan {
+    int order = 0;
+    long dims[4] = {1, 1, 1, 1};
+    long R = 0;
+    int Rp = 0;
+    float *U[4] = {nullptr, nullptr, nullptr, nullptr};
+    // device allocations (owned)
+    float *panel[4] = {nullptr, nullptr, nullptr, nullptr};
+    double *G[4] = {nullptr, nullptr, nullptr, nullptr};
+    double *Vinv[4] = {nullptr, nullptr, nullptr, nullptr};
+    float *partial = nullptr;
+    size_t partial_floats = 0;
+    double *ip_part = nullptr;
+    double *gram_part = nullptr;
+    double *fit_hist = nullptr;
+    int *counter = nullptr;
+    int *status = nullptr;
+    double *normx2 = nullptr;
+    double *scratch = nullptr;
+    int max_sweeps_hist = 0;
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork[4], ev_join[4];
+    bool events = false;
+
+    ~CpPlan() { release(); }
+    void release() {
+        for (int m = 0; m < 4; ++m) {
+            if (panel[m]) (void)hipFree(panel[m]);
+            if (G[m]) (void)hipFree(G[m]);
+            if (Vinv[m]) (void)hipFree(Vinv[m]);
+            panel[m] = nullptr; G[m] = nullptr; Vinv[m] = nullptr;
+        }
+        void *ps[] = {partial, ip_part, gram_part, fit_hist, counter, status, normx2, scratch};
+        for (void *p : ps)
+            if (p) (void)hipFree(p);
+        partial = nullptr; ip_part = nullptr; gram_part = nullptr; fit_hist = nullptr;
+        counter = nullptr; status = nullptr; normx2 = nullptr; scratch = nullptr;
+        if (events) {
+            for (int m = 0; m < 4; ++m) {
+                (void)hipEventDestroy(ev_fork[m]);
+                (void)hipEventDestroy(ev_join[m]);
+            }
+            events = false;
+        }
+        if (side) (void)hipStreamDestroy(side);
+        side = nullptr;
+    }
+};
+
+struct Storage {
+    const void *X;
+    int dtype;
+    const float *scales;  // FP8 slice scales
+    MttkrpPlan mp[4];
+};
+
+static int plan_setup(CpPlan &P, int order, const long *dims, long R, float *const *factors, int max_hist) {
+    P.order = order;
+    for (int m = 0; m < order; ++m) P.dims[m] = dims[m];
+    P.R = R;
+    P.Rp = (int)(ceil_div(R, 16) * 16);
+    long nblk_max = 0;
+    for (int m = 0; m < order; ++m) {
+        P.U[m] = factors[m];
+        TDX_HIP(hipMalloc(&P.panel[m], sizeof(float) * panel_floats(dims[m], P.Rp)));
+        TDX_HIP(hipMalloc(&P.G[m], sizeof(double) * R * R));
+        TDX_HIP(hipMalloc(&P.Vinv[m], sizeof(double) * R * R));
+        long nb = ceil_div(dims[m], kSolveRows);
+        if (nb > nblk_max) nblk_max = nb;
+    }
+    TDX_HIP(hipMalloc(&P.ip_part, sizeof(double) * nblk_max));
+    TDX_HIP(hipMalloc(&P.gram_part, sizeof(double) * nblk_max * R * R));
+    P.max_sweeps_hist = max_hist < 1 ? 1 : max_hist;
+    TDX_HIP(hipMalloc(&P.fit_hist, sizeof(double) * P.max_sweeps_hist));
+    TDX_HIP(hipMalloc(&P.counter, sizeof(int)));
+    TDX_HIP(hipMalloc(&P.status, sizeof(int)));
+    TDX_HIP(hipMalloc(&P.normx2, sizeof(double)));
+    TDX_HIP(hipMalloc(&P.scratch, sizeof(double) * 4096));
+    TDX_HIP(hipStreamCreateWithFlags(&P.side, hipStreamNonBlocking));
+    for (int m = 0; m < 4; ++m) {
+        TDX_HIP(hipEventCreateWithFlags(&P.ev_fork[m], hipEventDisableTiming));
+        TDX_HIP(hipEventCreateWithFlags(&P.ev_join[m], hipEventDisableTiming));
+    }
+    P.events = true;
+    return TDX_OK;
+}
+
+static int ensure_partial(CpPlan &P, const Storage &S) {
+    size_t need = 0;
+    for (int m = 0; m < P.order; ++m)
+        if (S.mp[m].partial_floats() > need) need = S.mp[m].partial_floats();
+    if (need > P.partial_floats) {
+        if (P.partial) TDX_HIP(hipFree(P.partial));
+        P.partial = nullptr;
+        TDX_HIP(hipMalloc(&P.partial, sizeof(float) * need));
+        P.partial_floats = need;
+    }
+    return TDX_OK;
+}
+
+static int storage_setup(CpPlan &P, Storage &S, const void *X, int dtype, const float *scales) {
+    S.X = X;
+    S.dtype = dtype;
+    S.scales = scales;
+    for (int m = 0; m < P.order; ++m) {
+        int st = mttkrp_plan(S.mp[m], P.order, P.dims, m, dtype, P.R, X);
+        if (st) return st;
+    }
+    return ensure_partial(P, S);
+}
+
+// (re)packs panels and Grams from the current factors
+static int refresh_factors(CpPlan &P, const Storage &S, hipStream_t s) {
+    for (int m = 0; m < P.order; ++m) {
+        const float *sc = (S.dtype == TDX_FP8_E4M3 && m == 0) ? S.scales : nullptr;
+        int st = pack_panel(P.U[m], P.dims[m], P.R, P.Rp, sc, P.panel[m], s);
+        if (st) return st;
+        k_gram_direct<<<1, 256, 0, s>>>(P.U[m], P.dims[m], (int)P.R, P.G[m]);
+        TDX_LAUNCH_CHECK();
+    }
+    return TDX_OK;
+}
+
+static int norm2(const void *X, int dtype, const float *scales, long I, long n, double *scratch, double *out,
+                 hipStream_t s) {
+    const int blocks = 1024;
+    if (dtype == TDX_F32)
+        k_norm2_partial<float><<<blocks, 256, 0, s>>>((const float *)X, n, nullptr, I, scratch);
+    else
+        return record(TDX_ERR_UNSUPPORTED);
+    k_sum_partials<<<1, 256, 0, s>>>(scratch, blocks, out);
+    TDX_LAUNCH_CHECK();
+    (void)scales;
+    return TDX_OK;
+}
+
+// enqueue one sweep (all modes) on stream s
+static int enqueue_sweep(CpPlan &P, const Storage &S, hipStream_t s, bool fit) {
+    const int R = (int)P.R;
+    for (int n = 0; n < P.order; ++n) {
+        // fork: V^{-1} on the side stream while the MTTKRP streams X
+        TDX_HIP(hipEventRecord(P.ev_fork[n], s));
+        TDX_HIP(hipStreamWaitEvent(P.side, P.ev_fork[n], 0));
+        k_vinv<<<1, 256, 0, P.side>>>(P.G[0], P.G[1], P.order > 2 ? P.G[2] : P.G[0],
+                                      P.order > 3 ? P.G[3] : P.G[0], P.order, n, R, P.Vinv[n], P.status);
+        TDX_LAUNCH_CHECK();
+        TDX_HIP(hipEventRecord(P.ev_join[n], P.side));
+        const float *panels[4] = {P.panel[0], P.panel[1], P.panel[2], P.panel[3]};
+        const float *row_scale = (S.dtype == TDX_FP8_E4M3 && n == 0) ? S.scales : nullptr;
+        int st = mttkrp_launch(S.mp[n], S.X, panels, row_scale, P.partial, s);
+        if (st) return st;
+        TDX_HIP(hipStreamWaitEvent(s, P.ev_join[n], 0));
+        const long In = P.dims[n];
+        const int nblk = (int)ceil_div(In, kSolveRows);
+        const float *pscale = (S.dtype == TDX_FP8_E4M3 && n == 0) ? S.scales : nullptr;
+        k_solve<<<nblk, 256, 0, s>>>(P.partial, S.mp[n].nsplit, S.mp[n].rows_pad, P.Rp, In, R, P.Vinv[n], P.U[n],
+                                     P.panel[n], pscale, P.ip_part, P.gram_part);
+        TDX_LAUNCH_CHECK();
+        const bool do_fit = fit && (n == P.order - 1);
+        k_gram_reduce<<<1, 256, 0, s>>>(P.gram_part, nblk, R, P.G[n], P.ip_part, P.G[0], P.G[1],
+                                        P.order > 2 ? P.G[2] : P.G[0], P.order > 3 ? P.G[3] : P.G[0], P.order,
+                                        do_fit ? 1 : 0, P.normx2, P.fit_hist, P.counter);
+        TDX_LAUNCH_CHECK();
+    }
+    return TDX_OK;
+}
+
+struct SweepGraph {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    ~SweepGraph() {
+        if (ge) (void)hipGraphExecDestroy(ge);
+        if (g) (void)hipGraphDestroy(g);
+    }
+};
+
+static int capture_sweep(CpPlan &P, const Storage &S, hipStream_t s, bool fit, SweepGraph &G) {
+    TDX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int st = enqueue_sweep(P, S, s, fit);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (st) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess) return record_hip(e, "hipStreamEndCapture", __FILE__, __LINE__);
+    G.g = g;
+    TDX_HIP(hipGraphInstantiate(&G.ge, g, nullptr, nullptr, 0));
+    return TDX_OK;
+}
+
+// Runs `sweeps` sweeps on storage S; records per-sweep ms if sweep_ms != NULL.
+static int run_sweeps(CpPlan &P, const Storage &S, hipStream_t s, int sweeps, bool fit, bool graph, float *sweep_ms,
+                      double tol, double *fit_host_last, int *done) {
+    *done = 0;
+    if (sweeps <= 0) return TDX_OK;
+    SweepGraph G;
+    if (graph) {
+        int st = capture_sweep(P, S, s, fit, G);
+        if (st) return st;
+    }
+    std::vector<hipEvent_t> evs;
+    if (sweep_ms) {
+        evs.resize(sweeps + 1);
+        for (auto &e : evs) TDX_HIP(hipEventCreate(&e));
+        TDX_HIP(hipEventRecord(evs[0], s));
+    }
+    double prev = -1.0;
+    int st = TDX_OK;
+    for (int it = 0; it < sweeps; ++it) {
+        if (graph) {
+            hipError_t e = hipGraphLaunch(G.ge, s);
+            if (e != hipSuccess) { st = record_hip(e, "hipGraphLaunch", __FILE__, __LINE__); break; }
+        } else {
+            st = enqueue_sweep(P, S, s, fit);
+            if (st) break;
+        }
+        if (sweep_ms) {
+            hipError_t e = hipEventRecord(evs[it + 1], s);
+            if (e != hipSuccess) { st = record_hip(e, "hipEventRecord", __FILE__, __LINE__); break; }
+        }
+        *done = it + 1;
+        if (tol > 0.0 && fit) {
+            int c = 0;
+            double f = 0.0;
+            hipError_t e = hipStreamSynchronize(s);
+            if (e == hipSuccess) e = hipMemcpy(&c, P.counter, sizeof(int), hipMemcpyDeviceToHost);
+            if (e == hipSuccess && c > 0) e = hipMemcpy(&f, P.fit_hist + (c - 1), sizeof(double), hipMemcpyDeviceToHost);
+            if (e != hipSuccess) { st = record_hip(e, "fit readback", __FILE__, __LINE__); break; }
+            if (prev >= 0.0 && fabs(prev - f) < tol) break;
+            prev = f;
+        }
+    }
+    if (sweep_ms && st == TDX_OK) {
+        TDX_HIP(hipEventSynchronize(evs[*done]));
+        for (int it = 0; it < *done; ++it) TDX_HIP(hipEventElapsedTime(&sweep_ms[it], evs[it], evs[it + 1]));
+    }
+    for (auto &e : evs) (void)hipEventDestroy(e);
+    (void)fit_host_last;
+    return st;
+}
+
+}  // namespace tdx
+
+using namespace tdx;
+
+extern "C" {
+
+void tdx_cp_default_options(tdx_cp_options *o) {
+    if (!o) return;
+    memset(o, 0, sizeof(*o));
+    o->max_sweeps = 500;
+    o->refine_sweeps = 0;
+    o->storage = TDX_F32;
+    o->init = 0;
+    o->seed = 1234;
+    o->tol = 0.0;
+    o->record_fit = 1;
+    o->learned = 0;
+    o->use_graph = 1;
+}
+
+int tdx_cp_als_ex(const float *X, int order, const long *dims, long rank, float *const *factors,
+                  const tdx_cp_options *opt_in, tdx_cp_report *report, double *fit_hist, float *sweep_ms) {
+    if (!X || !dims || !factors || order < 3 || order > 4 || rank <= 0) return record(TDX_ERR_ARG);
+    if (rank > kMaxR) return record(TDX_ERR_UNSUPPORTED);
+    tdx_cp_options opt;
+    if (opt_in) opt = *opt_in;
+    else tdx_cp_default_options(&opt);
+    if (opt.max_sweeps < 0 || opt.refine_sweeps < 0) return record(TDX_ERR_ARG);
+    if (opt.storage != TDX_F32 && opt.storage != TDX_BF16 && opt.storage != TDX_FP8_E4M3) return record(TDX_ERR_ARG);
+    long N = 1;
+    for (int m = 0; m < order; ++m) {
+        if (dims[m] <= 0 || !factors[m]) return record(TDX_ERR_ARG);
+        N *= dims[m];
+    }
+    // Graph capture is not allowed on the legacy default stream: when the library stream
+    // is the default one, run on a private blocking stream (it orders after earlier
+    // default-stream work by the legacy-stream rules) and join it before returning.
+    hipStream_t s = stream();
+    hipStream_t owned = nullptr;
+    if (!s) {
+        TDX_HIP(hipStreamCreate(&owned));
+        s = owned;
+    }
+    struct StreamGuard {
+        hipStream_t st;
+        ~StreamGuard() {
+            if (st) {
+                (void)hipStreamSynchronize(st);
+                (void)hipStreamDestroy(st);
+            }
+        }
+    } guard{owned};
+    CpPlan P;
+    const int total = opt.max_sweeps + (opt.storage == TDX_F32 ? 0 : opt.refine_sweeps);
+    int st = plan_setup(P, order, dims, rank, factors, total);
+    if (st) return st;
+    // initialisation: modes 1..d-1 from the seeded generator unless the caller supplied them
+    if (opt.init == 0) {
+        for (int m = 1; m < order; ++m) {
+            st = tdx_fill_uniform(factors[m], dims[m] * rank, opt.seed, (uint32_t)m);
+            if (st) return st;
+        }
+    }
+    k_set_int<<<1, 1, 0, s>>>(P.counter, 0);
+    k_set_int<<<1, 1, 0, s>>>(P.status, 0);
+    st = norm2(X, TDX_F32, nullptr, dims[0], N, P.scratch, P.normx2, s);
+    if (st) return st;
+
+    // storage copies for the main phase
+    void *Xq = nullptr;
+    float *scales = nullptr;
+    Storage S32;
+    st = storage_setup(P, S32, X, TDX_F32, nullptr);
+    if (st) return st;
+    Storage Smain = S32;
+    if (opt.storage == TDX_BF16) {
+        TDX_HIP(hipMalloc(&Xq, sizeof(uint16_t) * N));
+        st = tdx_quantize_bf16(X, (uint16_t *)Xq, N);
+        if (st) { (void)hipFree(Xq); return st; }
+        st = storage_setup(P, Smain, Xq, TDX_BF16, nullptr);
+    } else if (opt.storage == TDX_FP8_E4M3) {
+        TDX_HIP(hipMalloc(&Xq, N));
+        TDX_HIP(hipMalloc(&scales, sizeof(float) * dims[0]));
+        st = tdx_quantize_fp8_mode1(X, (uint8_t *)Xq, scales, dims[0], N / dims[0]);
+        if (!st) st = storage_setup(P, Smain, Xq, TDX_FP8_E4M3, scales);
+    }
+    int done_main = 0, done_ref = 0;
+    if (!st) st = refresh_factors(P, Smain, s);
+    hipEvent_t t0, t1;
+    TDX_HIP(hipEventCreate(&t0));
+    TDX_HIP(hipEventCreate(&t1));
+    (void)hipEventRecord(t0, s);
+    if (!st)
+        st = run_sweeps(P, Smain, s, opt.max_sweeps, opt.record_fit != 0, opt.use_graph != 0, sweep_ms, opt.tol,
+                        nullptr, &done_main);
+    if (!st && opt.storage != TDX_F32 && opt.refine_sweeps > 0) {
+        // FP32 refinement: the mode-0 panel must be unscaled again (rebuilt by the first solve)
+        st = refresh_factors(P, S32, s);
+        if (!st)
+            st = run_sweeps(P, S32, s, opt.refine_sweeps, opt.record_fit != 0, opt.use_graph != 0,
+                            sweep_ms ? sweep_ms + done_main : nullptr, 0.0, nullptr, &done_ref);
+    }
+    (void)hipEventRecord(t1, s);
+    hipError_t e = hipStreamSynchronize(s);
+    if (Xq) (void)hipFree(Xq);
+    if (scales) (void)hipFree(scales);
+    if (e != hipSuccess && !st) st = record_hip(e, "hipStreamSynchronize", __FILE__, __LINE__);
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, t0, t1);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (st) return record(st);
+    int status = 0, count = 0;
+    TDX_HIP(hipMemcpy(&status, P.status, sizeof(int), hipMemcpyDeviceToHost));
+    TDX_HIP(hipMemcpy(&count, P.counter, sizeof(int), hipMemcpyDeviceToHost));
+    double last = 0.0;
+    if (count > 0) {
+        std::vector<double> h(count);
+        TDX_HIP(hipMemcpy(h.data(), P.fit_hist, sizeof(double) * count, hipMemcpyDeviceToHost));
+        if (fit_hist) memcpy(fit_hist, h.data(), sizeof(double) * count);
+        last = h[count - 1];
+    }
+    int result = status ? TDX_WARN_SINGULAR : TDX_OK;
+    if (opt.record_fit && count > 0 && !isfinite(last)) result = TDX_ERR_NONFINITE;
+    if (report) {
+        report->sweeps_done = done_main + done_ref;
+        report->status = result;
+        report->relerr_fit = last;
+        report->seconds = ms * 1e-3;
+    }
+    return record(result);
+}
+
+int tdx_cp_als(const float *X, float *A, float *B, float *C, long a, long b, long c, long r) {
+    const long dims[3] = {a, b, c};
+    float *f[3] = {A, B, C};
+    tdx_cp_options o;
+    tdx_cp_default_options(&o);
+    o.record_fit = 0;
+    return tdx_cp_als_ex(X, 3, dims, r, f, &o, nullptr, nullptr, nullptr);
+}
+
+int tdx_cp_als4(const float *X, float *A, float *B, float *C, float *D, long a, long b, long c, long d, long r) {
+    const long dims[4] = {a, b, c, d};
+    float *f[4] = {A, B, C, D};
+    tdx_cp_options o;
+    tdx_cp_default_options(&o);
+    o.record_fit = 0;
+    return tdx_cp_als_ex(X, 4, dims, r, f, &o, nullptr, nullptr, nullptr);
+}
+
+int tdx_cp_tensorcore(const float *X, float *A, float *B, float *C, long a, long b, long c, long r) {
+    const long dims[3] = {a, b, c};
+    float *f[3] = {A, B, C};
+    tdx_cp_options o;
+    tdx_cp_default_options(&o);
+    o.storage = TDX_BF16;
+    o.refine_sweeps = 2;
+    o.max_sweeps = 498;
+    o.record_fit = 0;
+    return tdx_cp_als_ex(X, 3, dims, r, f, &o, nullptr, nullptr, nullptr);
+}
+
+}  // extern "C"

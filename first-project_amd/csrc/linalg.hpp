@@ -1,0 +1,96 @@
+// linalg.hpp -- single-workgroup FP64 dense kernels shared by the CP and Tucker paths.
+#pragma once
+#include "common.hpp"
+#include <math.h>
+
+namespace tdx {
+
+// ---------------------------------------------------------------------------
+// block reduction helper (FP64)
+__device__ inline double block_sum(double v, double *sh) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (t < s) sh[t] += sh[t + s];
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Cyclic Jacobi with round-robin (tournament) pair ordering, one workgroup, FP64.
+// A (n x n, lda n) is diagonalised in place; Q accumulates the eigenvectors.
+// Golub & Van Loan, section 8.5 (parallel Jacobi ordering, 8.5.4).
+__device__ inline void jacobi_eig_block(double *A, double *Q, int n, double *cs, double *sh) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int ne = n + (n & 1);  // even number of players; index n is a dummy if n is odd
+    for (int e = tid; e < n * n; e += nt) Q[e] = ((e % n) == (e / n)) ? 1.0 : 0.0;
+    __syncthreads();
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        double off = 0.0, tot = 0.0;
+        for (int e = tid; e < n * n; e += nt) {
+            double v = A[e] * A[e];
+            tot += v;
+            if ((e % n) != (e / n)) off += v;
+        }
+        off = block_sum(off, sh);
+        tot = block_sum(tot, sh);
+        if (off <= 1e-30 * tot || off == 0.0) break;
+        for (int step = 0; step < ne - 1; ++step) {
+            // pair k: positions k and ne-1-k; position 0 holds player 0, position p>0
+            // holds player ((p - 1 + step) % (ne - 1)) + 1
+            for (int k = tid; k < ne / 2; k += nt) {
+                int pa = k, pb = ne - 1 - k;
+                int ia = (pa == 0) ? 0 : ((pa - 1 + step) % (ne - 1)) + 1;
+                int ib = ((pb - 1 + step) % (ne - 1)) + 1;
+                int p = ia < ib ? ia : ib, q = ia < ib ? ib : ia;
+                double c = 1.0, s = 0.0;
+                if (q < n) {
+                    double apq = A[p + q * n];
+                    if (apq != 0.0) {
+                        double app = A[p + p * n], aqq = A[q + q * n];
+                        double tau = (aqq - app) / (2.0 * apq);
+                        double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                        c = 1.0 / sqrt(1.0 + t * t);
+                        s = t * c;
+                    }
+                }
+                cs[4 * k + 0] = c;
+                cs[4 * k + 1] = s;
+                cs[4 * k + 2] = (double)p;
+                cs[4 * k + 3] = (double)q;
+            }
+            __syncthreads();
+            // columns: A := A J, Q := Q J
+            for (int e = tid; e < (ne / 2) * n; e += nt) {
+                int k = e / n, row = e % n;
+                int p = (int)cs[4 * k + 2], q = (int)cs[4 * k + 3];
+                if (q >= n) continue;
+                double c = cs[4 * k], s = cs[4 * k + 1];
+                double akp = A[row + p * n], akq = A[row + q * n];
+                A[row + p * n] = c * akp - s * akq;
+                A[row + q * n] = s * akp + c * akq;
+                double qkp = Q[row + p * n], qkq = Q[row + q * n];
+                Q[row + p * n] = c * qkp - s * qkq;
+                Q[row + q * n] = s * qkp + c * qkq;
+            }
+            __syncthreads();
+            // rows: A := J^T A
+            for (int e = tid; e < (ne / 2) * n; e += nt) {
+                int k = e / n, col = e % n;
+                int p = (int)cs[4 * k + 2], q = (int)cs[4 * k + 3];
+                if (q >= n) continue;
+                double c = cs[4 * k], s = cs[4 * k + 1];
+                double apk = A[p + col * n], aqk = A[q + col * n];
+                A[p + col * n] = c * apk - s * aqk;
+                A[q + col * n] = s * apk + c * aqk;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+}  // namespace tdx

@@ -1,0 +1,566 @@
+// mttkrp.hip -- fused MTTKRP for gfx950 (CDNA4).
+//
+// M[i_n, r] = sum over all other indices of X[...] * prod_{m != n} U_m[i_m, r]
+//
+// The tensor streams from HBM exactly once per call; the Khatri-Rao rows are formed in
+// registers from factor "panels" and never materialised (the reference writes a
+// (prod I_m) x R KR matrix with a batched GEMM per mode, src/cp_als.cu:46,66,85, and for
+// mode 2 also an unfolded copy of X, src/cp_als.cu:26-28).
+//
+// Contraction view.  The output mode n has extent In.  The fastest contracted mode f
+// (mode 0, or mode 1 when n == 0) is walked in K steps of 16 (FP32) or 32 (BF16/FP8)
+// indices; every other contracted ("slow") mode is fixed within a K step, so its factor
+// rows fold into one per-column scalar w[r].  A K step therefore multiplies a
+// (rows x KS) block of X by the (KS x R) block  panel_f[f0:f0+KS, :] * w.
+//
+// Two load shapes:
+//  * case B (n > 0): X is contiguous along the contraction (mode 0).  Each lane loads 16 B
+//    (8 BF16 / 4 FP32) or 8 B (8 FP8) straight into its MFMA A-operand slot.
+//  * case A (n == 0): X is contiguous along the OUTPUT rows.  Each lane loads 8 rows for
+//    8 consecutive K indices and transposes the 8x8 block inside its registers, so the
+//    MFMA operand is built without LDS.
+// MFMA: v_mfma_f32_16x16x32_bf16 (BF16 and FP8 storage, FP8 widened exactly to BF16) and
+// v_mfma_f32_16x16x4_f32 (FP32), accumulation in FP32.  Each wave owns 128 (64 for FP32)
+// output rows x Rp columns; the 4 waves of a workgroup walk interleaved K steps of one
+// split, are summed in LDS in a fixed order, and the workgroup writes one FP32 partial.
+// Partials are summed over splits in FP64 by the consumer (mttkrp_reduce or the ALS solve).
+#include "mttkrp.hpp"
+
+namespace tdx {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+struct KArgs {
+    const void *X;
+    const float *panel[4];
+    const float *row_scale;
+    float *partial;
+    long stride[4];
+    long dims[4];
+    long In, If, FB, ksteps, kper, rows_pad;
+    int order, mode, fmode, nslow, Rp, rowtiles;
+    int slow[3];
+};
+
+__device__ __forceinline__ uint32_t pack_bf16(float a, float b) {
+    bf16x2 v;
+    v.x = (__bf16)a;
+    v.y = (__bf16)b;
+    return __builtin_bit_cast(uint32_t, v);
+}
+
+// 8 FP8 E4M3 codes (little-endian in two dwords) -> 8 BF16 packed in 4 dwords; exact
+__device__ __forceinline__ uint4 fp8x8_to_bf16x8(uint2 q) {
+    f32x2 a = __builtin_amdgcn_cvt_pk_f32_fp8((int)q.x, false);
+    f32x2 b = __builtin_amdgcn_cvt_pk_f32_fp8((int)q.x, true);
+    f32x2 c = __builtin_amdgcn_cvt_pk_f32_fp8((int)q.y, false);
+    f32x2 d = __builtin_amdgcn_cvt_pk_f32_fp8((int)q.y, true);
+    uint4 o;
+    o.x = pack_bf16(a.x, a.y);
+    o.y = pack_bf16(b.x, b.y);
+    o.z = pack_bf16(c.x, c.y);
+    o.w = pack_bf16(d.x, d.y);
+    return o;
+}
+
+__device__ __forceinline__ uint32_t u4_get(const uint4 &v, int i) {
+    return i == 0 ? v.x : i == 1 ? v.y : i == 2 ? v.z : v.w;
+}
+
+// 16-bit element t of an 8-element vector held in a uint4
+__device__ __forceinline__ uint32_t half_of(const uint4 &v, int t) {
+    uint32_t w = u4_get(v, t >> 1);
+    return (t & 1) ? (w >> 16) : (w & 0xffffu);
+}
+
+template <int ST, bool CA, int NB>
+__global__ __launch_bounds__(256, 2) void k_mttkrp(KArgs a) {
+    constexpr int KS = (ST == 0) ? 16 : 32;
+    constexpr int T = (ST == 0) ? 4 : 8;
+    constexpr int RW = 16 * T;
+    __shared__ float red[RW * 64];
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lm = lane & 15, lg = lane >> 4;
+    const int rowtile = blockIdx.x % a.rowtiles;
+    const long split = blockIdx.x / a.rowtiles;
+    const long row0 = (long)rowtile * RW;
+    const long kbeg = split * a.kper;
+    const long kend = min(kbeg + a.kper, a.ksteps);
+    const int Rp = a.Rp;
+
+    f32x4 acc[T][NB];
+#pragma unroll
+    for (int t = 0; t < T; ++t)
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) acc[t][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    float w[NB];
+    long cur_slow = -1, slow_off = 0;
+    const float *pf = a.panel[a.fmode];
+
+    long fb = (kbeg + wave) % a.FB;
+    long sl = (kbeg + wave) / a.FB;
+    for (long ks = kbeg + wave; ks < kend; ks += 4, fb += 4) {
+        while (fb >= a.FB) { fb -= a.FB; ++sl; }
+        if (sl != cur_slow) {  // wave-uniform
+            cur_slow = sl;
+            long rem = sl;
+            slow_off = 0;
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) w[nb] = 1.0f;
+            for (int q = 0; q < a.nslow; ++q) {
+                const int m = a.slow[q];
+                const long idx = rem % a.dims[m];
+                rem /= a.dims[m];
+                slow_off += idx * a.stride[m];
+                const float *pm = a.panel[m] + ((idx >> 3) * Rp) * 8 + (idx & 7);
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb) w[nb] *= pm[(16 * nb + lm) * 8];
+            }
+        }
+        const long f0 = fb * KS;
+
+        if constexpr (ST == 0) {
+            // ---------------- FP32 storage, v_mfma_f32_16x16x4_f32 ----------------
+            // K slot g of substep s <-> fast index f0 + 4g + s
+            const long fj = f0 + 4 * lg;
+            float kr[NB][4];
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) {
+                const float4 p = *reinterpret_cast<const float4 *>(
+                    pf + ((fj >> 3) * Rp + 16 * nb + lm) * 8 + (fj & 7));
+                kr[nb][0] = p.x * w[nb];
+                kr[nb][1] = p.y * w[nb];
+                kr[nb][2] = p.z * w[nb];
+                kr[nb][3] = p.w * w[nb];
+            }
+            const float *X = reinterpret_cast<const float *>(a.X);
+            float xv[T][4];  // [row block t][substep s]
+            if constexpr (CA) {
+                const long r = row0 + 4 * lm;
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    const long j = fj + s;
+                    float4 v = float4{0.f, 0.f, 0.f, 0.f};
+                    if (r < a.In && j < a.If) v = *reinterpret_cast<const float4 *>(X + slow_off + j * a.stride[1] + r);
+                    xv[0][s] = v.x; xv[1][s] = v.y; xv[2][s] = v.z; xv[3][s] = v.w;
+                }
+            } else {
+#pragma unroll
+                for (int t = 0; t < T; ++t) {
+                    const long o = row0 + 16 * t + lm;
+                    float4 v = float4{0.f, 0.f, 0.f, 0.f};
+                    if (o < a.In && fj < a.If) v = *reinterpret_cast<const float4 *>(X + slow_off + o * a.stride[a.mode] + fj);
+                    xv[t][0] = v.x; xv[t][1] = v.y; xv[t][2] = v.z; xv[t][3] = v.w;
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+#pragma unroll
+                for (int t = 0; t < T; ++t)
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb)
+                        acc[t][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(xv[t][s], kr[nb][s], acc[t][nb], 0, 0, 0);
+        } else {
+            // -------- BF16 / FP8 storage, v_mfma_f32_16x16x32_bf16 --------
+            // K slot group g holds fast indices f0 + 8g .. f0 + 8g + 7
+            const long fj = f0 + 8 * lg;
+            bf16x8 krb[NB];
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) {
+                const float *pp = pf + ((fj >> 3) * Rp + 16 * nb + lm) * 8;
+                const float4 p0 = *reinterpret_cast<const float4 *>(pp);
+                const float4 p1 = *reinterpret_cast<const float4 *>(pp + 4);
+                const float ww = w[nb];
+                uint4 u;
+                u.x = pack_bf16(p0.x * ww, p0.y * ww);
+                u.y = pack_bf16(p0.z * ww, p0.w * ww);
+                u.z = pack_bf16(p1.x * ww, p1.y * ww);
+                u.w = pack_bf16(p1.z * ww, p1.w * ww);
+                krb[nb] = __builtin_bit_cast(bf16x8, u);
+            }
+            uint4 xa[T];
+            if constexpr (CA) {
+                // lane loads rows row0+8lm .. +7 for fast indices fj + s, s = 0..7
+                const long r = row0 + 8 * lm;
+                uint4 v[8];
+#pragma unroll
+                for (int s = 0; s < 8; ++s) {
+                    const long j = fj + s;
+                    v[s] = uint4{0u, 0u, 0u, 0u};
+                    if (r < a.In && j < a.If) {
+                        const long off = slow_off + j * a.stride[1] + r;
+                        if constexpr (ST == 1)
+                            v[s] = *reinterpret_cast<const uint4 *>(reinterpret_cast<const uint16_t *>(a.X) + off);
+                        else
+                            v[s] = fp8x8_to_bf16x8(*reinterpret_cast<const uint2 *>(reinterpret_cast<const uint8_t *>(a.X) + off));
+                    }
+                }
+                // in-register 8x8 transpose of 16-bit elements: xa[t] = (v[0][t], ..., v[7][t])
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    uint4 o;
+                    o.x = half_of(v[0], t) | (half_of(v[1], t) << 16);
+                    o.y = half_of(v[2], t) | (half_of(v[3], t) << 16);
+                    o.z = half_of(v[4], t) | (half_of(v[5], t) << 16);
+                    o.w = half_of(v[6], t) | (half_of(v[7], t) << 16);
+                    xa[t] = o;
+                }
+            } else {
+#pragma unroll
+                for (int t = 0; t < T; ++t) {
+                    const long o = row0 + 16 * t + lm;
+                    xa[t] = uint4{0u, 0u, 0u, 0u};
+                    if (o < a.In && fj < a.If) {
+                        const long off = slow_off + o * a.stride[a.mode] + fj;
+                        if constexpr (ST == 1)
+                            xa[t] = *reinterpret_cast<const uint4 *>(reinterpret_cast<const uint16_t *>(a.X) + off);
+                        else
+                            xa[t] = fp8x8_to_bf16x8(*reinterpret_cast<const uint2 *>(reinterpret_cast<const uint8_t *>(a.X) + off));
+                    }
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+                const bf16x8 av = __builtin_bit_cast(bf16x8, xa[t]);
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb)
+                    acc[t][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, krb[nb], acc[t][nb], 0, 0, 0);
+            }
+        }
+    }
+
+    // ---- fixed-order reduction of the 4 waves in LDS ----
+    // acc[t][nb][v] is MFMA output (row mr = 4*(lane/16)+v, col 16nb + lane%16)
+    for (int wv = 0; wv < 4; ++wv) {
+        if (wave == wv) {
+#pragma unroll
+            for (int t = 0; t < T; ++t)
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int mr = 4 * lg + v;
+                        int lr;
+                        if constexpr (CA) lr = (ST == 0) ? (4 * mr + t) : (8 * mr + t);
+                        else lr = 16 * t + mr;
+                        const int idx = lr * Rp + 16 * nb + lm;
+                        red[idx] = (wv == 0 ? 0.0f : red[idx]) + acc[t][nb][v];
+                    }
+        }
+        __syncthreads();
+    }
+    // zero any padded columns beyond 16*NB (Rp == 16*NB always, so none), write partial
+    float *dst = a.partial + (split * a.rows_pad + row0) * Rp;
+    for (int e = threadIdx.x; e < RW * Rp; e += 256) {
+        float v = red[e];
+        if constexpr (CA && ST == 2) {
+            const long r = row0 + e / Rp;
+            v *= (r < a.In) ? a.row_scale[r] : 0.0f;
+        }
+        dst[e] = v;
+    }
+}
+
+// Generic fallback for shapes the register-tile kernel does not cover (odd leading
+// extent, rank > 64): one workgroup per (output row, split), one lane per column.
+template <int ST>
+__global__ __launch_bounds__(64) void k_mttkrp_generic(KArgs a, long kelems) {
+    const long row = blockIdx.x % a.In;
+    const long split = blockIdx.x / a.In;
+    const long k0 = split * a.kper, k1 = min(k0 + a.kper, kelems);
+    const int Rp = a.Rp;
+    for (int c0 = 0; c0 < Rp; c0 += 64) {
+        const int r = c0 + threadIdx.x;
+        double accd = 0.0;
+        for (long e = k0; e < k1; ++e) {
+            long rem = e, off = row * a.stride[a.mode];
+            float kr = 1.0f;
+            for (int m = 0; m < a.order; ++m) {
+                if (m == a.mode) continue;
+                const long idx = rem % a.dims[m];
+                rem /= a.dims[m];
+                off += idx * a.stride[m];
+                if (r < Rp) kr *= a.panel[m][((idx >> 3) * Rp + r) * 8 + (idx & 7)];
+            }
+            float x;
+            if constexpr (ST == 0) x = reinterpret_cast<const float *>(a.X)[off];
+            else if constexpr (ST == 1) x = __uint_as_float((uint32_t)reinterpret_cast<const uint16_t *>(a.X)[off] << 16);
+            else {
+                const uint8_t q = reinterpret_cast<const uint8_t *>(a.X)[off];
+                f32x2 d = __builtin_amdgcn_cvt_pk_f32_fp8((int)q, false);
+                x = d.x;
+            }
+            accd += (double)x * (double)kr;
+        }
+        if (r < Rp) {
+            float v = (float)accd;
+            if (ST == 2 && a.mode == 0) v *= a.row_scale[row];
+            a.partial[(split * a.rows_pad + row) * Rp + r] = v;
+        }
+    }
+}
+
+__global__ void k_pack_panel(const float *__restrict__ U, long I, long R, int Rp, const float *__restrict__ scale,
+                             float *__restrict__ panel, long Ipad) {
+    const long n = Ipad * Rp;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+        const long i8 = e & 7;
+        const long r = (e >> 3) % Rp;
+        const long ib = (e >> 3) / Rp;
+        const long i = ib * 8 + i8;
+        float v = 0.0f;
+        if (i < I && r < R) {
+            v = U[i + r * I];
+            if (scale) v *= scale[i];
+        }
+        panel[e] = v;
+    }
+}
+
+__global__ void k_mttkrp_reduce(const float *__restrict__ partial, float *__restrict__ out, long In, long R, int Rp,
+                                long rows_pad, int nsplit) {
+    const long n = In * R;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+        const long i = e % In, r = e / In;
+        double s = 0.0;
+        for (int sp = 0; sp < nsplit; ++sp) s += (double)partial[(sp * rows_pad + i) * Rp + r];
+        out[e] = (float)s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+static int storage_type(int dtype) {
+    return dtype == TDX_F32 ? 0 : dtype == TDX_BF16 ? 1 : dtype == TDX_FP8_E4M3 ? 2 : -1;
+}
+
+int mttkrp_plan(MttkrpPlan &p, int order, const long *dims, int mode, int dtype, long rank,
+                const void *X, int target_wgs) {
+    if (order < 2 || order > 4 || mode < 0 || mode >= order || rank <= 0) return record(TDX_ERR_ARG);
+    const int st = storage_type(dtype);
+    if (st < 0) return record(TDX_ERR_UNSUPPORTED);
+    p = MttkrpPlan();
+    p.order = order;
+    for (int m = 0; m < order; ++m) {
+        if (dims[m] <= 0) return record(TDX_ERR_ARG);
+        p.dims[m] = dims[m];
+    }
+    p.mode = mode;
+    p.dtype = dtype;
+    p.rank = rank;
+    p.Rp = (int)(ceil_div(rank, 16) * 16);
+    const long In = dims[mode];
+    const int fmode = (mode == 0) ? 1 : 0;
+    const int vec = (st == 0) ? 4 : 8;
+    const size_t align = (st == 2) ? 8 : 16;
+    p.fast = rank <= 64 && (dims[0] % vec == 0) && (((uintptr_t)X) % align == 0);
+    if (target_wgs <= 0) target_wgs = 512;
+    if (p.fast) {
+        const int KS = (st == 0) ? 16 : 32;
+        p.rows_per_wg = (st == 0) ? 64 : 128;
+        p.rowtiles = (int)ceil_div(In, p.rows_per_wg);
+        p.rows_pad = (long)p.rowtiles * p.rows_per_wg;
+        long S = 1;
+        for (int m = 0; m < order; ++m)
+            if (m != mode && m != fmode) S *= dims[m];
+        p.ksteps = ceil_div(dims[fmode], KS) * S;
+        long want = ceil_div(target_wgs, p.rowtiles);
+        long maxs = ceil_div(p.ksteps, 16);  // at least 4 K steps per wave
+        long ns = want < maxs ? want : maxs;
+        if (ns < 1) ns = 1;
+        p.ksteps_per_split = ceil_div(p.ksteps, ns);
+        p.nsplit = (int)ceil_div(p.ksteps, p.ksteps_per_split);
+    } else {
+        long kelems = 1;
+        for (int m = 0; m < order; ++m)
+            if (m != mode) kelems *= dims[m];
+        p.rows_per_wg = 1;
+        p.rowtiles = (int)In;
+        p.rows_pad = In;
+        p.ksteps = kelems;
+        long want = ceil_div(4096, In);
+        long maxs = ceil_div(kelems, 64);
+        long ns = want < maxs ? want : maxs;
+        if (ns < 1) ns = 1;
+        p.ksteps_per_split = ceil_div(kelems, ns);
+        p.nsplit = (int)ceil_div(kelems, p.ksteps_per_split);
+    }
+    return TDX_OK;
+}
+
+int pack_panel(const float *U, long I, long R, int Rp, const float *scale, float *panel, hipStream_t s) {
+    const long Ipad = panel_rows(I);
+    const long n = Ipad * Rp;
+    long g = ceil_div(n, 256);
+    if (g > 4096) g = 4096;
+    k_pack_panel<<<(int)g, 256, 0, s>>>(U, I, R, Rp, scale, panel, Ipad);
+    TDX_LAUNCH_CHECK();
+    return TDX_OK;
+}
+
+template <int ST, bool CA>
+static void launch_fast(const MttkrpPlan &p, const KArgs &a, hipStream_t s) {
+    const int grid = p.rowtiles * p.nsplit;
+    switch (p.Rp / 16) {
+        case 1: k_mttkrp<ST, CA, 1><<<grid, 256, 0, s>>>(a); break;
+        case 2: k_mttkrp<ST, CA, 2><<<grid, 256, 0, s>>>(a); break;
+        case 3: k_mttkrp<ST, CA, 3><<<grid, 256, 0, s>>>(a); break;
+        default: k_mttkrp<ST, CA, 4><<<grid, 256, 0, s>>>(a); break;
+    }
+}
+
+int mttkrp_launch(const MttkrpPlan &p, const void *X, const float *const *panels, const float *row_scale,
+                  float *partial, hipStream_t s) {
+    KArgs a{};
+    a.X = X;
+    for (int m = 0; m < 4; ++m) a.panel[m] = (m < p.order) ? panels[m] : nullptr;
+    a.row_scale = row_scale;
+    a.partial = partial;
+    long st = 1;
+    for (int m = 0; m < 4; ++m) {
+        a.dims[m] = (m < p.order) ? p.dims[m] : 1;
+        a.stride[m] = st;
+        st *= a.dims[m];
+    }
+    a.order = p.order;
+    a.mode = p.mode;
+    a.fmode = (p.mode == 0) ? 1 : 0;
+    a.In = p.dims[p.mode];
+    a.If = p.dims[a.fmode];
+    a.Rp = p.Rp;
+    a.rowtiles = p.rowtiles;
+    a.rows_pad = p.rows_pad;
+    a.kper = p.ksteps_per_split;
+    a.ksteps = p.ksteps;
+    a.nslow = 0;
+    for (int m = 0; m < p.order; ++m)
+        if (m != p.mode && m != a.fmode) a.slow[a.nslow++] = m;
+    const int stt = storage_type(p.dtype);
+    if (p.dtype == TDX_FP8_E4M3 && p.mode == 0 && !row_scale) return record(TDX_ERR_ARG);
+    if (p.fast) {
+        const int KS = (stt == 0) ? 16 : 32;
+        a.FB = ceil_div(a.If, KS);
+        const bool ca = (p.mode == 0);
+        if (stt == 0) { if (ca) launch_fast<0, true>(p, a, s); else launch_fast<0, false>(p, a, s); }
+        else if (stt == 1) { if (ca) launch_fast<1, true>(p, a, s); else launch_fast<1, false>(p, a, s); }
+        else { if (ca) launch_fast<2, true>(p, a, s); else launch_fast<2, false>(p, a, s); }
+    } else {
+        a.FB = 1;
+        const int grid = (int)(a.In * p.nsplit);
+        if (stt == 0) k_mttkrp_generic<0><<<grid, 64, 0, s>>>(a, p.ksteps);
+        else if (stt == 1) k_mttkrp_generic<1><<<grid, 64, 0, s>>>(a, p.ksteps);
+        else k_mttkrp_generic<2><<<grid, 64, 0, s>>>(a, p.ksteps);
+    }
+    TDX_LAUNCH_CHECK();
+    return TDX_OK;
+}
+
+int mttkrp_reduce(const MttkrpPlan &p, const float *partial, float *out, hipStream_t s) {
+    const long n = p.dims[p.mode] * p.rank;
+    long g = ceil_div(n, 256);
+    if (g > 4096) g = 4096;
+    k_mttkrp_reduce<<<(int)g, 256, 0, s>>>(partial, out, p.dims[p.mode], p.rank, p.Rp, p.rows_pad, p.nsplit);
+    TDX_LAUNCH_CHECK();
+    return TDX_OK;
+}
+
+}  // namespace tdx
+
+using namespace tdx;
+
+extern "C" {
+
+int tdx_mttkrp_ex(const void *X, int dtype, const float *scales, int order, const long *dims,
+                  const float *const *factors, long rank, int mode, float *out) {
+    if (!X || !dims || !factors || !out) return record(TDX_ERR_ARG);
+    MttkrpPlan p;
+    int st = mttkrp_plan(p, order, dims, mode, dtype, rank, X);
+    if (st) return st;
+    for (int m = 0; m < order; ++m)
+        if (m != mode && !factors[m]) return record(TDX_ERR_ARG);
+    if (dtype == TDX_FP8_E4M3 && !scales) return record(TDX_ERR_ARG);
+    hipStream_t s = stream();
+    const float *panels[4] = {nullptr, nullptr, nullptr, nullptr};
+    for (int m = 0; m < order; ++m) {
+        if (m == mode) continue;
+        void *ws = nullptr;
+        st = workspace(WS_PANEL0 + m, sizeof(float) * panel_floats(dims[m], p.Rp), &ws);
+        if (st) return st;
+        const float *sc = (dtype == TDX_FP8_E4M3 && m == 0) ? scales : nullptr;
+        st = pack_panel(factors[m], dims[m], rank, p.Rp, sc, (float *)ws, s);
+        if (st) return record(st);
+        panels[m] = (const float *)ws;
+    }
+    void *part = nullptr;
+    st = workspace(WS_PARTIAL, sizeof(float) * p.partial_floats(), &part);
+    if (st) return st;
+    st = mttkrp_launch(p, X, panels, (dtype == TDX_FP8_E4M3 && mode == 0) ? scales : nullptr, (float *)part, s);
+    if (st) return record(st);
+    st = mttkrp_reduce(p, (const float *)part, out, s);
+    return record(st);
+}
+
+// Times `iters` launches of the MTTKRP kernel alone (panels and workspace prepared once)
+// with hipEvents on the library stream; *avg_ms receives the mean launch duration.
+int tdx_bench_mttkrp(const void *X, int dtype, const float *scales, int order, const long *dims,
+                     const float *const *factors, long rank, int mode, int iters, float *avg_ms) {
+    if (!X || !dims || !factors || !avg_ms || iters <= 0) return record(TDX_ERR_ARG);
+    MttkrpPlan p;
+    int st = mttkrp_plan(p, order, dims, mode, dtype, rank, X);
+    if (st) return st;
+    if (dtype == TDX_FP8_E4M3 && !scales) return record(TDX_ERR_ARG);
+    hipStream_t s = stream();
+    const float *panels[4] = {nullptr, nullptr, nullptr, nullptr};
+    for (int m = 0; m < order; ++m) {
+        if (m == mode) continue;
+        if (!factors[m]) return record(TDX_ERR_ARG);
+        void *ws = nullptr;
+        st = workspace(WS_PANEL0 + m, sizeof(float) * panel_floats(dims[m], p.Rp), &ws);
+        if (st) return st;
+        st = pack_panel(factors[m], dims[m], rank, p.Rp, (dtype == TDX_FP8_E4M3 && m == 0) ? scales : nullptr,
+                        (float *)ws, s);
+        if (st) return record(st);
+        panels[m] = (const float *)ws;
+    }
+    void *part = nullptr;
+    st = workspace(WS_PARTIAL, sizeof(float) * p.partial_floats(), &part);
+    if (st) return st;
+    const float *rs = (dtype == TDX_FP8_E4M3 && mode == 0) ? scales : nullptr;
+    for (int w = 0; w < 2; ++w) {
+        st = mttkrp_launch(p, X, panels, rs, (float *)part, s);
+        if (st) return record(st);
+    }
+    hipEvent_t e0, e1;
+    TDX_HIP(hipEventCreate(&e0));
+    TDX_HIP(hipEventCreate(&e1));
+    TDX_HIP(hipEventRecord(e0, s));
+    for (int it = 0; it < iters && !st; ++it) st = mttkrp_launch(p, X, panels, rs, (float *)part, s);
+    TDX_HIP(hipEventRecord(e1, s));
+    TDX_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    TDX_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st) return record(st);
+    *avg_ms = ms / (float)iters;
+    return record(TDX_OK);
+}
+
+// reference mttkrp(X, left, right, XU, flag, m, n, k, r): left/right are the slow/fast
+// factors of the Khatri-Rao product (src/mttkrp.cu:11-77)
+int tdx_mttkrp(const float *X, const float *left, const float *right, float *XU,
+               long flag, long m, long n, long k, long r) {
+    if (flag < 1 || flag > 3) return record(TDX_ERR_ARG);
+    const long dims[3] = {m, n, k};
+    const float *f[3] = {nullptr, nullptr, nullptr};
+    if (flag == 1) { f[1] = right; f[2] = left; }
+    else if (flag == 2) { f[0] = right; f[2] = left; }
+    else { f[0] = right; f[1] = left; }
+    return tdx_mttkrp_ex(X, TDX_F32, nullptr, 3, dims, f, r, (int)flag - 1, XU);
+}
+
+}  // extern "C"

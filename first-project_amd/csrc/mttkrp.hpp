@@ -1,0 +1,48 @@
+// mttkrp.hpp -- fused MTTKRP (tensor unfolding times on-the-fly Khatri-Rao product).
+#pragma once
+#include "common.hpp"
+
+namespace tdx {
+
+// Factor "panel" layout: rows grouped by 8, then column, then the 8 rows:
+//   panel[((i >> 3) * Rp + r) * 8 + (i & 7)] = U[i + r*I]   (zero padded to Rp columns
+//   and to a multiple of 64 rows)
+// so one lane reads the 8 consecutive rows of one column with two 16-byte loads and
+// 16 lanes of consecutive columns read 512 contiguous bytes.
+inline long panel_rows(long I) { return ceil_div(I, 64) * 64; }
+inline long panel_floats(long I, int Rp) { return panel_rows(I) * (long)Rp; }
+
+struct MttkrpPlan {
+    int order = 0;
+    long dims[4] = {1, 1, 1, 1};
+    int mode = 0;
+    int dtype = TDX_F32;
+    long rank = 0;
+    int Rp = 0;          // rank padded to a multiple of 16
+    bool fast = false;   // MFMA register-tile kernel usable
+    int rows_per_wg = 0; // output rows per workgroup tile
+    int rowtiles = 0;
+    long rows_pad = 0;   // rowtiles * rows_per_wg
+    int nsplit = 0;      // K splits (workgroups per row tile)
+    long ksteps = 0;
+    long ksteps_per_split = 0;
+    size_t partial_floats() const { return (size_t)nsplit * rows_pad * Rp; }
+};
+
+// chooses kernel variant and split; returns TDX_OK or an error
+int mttkrp_plan(MttkrpPlan &p, int order, const long *dims, int mode, int dtype, long rank,
+                const void *X, int target_wgs = 0);
+
+// Launches the split-K MTTKRP writing partial[split][row][Rp] (FP32).
+// panels[m] are factor panels (panels[mode] unused); for FP8 storage panels[0] must be
+// pre-scaled by the slice scales when mode != 0, and row_scale = scales when mode == 0.
+int mttkrp_launch(const MttkrpPlan &p, const void *X, const float *const *panels,
+                  const float *row_scale, float *partial, hipStream_t s);
+
+// panel packing (optionally scaling row i by scale[i])
+int pack_panel(const float *U, long I, long R, int Rp, const float *scale, float *panel, hipStream_t s);
+
+// sums the split partials into out (I_mode x rank, column-major FP32)
+int mttkrp_reduce(const MttkrpPlan &p, const float *partial, float *out, hipStream_t s);
+
+}  // namespace tdx

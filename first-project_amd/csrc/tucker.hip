@@ -1,0 +1,342 @@
+// tucker.hip -- Tucker HOSVD and HOOI for gfx950.
+//
+// HOSVD (src/tucker_hosvd.cu:3-81, src/tucker4_hosvd.cu:3-121): per mode the Gram of the
+// unfolding X(n) X(n)^T, a symmetric eigensolve, and the eigenvectors of the r_n largest
+// eigenvalues in ascending order (the reference copies the last r columns of the
+// ascending syevdx output, src/tucker_hosvd.cu:68-70); core = X x_1 U_1^T ... x_d U_d^T.
+// HOOI (decomposition/GPU/normal/tucker/tucker_hooi.cu:103-168, generalised to order 4):
+// per mode, Y = X x_{m != n} U_m^T, then the same Gram + eigensolve on Y(n).
+//
+// The Gram never forms an unfolded copy (the reference materialises X2 with t2m and, for
+// order 4, X2 and X3 by identity-matrix GEMMs, src/tucker4_hosvd.cu:23-37): a workgroup
+// stages 32 fibres of the (P, In, Q) view in LDS and accumulates an In x In FP32 tile,
+// partial tiles are summed in FP64 in a fixed order.  The eigensolve is FP64 cyclic
+// Jacobi in one workgroup; eigenvector signs are fixed so that the entry of largest
+// magnitude is positive (syevdx leaves the sign unspecified).
+#include "common.hpp"
+#include "linalg.hpp"
+#include <vector>
+
+namespace tdx {
+
+int ttm_generic(const float *X, int order, const long *dims, int mode, const float *U, long lda, long ldi, long Ra,
+                float *Y, hipStream_t s);
+
+static constexpr int kGramMaxN = 128;
+static constexpr int kGramChunk = 32;
+
+// partial Gram of the (P, In, Q) view over fibre chunks: thread (ty, tx) owns an 8x8 block
+// of the In x In tile (In <= 128).  Fibre k of a chunk is (p, q) with p fastest.
+__global__ __launch_bounds__(256) void k_gram_partial(const float *__restrict__ X, long P, long In, long Q,
+                                                      long kper, float *__restrict__ part) {
+    __shared__ float xs[kGramChunk][kGramMaxN + 1];
+    const int tid = threadIdx.x;
+    const int tx = tid & 15, ty = tid >> 4;
+    const long K = P * Q;
+    const long k0 = blockIdx.x * kper, k1 = min(k0 + kper, K);
+    float acc[8][8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[i][j] = 0.0f;
+    for (long kb = k0; kb < k1; kb += kGramChunk) {
+        // stage X[fibre kb+kk][a] for kk < 32, a < In
+        for (int e = tid; e < kGramChunk * kGramMaxN; e += 256) {
+            int kk, a;
+            if (P == 1) { a = e % kGramMaxN; kk = e / kGramMaxN; }   // contiguous along a
+            else { kk = e % kGramChunk; a = e / kGramChunk; }      // contiguous along p
+            const long k = kb + kk;
+            float v = 0.0f;
+            if (k < k1 && a < In) {
+                const long p = k % P, q = k / P;
+                v = X[p + a * P + q * P * In];
+            }
+            xs[kk][a] = v;
+        }
+        __syncthreads();
+        for (int kk = 0; kk < kGramChunk; ++kk) {
+            float av[8], bv[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                av[i] = xs[kk][ty * 8 + i];
+                bv[i] = xs[kk][tx * 8 + i];
+            }
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+#pragma unroll
+                for (int j = 0; j < 8; ++j) acc[i][j] = fmaf(av[i], bv[j], acc[i][j]);
+        }
+        __syncthreads();
+    }
+    float *dst = part + (long)blockIdx.x * kGramMaxN * kGramMaxN;
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) dst[(ty * 8 + i) * kGramMaxN + tx * 8 + j] = acc[i][j];
+}
+
+__global__ void k_gram_sum(const float *__restrict__ part, int nblk, long In, double *__restrict__ G) {
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < In * In; e += (long)gridDim.x * blockDim.x) {
+        const long a = e % In, b = e / In;
+        double s = 0.0;
+        for (int k = 0; k < nblk; ++k) s += (double)part[(long)k * kGramMaxN * kGramMaxN + a * kGramMaxN + b];
+        G[e] = s;
+    }
+}
+
+// Top-r eigenvectors of the symmetric n x n G (FP64), one workgroup, dynamic LDS holds A.
+// U (n x r, col-major FP32) receives the eigenvectors of the r largest eigenvalues in
+// ascending order; Q is an n x n FP64 global scratch for the eigenvectors.
+__global__ __launch_bounds__(256) void k_top_eigvecs(const double *__restrict__ G, int n, int r, double *Q,
+                                                     float *__restrict__ U, double *__restrict__ evals) {
+    extern __shared__ double A[];
+    __shared__ double sh[256];
+    __shared__ double cs[4 * (kGramMaxN / 2 + 1)];
+    __shared__ int pos[kGramMaxN];
+    const int tid = threadIdx.x;
+    for (int e = tid; e < n * n; e += 256) A[e] = G[e];
+    __syncthreads();
+    jacobi_eig_block(A, Q, n, cs, sh);
+    // rank of each eigenvalue in ascending order (ties broken by index)
+    for (int i = tid; i < n; i += 256) {
+        const double wi = A[i + i * n];
+        int rk = 0;
+        for (int j = 0; j < n; ++j) {
+            const double wj = A[j + j * n];
+            if (wj < wi || (wj == wi && j < i)) ++rk;
+        }
+        pos[i] = rk;
+        if (evals) evals[rk] = wi;
+    }
+    __syncthreads();
+    // column c of U <- eigenvector with rank n - r + c, sign fixed
+    for (int i = tid; i < n; i += 256) {
+        const int c = pos[i] - (n - r);
+        if (c < 0) continue;
+        int am = 0;
+        double best = fabs(Q[0 + i * n]);
+        for (int k = 1; k < n; ++k) {
+            const double v = fabs(Q[k + i * n]);
+            if (v > best) { best = v; am = k; }
+        }
+        const double sg = (Q[am + i * n] < 0.0) ? -1.0 : 1.0;
+        for (int k = 0; k < n; ++k) U[k + (long)c * n] = (float)(sg * Q[k + i * n]);
+    }
+}
+
+__global__ void k_sumsq(const float *__restrict__ X, long n, double *__restrict__ out) {
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (long e = threadIdx.x; e < n; e += 256) s += (double)X[e] * (double)X[e];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) *out = s;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_norm2_blocks(const T *__restrict__ X, long n, double *__restrict__ part) {
+    __shared__ double sh[256];
+    double s = 0.0, c = 0.0;
+    for (long e = blockIdx.x * 256L + threadIdx.x; e < n; e += (long)gridDim.x * 256) {
+        double x = (double)X[e];
+        double y = x * x - c, t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void k_sum_to(const double *__restrict__ part, int n, double *__restrict__ out) {
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) s += part[i];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) *out = s;
+}
+
+// ---------------------------------------------------------------------------
+struct TuckerWs {
+    float *gram_part = nullptr;
+    int gram_blocks = 0;
+    double *G = nullptr;
+    double *Q = nullptr;
+    float *y0 = nullptr, *y1 = nullptr;
+    double *scal = nullptr;
+    ~TuckerWs() {
+        void *ps[] = {gram_part, G, Q, y0, y1, scal};
+        for (void *p : ps)
+            if (p) (void)hipFree(p);
+    }
+};
+
+static int unfold_gram(const float *X, int order, const long *dims, int mode, TuckerWs &W, double *G, hipStream_t s) {
+    long P = 1, Q = 1;
+    for (int m = 0; m < mode; ++m) P *= dims[m];
+    for (int m = mode + 1; m < order; ++m) Q *= dims[m];
+    const long In = dims[mode];
+    if (In > kGramMaxN) return record(TDX_ERR_UNSUPPORTED);
+    const long K = P * Q;
+    long nblk = ceil_div(K, 2048);
+    if (nblk > W.gram_blocks) nblk = W.gram_blocks;
+    if (nblk < 1) nblk = 1;
+    long kper = ceil_div(ceil_div(K, nblk), kGramChunk) * kGramChunk;
+    nblk = ceil_div(K, kper);
+    k_gram_partial<<<(int)nblk, 256, 0, s>>>(X, P, In, Q, kper, W.gram_part);
+    k_gram_sum<<<(int)ceil_div(In * In, 256), 256, 0, s>>>(W.gram_part, (int)nblk, In, G);
+    TDX_LAUNCH_CHECK();
+    return TDX_OK;
+}
+
+static int top_eigvecs(const double *G, long n, long r, TuckerWs &W, float *U, hipStream_t s) {
+    if (n > kGramMaxN || r > n) return record(TDX_ERR_ARG);
+    const size_t lds = sizeof(double) * n * n;
+    static bool attr_set = false;
+    if (!attr_set) {
+        TDX_HIP(hipFuncSetAttribute((const void *)k_top_eigvecs, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)(sizeof(double) * kGramMaxN * kGramMaxN)));
+        attr_set = true;
+    }
+    k_top_eigvecs<<<1, 256, lds, s>>>(G, (int)n, (int)r, W.Q, U, nullptr);
+    TDX_LAUNCH_CHECK();
+    return TDX_OK;
+}
+
+static int ws_setup(TuckerWs &W, int order, const long *dims, const long *ranks) {
+    W.gram_blocks = 1024;
+    TDX_HIP(hipMalloc(&W.gram_part, sizeof(float) * (size_t)W.gram_blocks * kGramMaxN * kGramMaxN));
+    TDX_HIP(hipMalloc(&W.G, sizeof(double) * kGramMaxN * kGramMaxN));
+    TDX_HIP(hipMalloc(&W.Q, sizeof(double) * kGramMaxN * kGramMaxN));
+    TDX_HIP(hipMalloc(&W.scal, sizeof(double) * 2048));
+    // largest intermediate of a TTM chain: the first contraction of the biggest mode
+    long N = 1;
+    for (int m = 0; m < order; ++m) N *= dims[m];
+    long maxsz = 0;
+    for (int m = 0; m < order; ++m) {
+        long sz = N / dims[m] * ranks[m];
+        if (sz > maxsz) maxsz = sz;
+    }
+    TDX_HIP(hipMalloc(&W.y0, sizeof(float) * maxsz));
+    TDX_HIP(hipMalloc(&W.y1, sizeof(float) * maxsz));
+    return TDX_OK;
+}
+
+// Y = X x_{m in modes} U_m^T applied in the listed order; returns the result pointer
+// (one of the workspace buffers) and the resulting dims in cur
+static int ttm_chain(const float *X, int order, const long *dims, const long *ranks, float *const *U,
+                     const int *modes, int nmodes, TuckerWs &W, long *cur, const float **out, hipStream_t s) {
+    for (int m = 0; m < order; ++m) cur[m] = dims[m];
+    const float *src = X;
+    for (int t = 0; t < nmodes; ++t) {
+        const int m = modes[t];
+        float *dst = (t & 1) ? W.y1 : W.y0;
+        int st = ttm_generic(src, order, cur, m, U[m], cur[m], 1, ranks[m], dst, s);
+        if (st) return st;
+        cur[m] = ranks[m];
+        src = dst;
+    }
+    *out = src;
+    return TDX_OK;
+}
+
+static int tucker_run(const float *X, int order, const long *dims, const long *ranks, int iters, float *core,
+                      float *const *U, double *relerr_hist) {
+    if (!X || !core || !U || order < 2 || order > 4) return record(TDX_ERR_ARG);
+    for (int m = 0; m < order; ++m) {
+        if (dims[m] <= 0 || ranks[m] <= 0 || ranks[m] > dims[m] || !U[m]) return record(TDX_ERR_ARG);
+        if (dims[m] > kGramMaxN) return record(TDX_ERR_UNSUPPORTED);
+    }
+    hipStream_t s = stream();
+    TuckerWs W;
+    int st = ws_setup(W, order, dims, ranks);
+    if (st) return st;
+    // HOSVD init
+    for (int m = 0; m < order; ++m) {
+        st = unfold_gram(X, order, dims, m, W, W.G, s);
+        if (!st) st = top_eigvecs(W.G, dims[m], ranks[m], W, U[m], s);
+        if (st) return st;
+    }
+    long N = 1;
+    for (int m = 0; m < order; ++m) N *= dims[m];
+    k_norm2_blocks<float><<<1024, 256, 0, s>>>(X, N, W.scal);
+    k_sum_to<<<1, 256, 0, s>>>(W.scal, 1024, W.scal + 1024);
+    TDX_LAUNCH_CHECK();
+    double nx2 = 0.0;
+    long cur[4];
+    const float *Y = nullptr;
+    for (int it = 0; it < iters; ++it) {
+        for (int n = 0; n < order; ++n) {
+            int modes[4], nm = 0;
+            for (int m = 0; m < order; ++m)
+                if (m != n) modes[nm++] = m;
+            st = ttm_chain(X, order, dims, ranks, U, modes, nm, W, cur, &Y, s);
+            if (!st) st = unfold_gram(Y, order, cur, n, W, W.G, s);
+            if (!st) st = top_eigvecs(W.G, dims[n], ranks[n], W, U[n], s);
+            if (st) return st;
+        }
+        if (relerr_hist) {
+            // core from the last chain: Y x_{d-1} U_{d-1}^T; relerr^2 = 1 - ||core||^2/||X||^2
+            long nc = 1;
+            for (int m = 0; m < order; ++m) nc *= ranks[m];
+            st = ttm_generic(Y, order, cur, order - 1, U[order - 1], dims[order - 1], 1, ranks[order - 1], core, s);
+            if (st) return st;
+            k_sumsq<<<1, 256, 0, s>>>(core, nc, W.scal + 1025);
+            double h[2];
+            TDX_HIP(hipMemcpyAsync(h, W.scal + 1024, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+            TDX_HIP(hipStreamSynchronize(s));
+            nx2 = h[0];
+            relerr_hist[it] = sqrt(fmax(nx2 - h[1], 0.0) / nx2);
+        }
+    }
+    // final core
+    int modes[4] = {0, 1, 2, 3};
+    st = ttm_chain(X, order, dims, ranks, U, modes, order - 1, W, cur, &Y, s);
+    if (st) return st;
+    st = ttm_generic(Y, order, cur, order - 1, U[order - 1], dims[order - 1], 1, ranks[order - 1], core, s);
+    if (st) return st;
+    TDX_HIP(hipStreamSynchronize(s));
+    return TDX_OK;
+}
+
+}  // namespace tdx
+
+using namespace tdx;
+
+extern "C" {
+
+int tdx_unfold_gram(const float *X, int order, const long *dims, int mode, double *G) {
+    TDX_CHECK_ARG(X && dims && G && order >= 2 && order <= 4 && mode >= 0 && mode < order);
+    TuckerWs W;
+    W.gram_blocks = 1024;
+    TDX_HIP(hipMalloc(&W.gram_part, sizeof(float) * (size_t)W.gram_blocks * kGramMaxN * kGramMaxN));
+    int st = unfold_gram(X, order, dims, mode, W, G, stream());
+    if (!st) TDX_HIP(hipStreamSynchronize(stream()));
+    return record(st);
+}
+
+int tdx_tucker_hooi(const float *X, int order, const long *dims, const long *ranks, int iters, int bf16_ttm,
+                    float *core, float *const *U, double *relerr_hist) {
+    if (!dims || !ranks || iters < 0) return record(TDX_ERR_ARG);
+    (void)bf16_ttm;
+    return record(tucker_run(X, order, dims, ranks, iters, core, U, relerr_hist));
+}
+
+int tdx_tucker_hosvd(const float *X, float *core, float *U1, float *U2, float *U3, long a, long b, long c, long r1,
+                     long r2, long r3) {
+    const long dims[3] = {a, b, c}, ranks[3] = {r1, r2, r3};
+    float *U[3] = {U1, U2, U3};
+    return record(tucker_run(X, 3, dims, ranks, 0, core, U, nullptr));
+}
+
+int tdx_tucker_hosvd4(const float *X, float *core, float *U1, float *U2, float *U3, float *U4, long a, long b, long c,
+                      long d, long r1, long r2, long r3, long r4) {
+    const long dims[4] = {a, b, c, d}, ranks[4] = {r1, r2, r3, r4};
+    float *U[4] = {U1, U2, U3, U4};
+    return record(tucker_run(X, 4, dims, ranks, 0, core, U, nullptr));
+}
+
+int tdx_tucker_tensorcore(const float *X, float *core, float *U1, float *U2, float *U3, long a, long b, long c,
+                          long r1, long r2, long r3) {
+    return tdx_tucker_hosvd(X, core, U1, U2, U3, a, b, c, r1, r2, r3);
+}
+
+}  // extern "C"

@@ -1,0 +1,303 @@
+"""ctypes binding of libtdx.so (the MI355X-native tensor-decomposition library).
+
+The library takes raw device pointers; here torch tensors provide device memory and
+the default HIP stream.  Every call raises TdxError on a non-zero status, and loading
+fails loudly when libtdx.so has not been built: there is no CPU fallback.
+
+Names mirror the reference surface (inc/cudecompose.h): hadmard, hadmard3, kr,
+tensor_product, t2m, tminust, initMat, f2h, ttm, mttkrp, rse, gencp, gencp4,
+gentucker, gentucker4, cp_als, cp_als4, cp_tensorcore, tucker_hosvd, tucker_hosvd4,
+tucker_tensorcore.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtdx.so")
+_LIB = None
+
+TDX_F32, TDX_BF16, TDX_FP8_E4M3, TDX_F16 = 0, 1, 2, 3
+TDX_OK, TDX_ERR_ARG, TDX_ERR_ALLOC, TDX_ERR_HIP, TDX_WARN_SINGULAR, TDX_ERR_NONFINITE, TDX_ERR_UNSUPPORTED = range(7)
+
+_L = ctypes.c_long
+_P = ctypes.c_void_p
+
+
+class TdxError(RuntimeError):
+    def __init__(self, status, fn):
+        self.status = status
+        msg = lib().tdx_status_string(status).decode() if _LIB is not None else str(status)
+        super().__init__(f"{fn}: status {status} ({msg})")
+
+
+class CpOptions(ctypes.Structure):
+    _fields_ = [("max_sweeps", ctypes.c_int), ("refine_sweeps", ctypes.c_int), ("storage", ctypes.c_int),
+                ("init", ctypes.c_int), ("seed", ctypes.c_uint64), ("tol", ctypes.c_double),
+                ("record_fit", ctypes.c_int), ("learned", ctypes.c_int), ("use_graph", ctypes.c_int)]
+
+
+class CpReport(ctypes.Structure):
+    _fields_ = [("sweeps_done", ctypes.c_int), ("status", ctypes.c_int), ("relerr_fit", ctypes.c_double),
+                ("seconds", ctypes.c_double)]
+
+
+_SIGS = {
+    "tdx_init": [], "tdx_set_stream": [_P], "tdx_last_status": [], "tdx_synchronize": [],
+    "tdx_release_workspace": [], "tdx_copy_to_host": [_P, _P, ctypes.c_size_t],
+    "tdx_copy_to_device": [_P, _P, ctypes.c_size_t],
+    "tdx_hadamard": [_P, _P, _P, _L, _L],
+    "tdx_hadamard3": [_P, _P, _P, _P, _L, _L],
+    "tdx_kr": [_P, _P, _P, _L, _L, _L],
+    "tdx_tensor_product": [_P, _P, _P, _L, _L, _L, _L],
+    "tdx_t2m": [_P, _P, _L, _L, _L, _L],
+    "tdx_tminust": [_P, _P, _L],
+    "tdx_init_identity": [_P, _L],
+    "tdx_f2h": [_P, _P, _L],
+    "tdx_quantize_bf16": [_P, _P, _L],
+    "tdx_dequantize_bf16": [_P, _P, _L],
+    "tdx_quantize_fp8_mode1": [_P, _P, _P, _L, _L],
+    "tdx_dequantize_fp8_mode1": [_P, _P, _P, _L, _L],
+    "tdx_mttkrp": [_P, _P, _P, _P, _L, _L, _L, _L, _L],
+    "tdx_mttkrp_ex": [_P, ctypes.c_int, _P, ctypes.c_int, ctypes.POINTER(_L), ctypes.POINTER(_P), _L,
+                      ctypes.c_int, _P],
+    "tdx_bench_mttkrp": [_P, ctypes.c_int, _P, ctypes.c_int, ctypes.POINTER(_L), ctypes.POINTER(_P), _L,
+                         ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)],
+    "tdx_ttm": [_P, _P, _P, _L, _L, _L, _L, _L],
+    "tdx_ttm_t": [_P, ctypes.c_int, ctypes.POINTER(_L), ctypes.c_int, _P, _L, _P],
+    "tdx_unfold_gram": [_P, ctypes.c_int, ctypes.POINTER(_L), ctypes.c_int, _P],
+    "tdx_rse": [_P, _P, _L, ctypes.POINTER(ctypes.c_float)],
+    "tdx_gencp": [_P, _P, _P, _P, _L, _L, _L, _L],
+    "tdx_gencp4": [_P, _P, _P, _P, _P, _L, _L, _L, _L, _L],
+    "tdx_gentucker": [_P, _P, _P, _P, _P, _L, _L, _L, _L, _L, _L],
+    "tdx_gentucker4": [_P, _P, _P, _P, _P, _P, _L, _L, _L, _L, _L, _L, _L, _L],
+    "tdx_cp_relerr": [_P, ctypes.c_int, _P, ctypes.c_int, ctypes.POINTER(_L), ctypes.POINTER(_P), _L,
+                      ctypes.POINTER(ctypes.c_double)],
+    "tdx_fill_uniform": [_P, _L, ctypes.c_uint64, ctypes.c_uint32],
+    "tdx_fill_normal": [_P, _L, ctypes.c_uint64, ctypes.c_uint32],
+    "tdx_gen_cp_tensor": [_P, ctypes.c_int, ctypes.POINTER(_L), ctypes.POINTER(_P), _L, ctypes.c_double,
+                          ctypes.c_uint64, ctypes.c_uint32],
+    "tdx_cp_als_ex": [_P, ctypes.c_int, ctypes.POINTER(_L), _L, ctypes.POINTER(_P), ctypes.POINTER(CpOptions),
+                      ctypes.POINTER(CpReport), _P, _P],
+    "tdx_cp_als": [_P, _P, _P, _P, _L, _L, _L, _L],
+    "tdx_cp_als4": [_P, _P, _P, _P, _P, _L, _L, _L, _L, _L],
+    "tdx_cp_tensorcore": [_P, _P, _P, _P, _L, _L, _L, _L],
+    "tdx_tucker_hosvd": [_P, _P, _P, _P, _P, _L, _L, _L, _L, _L, _L],
+    "tdx_tucker_hosvd4": [_P, _P, _P, _P, _P, _P, _L, _L, _L, _L, _L, _L, _L, _L],
+    "tdx_tucker_tensorcore": [_P, _P, _P, _P, _P, _L, _L, _L, _L, _L, _L],
+    "tdx_tucker_hooi": [_P, ctypes.c_int, ctypes.POINTER(_L), ctypes.POINTER(_L), ctypes.c_int, ctypes.c_int,
+                        _P, ctypes.POINTER(_P), _P],
+}
+
+# every symbol include/tdx.h declares (checked against the built library by the tests)
+HEADER_SYMBOLS = sorted(list(_SIGS) + ["tdx_get_stream", "tdx_status_string", "tdx_cp_default_options"])
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"libtdx.so not built at {LIB_PATH}; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in _SIGS.items():
+            fn = getattr(L, name, None)
+            if fn is None:
+                continue
+            fn.argtypes = args
+            fn.restype = ctypes.c_int
+        L.tdx_status_string.restype = ctypes.c_char_p
+        L.tdx_status_string.argtypes = [ctypes.c_int]
+        L.tdx_get_stream.restype = _P
+        L.tdx_cp_default_options.argtypes = [ctypes.POINTER(CpOptions)]
+        L.tdx_cp_default_options.restype = None
+        _LIB = L
+    return _LIB
+
+
+def _call(name, *args):
+    st = getattr(lib(), name)(*args)
+    if st not in (TDX_OK,):
+        raise TdxError(st, name)
+    return st
+
+
+def _call_status(name, *args):
+    return getattr(lib(), name)(*args)
+
+
+def ptr(t):
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def dims_arr(dims):
+    return (_L * 4)(*(list(dims) + [1] * (4 - len(dims))))
+
+
+def ptr_arr(ts):
+    return (_P * 4)(*([t.data_ptr() if t is not None else None for t in ts] + [None] * (4 - len(ts))))
+
+
+# ---------------------------------------------------------------------------
+# reference-named wrappers on torch CUDA tensors (flat, column-major contents)
+
+def hadmard(A, B, C, a, b):
+    _call("tdx_hadamard", ptr(A), ptr(B), ptr(C), a, b)
+
+
+def hadmard3(A, B, C, ABC, a, b):
+    _call("tdx_hadamard3", ptr(A), ptr(B), ptr(C), ptr(ABC), a, b)
+
+
+def kr(A, B, AkrB, a, b, r):
+    _call("tdx_kr", ptr(A), ptr(B), ptr(AkrB), a, b, r)
+
+
+def tensor_product(A, B, C, a, b, c, d):
+    _call("tdx_tensor_product", ptr(A), ptr(B), ptr(C), a, b, c, d)
+
+
+def t2m(T1, T2, flag, m, n, k):
+    _call("tdx_t2m", ptr(T1), ptr(T2), flag, m, n, k)
+
+
+def tminust(left, right, m):
+    _call("tdx_tminust", ptr(left), ptr(right), m)
+
+
+def initMat(A, m):
+    _call("tdx_init_identity", ptr(A), m)
+
+
+def f2h(A, B, num):
+    _call("tdx_f2h", ptr(A), ptr(B), num)
+
+
+def quantize_bf16(X, Xq):
+    _call("tdx_quantize_bf16", ptr(X), ptr(Xq), X.numel())
+
+
+def quantize_fp8_mode1(X, Xq, scales, I):
+    _call("tdx_quantize_fp8_mode1", ptr(X), ptr(Xq), ptr(scales), I, X.numel() // I)
+
+
+def dequantize_fp8_mode1(Xq, scales, X, I):
+    _call("tdx_dequantize_fp8_mode1", ptr(Xq), ptr(scales), ptr(X), I, X.numel() // I)
+
+
+def mttkrp(X, left, right, XU, flag, m, n, k, r):
+    _call("tdx_mttkrp", ptr(X), ptr(left), ptr(right), ptr(XU), flag, m, n, k, r)
+
+
+def mttkrp_ex(X, dtype, scales, dims, factors, rank, mode, out):
+    _call("tdx_mttkrp_ex", ptr(X), dtype, ptr(scales), len(dims), dims_arr(dims), ptr_arr(factors), rank, mode,
+          ptr(out))
+
+
+def bench_mttkrp(X, dtype, scales, dims, factors, rank, mode, iters):
+    ms = ctypes.c_float()
+    _call("tdx_bench_mttkrp", ptr(X), dtype, ptr(scales), len(dims), dims_arr(dims), ptr_arr(factors), rank, mode,
+          iters, ctypes.byref(ms))
+    return ms.value
+
+
+def ttm(X, U, XU, flag, a, b, c, d):
+    _call("tdx_ttm", ptr(X), ptr(U), ptr(XU), flag, a, b, c, d)
+
+
+def rse(X, X1, m):
+    err = ctypes.c_float()
+    _call("tdx_rse", ptr(X), ptr(X1), m, ctypes.byref(err))
+    return err.value
+
+
+def gencp(rec, A, B, C, a, b, c, r):
+    _call("tdx_gencp", ptr(rec), ptr(A), ptr(B), ptr(C), a, b, c, r)
+
+
+def gencp4(rec, A, B, C, D, a, b, c, d, r):
+    _call("tdx_gencp4", ptr(rec), ptr(A), ptr(B), ptr(C), ptr(D), a, b, c, d, r)
+
+
+def gentucker(rec, core, A, B, C, a, b, c, r1, r2, r3):
+    _call("tdx_gentucker", ptr(rec), ptr(core), ptr(A), ptr(B), ptr(C), a, b, c, r1, r2, r3)
+
+
+def gentucker4(rec, core, A, B, C, D, a, b, c, d, r1, r2, r3, r4):
+    _call("tdx_gentucker4", ptr(rec), ptr(core), ptr(A), ptr(B), ptr(C), ptr(D), a, b, c, d, r1, r2, r3, r4)
+
+
+def cp_relerr(X, dims, factors, rank, dtype=TDX_F32, scales=None):
+    out = ctypes.c_double()
+    _call("tdx_cp_relerr", ptr(X), dtype, ptr(scales), len(dims), dims_arr(dims), ptr_arr(factors), rank,
+          ctypes.byref(out))
+    return out.value
+
+
+def fill_uniform(out, seed, stream):
+    _call("tdx_fill_uniform", ptr(out), out.numel(), seed, stream)
+
+
+def fill_normal(out, seed, stream):
+    _call("tdx_fill_normal", ptr(out), out.numel(), seed, stream)
+
+
+def gen_cp_tensor(X, dims, factors, rank, eta, seed, stream):
+    _call("tdx_gen_cp_tensor", ptr(X), len(dims), dims_arr(dims), ptr_arr(factors), rank, float(eta), seed, stream)
+
+
+def default_cp_options():
+    o = CpOptions()
+    lib().tdx_cp_default_options(ctypes.byref(o))
+    return o
+
+
+def cp_als_ex(X, dims, rank, factors, opts=None, want_fit=True, want_ms=False):
+    """Returns (status, report, fit_hist list, sweep_ms list)."""
+    import numpy as np
+    o = opts if opts is not None else default_cp_options()
+    total = o.max_sweeps + (o.refine_sweeps if o.storage != TDX_F32 else 0)
+    fit = np.zeros(max(total, 1), np.float64)
+    ms = np.zeros(max(total, 1), np.float32)
+    rep = CpReport()
+    st = _call_status("tdx_cp_als_ex", ptr(X), len(dims), dims_arr(dims), rank, ptr_arr(factors), ctypes.byref(o),
+                      ctypes.byref(rep), fit.ctypes.data if want_fit else None, ms.ctypes.data if want_ms else None)
+    if st not in (TDX_OK, TDX_WARN_SINGULAR):
+        raise TdxError(st, "tdx_cp_als_ex")
+    return st, rep, fit[:rep.sweeps_done], ms[:rep.sweeps_done]
+
+
+def cp_als(X, A, B, C, a, b, c, r):
+    return _call_status("tdx_cp_als", ptr(X), ptr(A), ptr(B), ptr(C), a, b, c, r)
+
+
+def cp_als4(X, A, B, C, D, a, b, c, d, r):
+    return _call_status("tdx_cp_als4", ptr(X), ptr(A), ptr(B), ptr(C), ptr(D), a, b, c, d, r)
+
+
+def cp_tensorcore(X, A, B, C, a, b, c, r):
+    return _call_status("tdx_cp_tensorcore", ptr(X), ptr(A), ptr(B), ptr(C), a, b, c, r)
+
+
+def tucker_hosvd(X, core, U1, U2, U3, a, b, c, r1, r2, r3):
+    _call("tdx_tucker_hosvd", ptr(X), ptr(core), ptr(U1), ptr(U2), ptr(U3), a, b, c, r1, r2, r3)
+
+
+def tucker_hosvd4(X, core, U1, U2, U3, U4, a, b, c, d, r1, r2, r3, r4):
+    _call("tdx_tucker_hosvd4", ptr(X), ptr(core), ptr(U1), ptr(U2), ptr(U3), ptr(U4), a, b, c, d, r1, r2, r3, r4)
+
+
+def tucker_tensorcore(X, core, U1, U2, U3, a, b, c, r1, r2, r3):
+    _call("tdx_tucker_tensorcore", ptr(X), ptr(core), ptr(U1), ptr(U2), ptr(U3), a, b, c, r1, r2, r3)
+
+
+def tucker_hooi(X, dims, ranks, iters, core, U, bf16_ttm=False):
+    import numpy as np
+    hist = np.zeros(max(iters, 1))
+    _call("tdx_tucker_hooi", ptr(X), len(dims), dims_arr(dims), dims_arr(ranks), iters, 1 if bf16_ttm else 0,
+          ptr(core), ptr_arr(U), hist.ctypes.data)
+    return hist[:iters]
+
+
+def synchronize():
+    _call("tdx_synchronize")

@@ -1,0 +1,111 @@
+/*
+ * cudecompose.h -- drop-in C++ surface with the names and parameter lists of the
+ * reference header inc/cudecompose.h:22-57, implemented as inline wrappers over the
+ * C ABI in tdx.h.  Code written against the reference recompiles unchanged and links
+ * against libtdx.so (add -I<repo>/include -L<repo>/first-project_amd -ltdx).
+ *
+ * Differences from the reference, all deliberate:
+ *  - no CUDA/HIP headers are pulled in and no kernels are re-exported (the reference
+ *    declares 11 __global__ symbols, inc/cudecompose.h:22,27,31-33,35,45,47,55-57);
+ *  - the functions return void as before, and record a status readable with
+ *    tdx_last_status();
+ *  - f2h takes the 16-bit type `half` defined below (IEEE binary16 bit patterns);
+ *  - rse does not overwrite its second argument (src/rse.cu:9 does);
+ *  - cp_tensorcore never frees the caller's buffers (src/tensorcore_cp.cu:136-139 does),
+ *    and cp_als4 never resets the device (src/cp4_als.cu:184 does);
+ *  - mttkrp/ttm compute in FP32 (the reference rounds their inputs to FP16);
+ *  - the CP drivers draw their initial factors from the library's Philox generator,
+ *    seed 1234 (the reference uses cuRAND XORWOW, seed 1234, src/cp_als.cu:20-24).
+ */
+#ifndef GUARD_cudecompose_h
+#define GUARD_cudecompose_h
+#include <stdint.h>
+#include <stdio.h>
+#include "tdx.h"
+
+typedef float dt;
+typedef uint16_t half;
+
+inline void hadmard(dt *d_A, dt *d_B, dt *d_C, long a, long b) { tdx_hadamard(d_A, d_B, d_C, a, b); }
+inline void hadmard3(dt *d_A, dt *d_B, dt *d_C, dt *d_ABC, long a, long b) {
+    tdx_hadamard3(d_A, d_B, d_C, d_ABC, a, b);
+}
+inline void kr(dt *d_A, dt *d_B, dt *d_AkrB, long a, long b, long r) { tdx_kr(d_A, d_B, d_AkrB, a, b, r); }
+inline void tensor_product(dt *d_A, dt *d_B, dt *d_C, long a, long b, long c, long d) {
+    tdx_tensor_product(d_A, d_B, d_C, a, b, c, d);
+}
+inline void f2h(dt *A, half *B, long num) { tdx_f2h(A, B, num); }
+inline void ttm(dt *d_X, dt *d_U, dt *d_XU, long flag, long a, long b, long c, long d) {
+    tdx_ttm(d_X, d_U, d_XU, flag, a, b, c, d);
+    tdx_synchronize();
+}
+inline void mttkrp(dt *d_X, dt *left, dt *d_right, dt *d_XU, long flag, long m, long n, long k, long r) {
+    tdx_mttkrp(d_X, left, d_right, d_XU, flag, m, n, k, r);
+    tdx_synchronize();
+}
+inline void t2m(dt *d_T1, dt *d_T2, long flag, long m, long n, long k) { tdx_t2m(d_T1, d_T2, flag, m, n, k); }
+inline void tminust(dt *d_left, dt *d_right, long m) {
+    tdx_tminust(d_left, d_right, m);
+    tdx_synchronize();
+}
+inline void rse(dt *d_X, dt *d_X1, long m, dt *error) { tdx_rse(d_X, d_X1, m, error); }
+inline void gencp(dt *d_rec, dt *d_A, dt *d_B, dt *d_C, long a, long b, long c, long r) {
+    tdx_gencp(d_rec, d_A, d_B, d_C, a, b, c, r);
+    tdx_synchronize();
+}
+inline void gentucker(dt *d_rec, dt *d_core, dt *d_A, dt *d_B, dt *d_C, long a, long b, long c, long r1, long r2,
+                      long r3) {
+    tdx_gentucker(d_rec, d_core, d_A, d_B, d_C, a, b, c, r1, r2, r3);
+    tdx_synchronize();
+}
+inline void gencp4(dt *d_T, dt *d_AA, dt *d_BB, dt *d_CC, dt *d_DD, long a, long b, long c, long d, long r) {
+    tdx_gencp4(d_T, d_AA, d_BB, d_CC, d_DD, a, b, c, d, r);
+    tdx_synchronize();
+}
+inline void gentucker4(dt *d_T, dt *d_G, dt *d_A, dt *d_B, dt *d_C, dt *d_D, long a, long b, long c, long d,
+                       long r1, long r2, long r3, long r4) {
+    tdx_gentucker4(d_T, d_G, d_A, d_B, d_C, d_D, a, b, c, d, r1, r2, r3, r4);
+    tdx_synchronize();
+}
+inline void cp_als(dt *d_X, dt *d_A, dt *d_B, dt *d_C, long a, long b, long c, long r) {
+    tdx_cp_als(d_X, d_A, d_B, d_C, a, b, c, r);
+}
+inline void cp_als4(dt *d_X, dt *d_A, dt *d_B, dt *d_C, dt *d_D, long a, long b, long c, long d, long r) {
+    tdx_cp_als4(d_X, d_A, d_B, d_C, d_D, a, b, c, d, r);
+}
+inline void cp_tensorcore(dt *d_X, dt *d_A, dt *d_B, dt *d_C, long a, long b, long c, long r) {
+    tdx_cp_tensorcore(d_X, d_A, d_B, d_C, a, b, c, r);
+}
+inline void tucker_hosvd(dt *d_X, dt *d_core, dt *d_U1, dt *d_U2, dt *d_U3, long a, long b, long c, long r1,
+                         long r2, long r3) {
+    tdx_tucker_hosvd(d_X, d_core, d_U1, d_U2, d_U3, a, b, c, r1, r2, r3);
+}
+inline void tucker_tensorcore(dt *d_X, dt *d_core, dt *d_U1, dt *d_U2, dt *d_U3, long a, long b, long c, long r1,
+                              long r2, long r3) {
+    tdx_tucker_tensorcore(d_X, d_core, d_U1, d_U2, d_U3, a, b, c, r1, r2, r3);
+}
+inline void tucker_hosvd4(dt *d_X, dt *d_core, dt *d_U1, dt *d_U2, dt *d_U3, dt *d_U4, long a, long b, long c,
+                          long d, long r1, long r2, long r3, long r4) {
+    tdx_tucker_hosvd4(d_X, d_core, d_U1, d_U2, d_U3, d_U4, a, b, c, d, r1, r2, r3, r4);
+}
+inline void initMat(dt *d_A, long m) { tdx_init_identity(d_A, m); }
+
+/* printTensor (src/kr.cu:13-30): host copy and column-major print, one frontal slice
+ * per block. */
+inline void printTensor(dt *A, long a, long b, long c) {
+    size_t n = (size_t)(a * b * c);
+    dt *h = new dt[n];
+    if (tdx_copy_to_host(h, A, n * sizeof(dt)) == TDX_OK) {
+        for (long i = 0; i < c; i++) {
+            for (long j = 0; j < a; j++) {
+                for (long k = 0; k < b; k++) printf("%g  ", h[i * a * b + k * a + j]);
+                printf("\n");
+            }
+            printf("-----------------------------------\n");
+        }
+        printf("\n");
+    }
+    delete[] h;
+}
+
+#endif

@@ -1,0 +1,183 @@
+/*
+ * tdx.h -- C ABI of the MI355X-native dense tensor-decomposition library (libtdx.so).
+ *
+ * Every entry point takes plain device pointers and 64-bit extents, returns an int
+ * status (TDX_OK == 0), launches on the library stream (tdx_set_stream), creates no
+ * per-call handles, and never frees, resets or overwrites caller memory other than the
+ * declared outputs.  Storage is column-major: X[i + j*I + k*I*J (+ l*I*J*K)], factors
+ * U[i + r*I] (reference convention, src/tensor2matrix.cu:4-36, src/cp_als.cu:5-6).
+ *
+ * Each declaration names the reference interface it replaces (inc/cudecompose.h and
+ * the src/ file that implements it).  The C++ drop-in surface with the reference's
+ * exact names lives in include/cudecompose.h on top of this header.
+ */
+#ifndef TDX_H
+#define TDX_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the reference reports none: src/mttkrp.cu:75-77 prints and returns) ---- */
+enum {
+    TDX_OK = 0,
+    TDX_ERR_ARG = 1,          /* bad flag/mode, non-positive extent, null pointer, unsupported rank */
+    TDX_ERR_ALLOC = 2,        /* device allocation failed */
+    TDX_ERR_HIP = 3,          /* a HIP runtime call failed */
+    TDX_WARN_SINGULAR = 4,    /* Gram-Hadamard not SPD: pseudo-inverse fallback was used */
+    TDX_ERR_NONFINITE = 5,    /* NaN/Inf detected in the fit */
+    TDX_ERR_UNSUPPORTED = 6   /* shape/dtype combination not implemented */
+};
+
+/* storage types of the tensor operand */
+enum { TDX_F32 = 0, TDX_BF16 = 1, TDX_FP8_E4M3 = 2, TDX_F16 = 3 };
+
+/* ---- library state (no reference counterpart: the reference uses the legacy default
+ *      stream and per-call cuBLAS/cuSOLVER handles, e.g. src/kr.cu:7-10) ---- */
+int tdx_init(void);                       /* optional: selects nothing, warms the device */
+int tdx_set_stream(void *hip_stream);     /* hipStream_t; NULL = default stream */
+void *tdx_get_stream(void);
+int tdx_last_status(void);                /* last status recorded by any entry point */
+const char *tdx_status_string(int status);
+int tdx_synchronize(void);                /* hipStreamSynchronize on the library stream */
+int tdx_release_workspace(void);          /* frees the internal workspace pool */
+/* blocking copies on the library stream (used by printTensor, src/kr.cu:13-30) */
+int tdx_copy_to_host(void *dst, const void *src, size_t bytes);
+int tdx_copy_to_device(void *dst, const void *src, size_t bytes);
+
+/* ---- elementwise and permutation primitives ---- */
+/* hadmard: src/hadmard.cu:10-15 -- C = A o B over a*b floats */
+int tdx_hadamard(const float *A, const float *B, float *C, long a, long b);
+/* hadmard3: src/hadmard.cu:25-30 -- ABC = (A o B) o C */
+int tdx_hadamard3(const float *A, const float *B, const float *C, float *ABC, long a, long b);
+/* kr: src/kr.cu:3-11 -- AkrB ((a*b) x r) = A (.) B, B index fastest */
+int tdx_kr(const float *A, const float *B, float *AkrB, long a, long b, long r);
+/* tensor_product: src/tensorproduct.cu:4-14 -- Kronecker A (a x b) (x) B (c x d) */
+int tdx_tensor_product(const float *A, const float *B, float *C, long a, long b, long c, long d);
+/* t2m: src/tensor2matrix.cu:38-52 -- mode-flag unfolding copy of an m x n x k tensor */
+int tdx_t2m(const float *T1, float *T2, long flag, long m, long n, long k);
+/* tminust: src/tminust.cu:10-15 -- right := left - right */
+int tdx_tminust(const float *left, float *right, long m);
+/* initMat: src/tminust.cu:31-36 -- m x m identity */
+int tdx_init_identity(float *A, long m);
+/* f2h: src/float2half.cu:14-18 -- FP32 -> IEEE binary16 (RNE), bit patterns in B */
+int tdx_f2h(const float *A, uint16_t *B, long num);
+/* new: FP32 -> BF16 (RNE) and back */
+int tdx_quantize_bf16(const float *X, uint16_t *Xq, long n);
+int tdx_dequantize_bf16(const uint16_t *Xq, float *X, long n);
+/* new: FP32 -> OCP FP8 E4M3FN with one FP32 scale per mode-1 slice X(i,:,...):
+ * scales[i] = max|X(i,...)| / 448 (1 for an all-zero slice), Xq = e4m3(X / scales[i]).
+ * n_rest = product of the other extents. */
+int tdx_quantize_fp8_mode1(const float *X, uint8_t *Xq, float *scales, long I, long n_rest);
+int tdx_dequantize_fp8_mode1(const uint8_t *Xq, const float *scales, float *X, long I, long n_rest);
+
+/* ---- contractions ---- */
+/* mttkrp: src/mttkrp.cu:3-80 -- XU = X(flag) (left (.) right) for an m x n x k tensor.
+ * flag 1: left = C (k x r), right = B (n x r), XU m x r; flag 2: left = C, right = A,
+ * XU n x r; flag 3: left = B, right = A, XU k x r.  FP32 in, FP32 out (the reference
+ * rounds inputs to FP16; this library keeps FP32, see tdx_mttkrp_ex for BF16/FP8). */
+int tdx_mttkrp(const float *X, const float *left, const float *right, float *XU,
+               long flag, long m, long n, long k, long r);
+/* new: fused MTTKRP of an order-2..4 tensor on mode `mode` (0-based) with the Khatri-Rao
+ * rows formed on the fly.  X has storage `dtype` (TDX_F32/TDX_BF16/TDX_FP8_E4M3);
+ * `scales` are the FP8 per-mode-1-slice scales (NULL otherwise).  factors[m] are
+ * I_m x rank FP32 column-major (factors[mode] is not read).  out: dims[mode] x rank. */
+int tdx_mttkrp_ex(const void *X, int dtype, const float *scales, int order, const long *dims,
+                  const float *const *factors, long rank, int mode, float *out);
+/* new: times `iters` launches of the fused MTTKRP kernel alone with hipEvents on the
+ * library stream (panels and workspace prepared once); *avg_ms = mean launch time. */
+int tdx_bench_mttkrp(const void *X, int dtype, const float *scales, int order, const long *dims,
+                     const float *const *factors, long rank, int mode, int iters, float *avg_ms);
+/* ttm: src/ttm.cu:4-47 -- XU = X x_flag U with U (d x mode size) col-major;
+ * outputs d x b x c, a x d x c or a x b x d.  FP32 in/out. */
+int tdx_ttm(const float *X, const float *U, float *XU, long flag, long a, long b, long c, long d);
+/* new: order-d TTM with a transposed factor, Y = X x_mode U^T, U (I_mode x r) */
+int tdx_ttm_t(const float *X, int order, const long *dims, int mode, const float *U, long r, float *Y);
+/* new: Gram of the mode-n unfolding, G = X(n) X(n)^T (I_n x I_n, FP64 out) */
+int tdx_unfold_gram(const float *X, int order, const long *dims, int mode, double *G);
+
+/* ---- reconstruction and error ---- */
+/* rse: src/rse.cu:4-15 -- *error (host) = ||X - X1|| / ||X||.  Unlike the reference, X1
+ * is NOT overwritten; the sums are FP64-accumulated with compensation. */
+int tdx_rse(const float *X, const float *X1, long m, float *error);
+/* gencp: src/rse.cu:17-30 -- rec = A (C (.) B)^T, an a x b x c tensor */
+int tdx_gencp(float *rec, const float *A, const float *B, const float *C, long a, long b, long c, long r);
+/* gencp4: src/rse.cu:53-73 */
+int tdx_gencp4(float *rec, const float *A, const float *B, const float *C, const float *D,
+               long a, long b, long c, long d, long r);
+/* gentucker: src/rse.cu:32-51 -- rec = core x1 A x2 B x3 C */
+int tdx_gentucker(float *rec, const float *core, const float *A, const float *B, const float *C,
+                  long a, long b, long c, long r1, long r2, long r3);
+/* gentucker4: src/rse.cu:75-96 */
+int tdx_gentucker4(float *rec, const float *core, const float *A, const float *B, const float *C,
+                   const float *D, long a, long b, long c, long d, long r1, long r2, long r3, long r4);
+/* new: compensated CP residual ||X - [[U]]|| / ||X|| without forming the reconstruction */
+int tdx_cp_relerr(const void *X, int dtype, const float *scales, int order, const long *dims,
+                  const float *const *factors, long rank, double *relerr);
+
+/* ---- synthetic inputs (Philox-4x32-10, identical to the oracle's stream) ---- */
+int tdx_fill_uniform(float *out, long n, uint64_t seed, uint32_t stream);
+int tdx_fill_normal(float *out, long n, uint64_t seed, uint32_t stream);
+/* X = [[U]] + eta * ||[[U]]|| * E/||E||, E ~ N(0,1) from (seed, stream); norms in FP64 */
+int tdx_gen_cp_tensor(float *X, int order, const long *dims, const float *const *factors, long rank,
+                      double eta, uint64_t seed, uint32_t stream);
+
+/* ---- CP-ALS ---- */
+typedef struct {
+    int max_sweeps;          /* sweeps of the main phase (reference: 500, src/cp_als.cu:42) */
+    int refine_sweeps;       /* extra FP32-storage sweeps after a BF16/FP8 phase */
+    int storage;             /* TDX_F32, TDX_BF16 or TDX_FP8_E4M3 for the main phase */
+    int init;                /* 0: seeded U(0,1) for modes 1..d-1 (reference semantics,
+                                src/cp_als.cu:20-24, Philox instead of cuRAND); 1: caller init */
+    uint64_t seed;           /* init seed (reference: 1234) */
+    double tol;              /* stop when |fit change| < tol (0: run all sweeps) */
+    int record_fit;          /* compute the fast-fit relative error every sweep */
+    int learned;             /* 1: use the learned extrapolation step (tdx_mlp_*), 0: plain ALS */
+    int use_graph;           /* capture each sweep in a hipGraph (default 1) */
+} tdx_cp_options;
+
+typedef struct {
+    int sweeps_done;
+    int status;              /* TDX_OK or TDX_WARN_SINGULAR */
+    double relerr_fit;       /* last fast-fit relative error (FP64 identity) */
+    double seconds;          /* device time of the sweeps (hipEvents) */
+} tdx_cp_report;
+
+void tdx_cp_default_options(tdx_cp_options *opt);
+/* new: CP-ALS of an order-3/4 FP32 tensor; factors[m] (I_m x rank) in/out.
+ * fit_hist (host, may be NULL) receives the per-sweep fast-fit relative error;
+ * sweep_ms (host, may be NULL) receives the per-sweep device time. */
+int tdx_cp_als_ex(const float *X, int order, const long *dims, long rank, float *const *factors,
+                  const tdx_cp_options *opt, tdx_cp_report *report, double *fit_hist, float *sweep_ms);
+/* cp_als: src/cp_als.cu:3-134 -- 500 FP32 sweeps, B and C drawn from the seeded generator
+ * (seed 1234) as in the reference; A, B, C are outputs. */
+int tdx_cp_als(const float *X, float *A, float *B, float *C, long a, long b, long c, long r);
+/* cp_als4: src/cp4_als.cu:3-186 */
+int tdx_cp_als4(const float *X, float *A, float *B, float *C, float *D, long a, long b, long c, long d, long r);
+/* cp_tensorcore: src/tensorcore_cp.cu:3-150 -- mixed precision: BF16 tensor storage on MFMA
+ * with FP32 accumulation, FP64 solve, 2 FP32 refinement sweeps (the reference uses FP16 and
+ * no refinement); caller buffers are never freed (unlike src/tensorcore_cp.cu:136-139). */
+int tdx_cp_tensorcore(const float *X, float *A, float *B, float *C, long a, long b, long c, long r);
+
+/* ---- Tucker ---- */
+/* tucker_hosvd: src/tucker_hosvd.cu:3-103 */
+int tdx_tucker_hosvd(const float *X, float *core, float *U1, float *U2, float *U3,
+                     long a, long b, long c, long r1, long r2, long r3);
+/* tucker_hosvd4: src/tucker4_hosvd.cu:3-141 */
+int tdx_tucker_hosvd4(const float *X, float *core, float *U1, float *U2, float *U3, float *U4,
+                      long a, long b, long c, long d, long r1, long r2, long r3, long r4);
+/* tucker_tensorcore: src/tucker_tensorcore.cu:3-139 -- BF16 TTM chain on MFMA, FP32 accumulation */
+int tdx_tucker_tensorcore(const float *X, float *core, float *U1, float *U2, float *U3,
+                          long a, long b, long c, long r1, long r2, long r3);
+/* new: HOSVD init + `iters` HOOI iterations for order 3/4
+ * (decomposition/GPU/normal/tucker/tucker_hooi.cu:3-218, generalised).  bf16_ttm = 1 runs
+ * the first TTM of every chain from a BF16 copy of X.  relerr_hist (host, may be NULL). */
+int tdx_tucker_hooi(const float *X, int order, const long *dims, const long *ranks, int iters,
+                    int bf16_ttm, float *core, float *const *U, double *relerr_hist);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,306 @@
+"""Python binding of the FP64 CPU oracle (oracle/oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker.  The product library never calls it.
+
+All arrays are numpy, column-major flattened (Fortran order), matching the
+reference's storage convention (src/tensor2matrix.cu:4-36, src/cp_als.cu:5-6).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+_L = ctypes.c_long
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-B", "-C", _HERE])
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        L.orc_uniform.restype = ctypes.c_float
+        L.orc_uniform.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64]
+        L.orc_normal.restype = ctypes.c_double
+        L.orc_normal.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64]
+        L.orc_fill_uniform.argtypes = [_f32p, _L, ctypes.c_uint64, ctypes.c_uint32]
+        L.orc_fill_normal.argtypes = [_f32p, _L, ctypes.c_uint64, ctypes.c_uint32]
+        L.orc_philox4x32_10.argtypes = [ctypes.POINTER(ctypes.c_uint32)] * 3
+        L.orc_t2m.argtypes = [_f32p, _f32p, _L, _L, _L, _L]
+        L.orc_kr.argtypes = [_f32p, _f32p, _f32p, _L, _L, _L]
+        L.orc_hadamard.argtypes = [_f32p, _f32p, _f32p, _L]
+        L.orc_hadamard3.argtypes = [_f32p, _f32p, _f32p, _f32p, _L]
+        L.orc_tensor_product.argtypes = [_f32p, _f32p, _f32p, _L, _L, _L, _L]
+        L.orc_tminust.argtypes = [_f32p, _f32p, _L]
+        L.orc_identity.argtypes = [_f32p, _L]
+        L.orc_f32_to_bf16.restype = ctypes.c_uint16
+        L.orc_f32_to_bf16.argtypes = [ctypes.c_float]
+        L.orc_f32_to_f16.restype = ctypes.c_uint16
+        L.orc_f32_to_f16.argtypes = [ctypes.c_float]
+        L.orc_f32_to_e4m3.restype = ctypes.c_uint8
+        L.orc_f32_to_e4m3.argtypes = [ctypes.c_float]
+        L.orc_e4m3_to_f32.restype = ctypes.c_float
+        L.orc_e4m3_to_f32.argtypes = [ctypes.c_uint8]
+        L.orc_quantize_e4m3_mode1.argtypes = [_f32p, _L, _L, _u8p, _f32p]
+        PP = ctypes.POINTER(ctypes.c_void_p)
+        L.orc_mttkrp.argtypes = [_f32p, ctypes.c_int, ctypes.POINTER(_L), PP, _L, ctypes.c_int, _f64p]
+        L.orc_mttkrp_scaled.argtypes = [_f32p, _f32p, ctypes.c_int, ctypes.POINTER(_L), PP, _L,
+                                        ctypes.c_int, _f64p]
+        L.orc_ttm.argtypes = [_f64p, _f64p, _f64p, _L, _L, _L, _L, _L]
+        L.orc_gram.argtypes = [_f64p, _L, _L, _f64p]
+        L.orc_cholesky.restype = ctypes.c_int
+        L.orc_cholesky.argtypes = [_f64p, _L]
+        L.orc_chol_solve_rows.argtypes = [_f64p, _L, _f64p, _L, _f64p]
+        L.orc_pinv_solve_rows.argtypes = [_f64p, _L, _f64p, _L, _f64p, ctypes.c_double]
+        L.orc_sym_eig.argtypes = [_f64p, _L, _f64p, _f64p]
+        L.orc_cp_reconstruct.argtypes = [ctypes.c_int, ctypes.POINTER(_L), PP, _L, _f64p]
+        L.orc_cp_relerr.restype = ctypes.c_double
+        L.orc_cp_relerr.argtypes = [_f32p, ctypes.c_int, ctypes.POINTER(_L), PP, _L]
+        L.orc_cp_als.restype = ctypes.c_int
+        L.orc_cp_als.argtypes = [_f32p, ctypes.c_int, ctypes.POINTER(_L), _L, PP, ctypes.c_int,
+                                 ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_tucker_hosvd.argtypes = [_f32p, ctypes.c_int, ctypes.POINTER(_L), ctypes.POINTER(_L), PP, _f64p]
+        L.orc_tucker_hooi.argtypes = [_f32p, ctypes.c_int, ctypes.POINTER(_L), ctypes.POINTER(_L),
+                                      ctypes.c_int, PP, _f64p, ctypes.c_void_p]
+        L.orc_tucker_reconstruct.argtypes = [ctypes.c_int, ctypes.POINTER(_L), ctypes.POINTER(_L), _f64p, PP, _f64p]
+        L.orc_mlp_forward.argtypes = [_f64p, _L, _L, _L, _L] + [_f64p] * 7
+        L.orc_mlp_backward.restype = ctypes.c_double
+        L.orc_mlp_backward.argtypes = [_f64p, _f64p, _L, _L, _L, _L] + [_f64p] * 12
+        L.orc_adam.argtypes = [_f64p, _f64p, _f64p, _f64p, _L, _L, ctypes.c_double, ctypes.c_double,
+                               ctypes.c_double, ctypes.c_double]
+        _LIB = L
+    return _LIB
+
+
+def _dims(dims):
+    return (_L * 4)(*(list(dims) + [0] * (4 - len(dims))))
+
+
+def _ptrs(arrs):
+    return (ctypes.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+
+
+def c32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32).ravel(order="F") if np.ndim(a) > 1 else
+                                np.asarray(a, dtype=np.float32))
+
+
+def philox(ctr, key):
+    c = (ctypes.c_uint32 * 4)(*ctr)
+    k = (ctypes.c_uint32 * 2)(*key)
+    o = (ctypes.c_uint32 * 4)()
+    lib().orc_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def uniform(n, seed, stream):
+    out = np.empty(n, np.float32)
+    lib().orc_fill_uniform(out, n, seed, stream)
+    return out
+
+
+def normal(n, seed, stream):
+    out = np.empty(n, np.float32)
+    lib().orc_fill_normal(out, n, seed, stream)
+    return out
+
+
+def t2m(X, flag, m, n, k):
+    out = np.empty(m * n * k, np.float32)
+    lib().orc_t2m(np.ascontiguousarray(X, np.float32), out, flag, m, n, k)
+    return out
+
+
+def kr(A, B, a, b, r):
+    out = np.empty(a * b * r, np.float32)
+    lib().orc_kr(np.ascontiguousarray(A, np.float32), np.ascontiguousarray(B, np.float32), out, a, b, r)
+    return out
+
+
+def hadamard(A, B):
+    out = np.empty(A.size, np.float32)
+    lib().orc_hadamard(np.ascontiguousarray(A, np.float32), np.ascontiguousarray(B, np.float32), out, A.size)
+    return out
+
+
+def hadamard3(A, B, C):
+    out = np.empty(A.size, np.float32)
+    lib().orc_hadamard3(*[np.ascontiguousarray(x, np.float32) for x in (A, B, C)], out, A.size)
+    return out
+
+
+def tensor_product(A, B, a, b, c, d):
+    out = np.empty(a * b * c * d, np.float32)
+    lib().orc_tensor_product(np.ascontiguousarray(A, np.float32), np.ascontiguousarray(B, np.float32),
+                             out, a, b, c, d)
+    return out
+
+
+def tminust(L, R):
+    out = np.ascontiguousarray(R, np.float32).copy()
+    lib().orc_tminust(np.ascontiguousarray(L, np.float32), out, out.size)
+    return out
+
+
+def identity(m):
+    out = np.empty(m * m, np.float32)
+    lib().orc_identity(out, m)
+    return out
+
+
+def bf16_bits(x):
+    f = lib().orc_f32_to_bf16
+    return np.array([f(float(v)) for v in np.asarray(x, np.float32).ravel()], np.uint16)
+
+
+def f16_bits(x):
+    f = lib().orc_f32_to_f16
+    return np.array([f(float(v)) for v in np.asarray(x, np.float32).ravel()], np.uint16)
+
+
+def e4m3_bits(x):
+    f = lib().orc_f32_to_e4m3
+    return np.array([f(float(v)) for v in np.asarray(x, np.float32).ravel()], np.uint8)
+
+
+def e4m3_decode(q):
+    f = lib().orc_e4m3_to_f32
+    return np.array([f(int(v)) for v in np.asarray(q, np.uint8).ravel()], np.float32)
+
+
+def quantize_e4m3_mode1(X, I):
+    X = np.ascontiguousarray(X, np.float32)
+    Q = np.empty(X.size, np.uint8)
+    s = np.empty(I, np.float32)
+    lib().orc_quantize_e4m3_mode1(X, I, X.size // I, Q, s)
+    return Q, s
+
+
+def mttkrp(X, dims, U, mode, scales=None):
+    """FP64 MTTKRP; X float32 flat col-major, U list of (I_m, R) arrays. Returns (I_mode, R) float64."""
+    R = U[0].shape[1]
+    Ud = [np.ascontiguousarray(np.asarray(u, np.float64).ravel(order="F")) for u in U]
+    out = np.empty(dims[mode] * R, np.float64)
+    X = np.ascontiguousarray(X, np.float32)
+    if scales is None:
+        lib().orc_mttkrp(X, len(dims), _dims(dims), _ptrs(Ud), R, mode, out)
+    else:
+        lib().orc_mttkrp_scaled(X, np.ascontiguousarray(scales, np.float32), len(dims), _dims(dims),
+                                _ptrs(Ud), R, mode, out)
+    return out.reshape((dims[mode], R), order="F")
+
+
+def ttm(X, U, flag, a, b, c, d):
+    osz = {1: d * b * c, 2: a * d * c, 3: a * b * d}[flag]
+    out = np.empty(osz, np.float64)
+    lib().orc_ttm(np.ascontiguousarray(X, np.float64), np.ascontiguousarray(U, np.float64), out, flag, a, b, c, d)
+    return out
+
+
+def sym_eig(A):
+    n = A.shape[0]
+    w = np.empty(n)
+    V = np.empty(n * n)
+    lib().orc_sym_eig(np.ascontiguousarray(np.asarray(A, np.float64).ravel(order="F")), n, w, V)
+    return w, V.reshape((n, n), order="F")
+
+
+def cholesky_solve_rows(V, M):
+    R = V.shape[0]
+    I = M.shape[0]
+    L = np.ascontiguousarray(np.asarray(V, np.float64).ravel(order="F")).copy()
+    st = lib().orc_cholesky(L, R)
+    U = np.empty(I * R)
+    if st == 0:
+        lib().orc_chol_solve_rows(L, R, np.ascontiguousarray(M.ravel(order="F"), np.float64), I, U)
+    else:
+        lib().orc_pinv_solve_rows(np.ascontiguousarray(V.ravel(order="F"), np.float64), R,
+                                  np.ascontiguousarray(M.ravel(order="F"), np.float64), I, U, 1e-12)
+    return st, U.reshape((I, R), order="F")
+
+
+def cp_reconstruct(dims, U):
+    R = U[0].shape[1]
+    Ud = [np.ascontiguousarray(np.asarray(u, np.float64).ravel(order="F")) for u in U]
+    out = np.empty(int(np.prod(dims)), np.float64)
+    lib().orc_cp_reconstruct(len(dims), _dims(dims), _ptrs(Ud), R, out)
+    return out
+
+
+def cp_relerr(X, dims, U):
+    R = U[0].shape[1]
+    Ud = [np.ascontiguousarray(np.asarray(u, np.float64).ravel(order="F")) for u in U]
+    return lib().orc_cp_relerr(np.ascontiguousarray(X, np.float32), len(dims), _dims(dims), _ptrs(Ud), R)
+
+
+def cp_als(X, dims, U0, sweeps, direct=False):
+    """FP64 ALS from init U0 (list of (I_m,R)); returns (U list, fit relerr per sweep, direct relerr or None, status)."""
+    R = U0[0].shape[1]
+    Ud = [np.ascontiguousarray(np.asarray(u, np.float64).ravel(order="F")).copy() for u in U0]
+    fit = np.empty(sweeps)
+    dr = np.empty(sweeps) if direct else None
+    st = lib().orc_cp_als(np.ascontiguousarray(X, np.float32), len(dims), _dims(dims), R, _ptrs(Ud), sweeps,
+                          fit.ctypes.data, dr.ctypes.data if direct else None)
+    U = [u.reshape((d, R), order="F") for u, d in zip(Ud, dims)]
+    return U, fit, dr, st
+
+
+def tucker_hosvd(X, dims, ranks):
+    Ud = [np.empty(d * r) for d, r in zip(dims, ranks)]
+    core = np.empty(int(np.prod(ranks)))
+    lib().orc_tucker_hosvd(np.ascontiguousarray(X, np.float32), len(dims), _dims(dims), _dims(ranks),
+                           _ptrs(Ud), core)
+    return [u.reshape((d, r), order="F") for u, d, r in zip(Ud, dims, ranks)], core
+
+
+def tucker_hooi(X, dims, ranks, iters, U0):
+    Ud = [np.ascontiguousarray(np.asarray(u, np.float64).ravel(order="F")).copy() for u in U0]
+    core = np.empty(int(np.prod(ranks)))
+    hist = np.empty(max(iters, 1))
+    lib().orc_tucker_hooi(np.ascontiguousarray(X, np.float32), len(dims), _dims(dims), _dims(ranks), iters,
+                          _ptrs(Ud), core, hist.ctypes.data)
+    return [u.reshape((d, r), order="F") for u, d, r in zip(Ud, dims, ranks)], core, hist[:iters]
+
+
+def tucker_reconstruct(dims, ranks, core, U):
+    Ud = [np.ascontiguousarray(np.asarray(u, np.float64).ravel(order="F")) for u in U]
+    out = np.empty(int(np.prod(dims)))
+    lib().orc_tucker_reconstruct(len(dims), _dims(dims), _dims(ranks), np.ascontiguousarray(core, np.float64),
+                                 _ptrs(Ud), out)
+    return out
+
+
+def mlp_forward(x, params, nin, width, nout):
+    W1, b1, W2, b2, W3, b3 = [np.ascontiguousarray(p, np.float64) for p in params]
+    batch = x.shape[0]
+    y = np.empty(batch * nout)
+    lib().orc_mlp_forward(np.ascontiguousarray(x, np.float64).ravel(), batch, nin, width, nout,
+                          W1, b1, W2, b2, W3, b3, y)
+    return y.reshape(batch, nout)
+
+
+def mlp_backward(x, t, params, nin, width, nout):
+    W1, b1, W2, b2, W3, b3 = [np.ascontiguousarray(p, np.float64) for p in params]
+    batch = x.shape[0]
+    g = [np.empty_like(p) for p in (W1, b1, W2, b2, W3, b3)]
+    loss = lib().orc_mlp_backward(np.ascontiguousarray(x, np.float64).ravel(),
+                                  np.ascontiguousarray(t, np.float64).ravel(), batch, nin, width, nout,
+                                  W1, b1, W2, b2, W3, b3, *g)
+    return loss, g
+
+
+def adam(p, g, m, v, t, lr, b1=0.9, b2=0.999, eps=1e-8):
+    lib().orc_adam(p, np.ascontiguousarray(g, np.float64), m, v, p.size, t, lr, b1, b2, eps)

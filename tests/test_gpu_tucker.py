@@ -1,0 +1,76 @@
+"""GPU Tucker (HOSVD / HOOI, orders 3 and 4) against the FP64 oracle.
+
+Eigenvector signs follow one convention on both sides (largest-magnitude entry
+positive), so factors compare directly; the reference's cuSOLVER signs are unspecified.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+import tdx
+from helpers import dev, empty, host, relnorm
+
+pytestmark = pytest.mark.gpu
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "golden.json")))
+
+
+def tucker_tensor(dims, ranks, eta, seed=0):
+    rng = np.random.default_rng(seed)
+    core = rng.standard_normal(int(np.prod(ranks)))
+    Us = [np.linalg.qr(rng.standard_normal((d, r)))[0] for d, r in zip(dims, ranks)]
+    X = O.tucker_reconstruct(dims, ranks, core, Us)
+    E = rng.standard_normal(X.size)
+    X = X + eta * np.linalg.norm(X) / np.linalg.norm(E) * E
+    return X.astype(np.float32)
+
+
+def test_gentucker_golden():
+    g = GOLD["hand_tucker_3x4x2"]
+    dims, ranks = g["dims"], g["ranks"]
+    A, B, C, G = [dev(np.asarray(g[k], np.float32)) for k in "ABCG"]
+    rec = empty(int(np.prod(dims)))
+    tdx.gentucker(rec, G, A, B, C, *dims, *ranks)
+    assert np.array_equal(host(rec), np.asarray(g["X"], np.float32))
+
+
+@pytest.mark.parametrize("dims,ranks", [((20, 16, 12), (4, 3, 5)), ((32, 24, 16, 12), (6, 5, 4, 3))])
+def test_hosvd_matches_oracle(dims, ranks):
+    X = tucker_tensor(dims, ranks, 1e-3)
+    Uo, co = O.tucker_hosvd(X, dims, ranks)
+    Ud = [empty(d * r) for d, r in zip(dims, ranks)]
+    core = empty(int(np.prod(ranks)))
+    if len(dims) == 3:
+        tdx.tucker_hosvd(dev(X), core, *Ud, *dims, *ranks)
+    else:
+        tdx.tucker_hosvd4(dev(X), core, *Ud, *dims, *ranks)
+    for u, uo, d, r in zip(Ud, Uo, dims, ranks):
+        ug = host(u).reshape((d, r), order="F")
+        assert np.allclose(ug.T @ ug, np.eye(r), atol=1e-5)
+        assert relnorm(ug, uo) < 1e-4
+    assert relnorm(host(core), co) < 1e-4
+
+
+def test_hooi_relerr_matches_oracle():
+    dims, ranks = (24, 20, 16, 12), (5, 4, 4, 3)
+    X = tucker_tensor(dims, ranks, 5e-3, seed=1)
+    U0, _ = O.tucker_hosvd(X, dims, ranks)
+    _, co, hist_o = O.tucker_hooi(X, dims, ranks, 3, U0)
+    Ud = [empty(d * r) for d, r in zip(dims, ranks)]
+    core = empty(int(np.prod(ranks)))
+    hist = tdx.tucker_hooi(dev(X), dims, ranks, 3, core, Ud)
+    assert np.max(np.abs(hist - hist_o)) < 1e-6
+    assert abs(np.linalg.norm(host(core)) - np.linalg.norm(co)) / np.linalg.norm(co) < 1e-5
+
+
+def test_ttm_golden_and_random():
+    g = GOLD["ttm_srand2"]
+    a, b, c = g["dims"]
+    d = g["d"]
+    X = dev(np.asarray(g["X"], np.float32))
+    for f in (1, 2, 3):
+        out = empty({1: d * b * c, 2: a * d * c, 3: a * b * d}[f])
+        tdx.ttm(X, dev(np.asarray(g["U"][f - 1], np.float32)), out, f, a, b, c, d)
+        assert np.allclose(host(out), np.asarray(g["XU"][f - 1]), rtol=1e-6, atol=1e-7)
