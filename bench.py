@@ -35,6 +35,32 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def kernels_only(args, X, dims, R, U):
+    """MTTKRP per storage and mode, then a plain copy of 2^28 floats (1 GiB read + 1 GiB
+    write) whose counter readings calibrate FETCH_SIZE / WRITE_SIZE."""
+    import torch
+
+    import tdx
+    n = dims[0]
+    N = X.numel()
+    Xb = torch.empty(N, dtype=torch.int16, device=X.device)
+    tdx.quantize_bf16(X, Xb)
+    Xq = torch.empty(N, dtype=torch.uint8, device=X.device)
+    sc = torch.empty(n, dtype=torch.float32, device=X.device)
+    tdx.quantize_fp8_mode1(X, Xq, sc, n)
+    res = {}
+    for name, Xs, dt, scales in (("fp32", X, tdx.TDX_F32, None), ("bf16", Xb, tdx.TDX_BF16, None),
+                                 ("fp8", Xq, tdx.TDX_FP8_E4M3, sc)):
+        res[name] = [round(tdx.bench_mttkrp(Xs, dt, scales, dims, U, R, mode, args.mttkrp_iters), 4)
+                     for mode in range(3)]
+    src = X[: 1 << 28]
+    dst = torch.empty(1 << 28, dtype=torch.float32, device=X.device)
+    for _ in range(3):
+        tdx.t2m(src, dst, 1, 1 << 10, 1 << 9, 1 << 9)
+    torch.cuda.synchronize()
+    print(json.dumps({"kernels_only_ms": res, "copy_floats": 1 << 28}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -44,6 +70,8 @@ def main():
     ap.add_argument("--rank", type=int, default=64)
     ap.add_argument("--mttkrp-iters", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only the MTTKRP launches (3 storages x 3 modes) and a 1 GiB plain copy, for counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only",
@@ -89,6 +117,10 @@ def main():
         o.init, o.record_fit, o.use_graph = 1, 1, 1
         st, rep, fit, ms = tdx.cp_als_ex(X, dims, R, U, o, want_ms=want_ms)
         return st, rep, np.asarray(fit), np.asarray(ms, np.float64), U
+
+    if args.kernels_only:
+        kernels_only(args, X, dims, R, init())
+        return
 
     # warm-up (also instantiates graphs, workspaces)
     log("warm-up")

@@ -15,10 +15,14 @@ from helpers import dev, empty, host, relnorm
 pytestmark = pytest.mark.gpu
 
 
-def make_cp_tensor(dims, R, seed_f=0, eta=1e-2, noise_seed=2):
+def make_cp_tensor(dims, R, seed_f=0, eta=1e-2, noise_seed=2, kind="uniform"):
     """Low-rank-plus-noise tensor built on the device by tdx_gen_cp_tensor from U(0,1)
-    factors of the Philox stream (seed_f, mode)."""
-    F = [O.uniform(d * R, seed_f, m).reshape((d, R), order="F") for m, d in enumerate(dims)]
+    factors of the Philox stream (seed_f, mode), or N(0,1) factors with unit columns."""
+    if kind == "uniform":
+        F = [O.uniform(d * R, seed_f, m).reshape((d, R), order="F") for m, d in enumerate(dims)]
+    else:
+        F = [O.normal(d * R, seed_f, m).reshape((d, R), order="F") for m, d in enumerate(dims)]
+        F = [(f / np.linalg.norm(f.astype(np.float64), axis=0)).astype(np.float32) for f in F]
     X = empty(int(np.prod(dims)))
     tdx.gen_cp_tensor(X, dims, [dev(f.ravel(order="F")) for f in F], R, eta, noise_seed, 0)
     return X, F
@@ -65,9 +69,10 @@ def test_c1_plain_fp32_tracks_fp64_oracle():
     # final factors: the GPU direct residual of its own factors vs the oracle
     rel_gpu = O.cp_relerr(Xh, dims, [u.astype(np.float64) for u in U])
     assert abs(rel_gpu - dr_o[-1]) <= 1e-5
-    assert rel_gpu < 0.0101
     for u, uo in zip(U, Uo):
         assert relnorm(u, uo) < 1e-3
+    # ALS error never increases (U(0,1) factors: a slow "swamp", far from the 1% floor at 50 sweeps)
+    assert np.all(np.diff(fit) <= 1e-6)
 
 
 def test_fit_identity_vs_direct_residual_on_device():
@@ -93,8 +98,9 @@ def test_graph_and_eager_identical():
 
 @pytest.mark.parametrize("storage", [tdx.TDX_BF16, tdx.TDX_FP8_E4M3])
 def test_mixed_precision_reaches_oracle(storage):
+    # converging problem (N(0,1) unit-column factors, as in the 1024^3 workload)
     dims, R = (128, 96, 64), 16
-    X, _ = make_cp_tensor(dims, R, eta=1e-3)
+    X, _ = make_cp_tensor(dims, R, eta=1e-3, kind="normal")
     U0 = init_factors(dims, R, seed=5)
     st, rep, fit, U = run_gpu_als(X, dims, R, U0, 25, storage=storage, refine=2)
     Xh = host(X)
@@ -116,7 +122,9 @@ def test_exact_factors_fixed_point():
     dims, R = (40, 32, 24), 4
     X, F = make_cp_tensor(dims, R, eta=0.0)
     st, rep, fit, U = run_gpu_als(X, dims, R, F, 3)
-    assert fit[-1] < 1e-5
+    # the fast-fit identity cancels at this level (FP32 MTTKRP); the direct residual does not
+    direct = tdx.cp_relerr(X, dims, [dev(u.ravel(order="F")) for u in U], R)
+    assert direct < 2e-6 and fit[-1] < 2e-4
 
 
 def test_singular_normal_equations_fall_back():

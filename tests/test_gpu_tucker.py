@@ -61,7 +61,9 @@ def test_hooi_relerr_matches_oracle():
     Ud = [empty(d * r) for d, r in zip(dims, ranks)]
     core = empty(int(np.prod(ranks)))
     hist = tdx.tucker_hooi(dev(X), dims, ranks, 3, core, Ud)
-    assert np.max(np.abs(hist - hist_o)) < 1e-6
+    # relerr = sqrt(1 - ||core||^2/||X||^2) from an FP32 TTM chain: an FP32 error of ~1e-7 in
+    # ||core||^2/||X||^2 moves relerr (5e-3 here) by ~1e-5
+    assert np.max(np.abs(hist - hist_o)) < 2e-5
     assert abs(np.linalg.norm(host(core)) - np.linalg.norm(co)) / np.linalg.norm(co) < 1e-5
 
 
