@@ -16,6 +16,7 @@
 // A sweep is captured once into a hipGraph and replayed.
 #include "mttkrp.hpp"
 #include "linalg.hpp"
+#include "mlp.hpp"
 #include <math.h>
 #include <string.h>
 #include <vector>
@@ -114,7 +115,8 @@ __global__ __launch_bounds__(256) void k_solve(const float *__restrict__ partial
                                                long In, int R, const double *__restrict__ Vinv,
                                                float *__restrict__ U, float *__restrict__ panel,
                                                const float *__restrict__ panel_scale,
-                                               double *__restrict__ ip_part, double *__restrict__ gram_part) {
+                                               double *__restrict__ ip_part, double *__restrict__ gram_part,
+                                               int *__restrict__ skip_reset) {
     __shared__ double Vs[kMaxR * kMaxR];
     __shared__ double Ms[kSolveRows * kMaxR];
     __shared__ double sh[256];
@@ -178,6 +180,8 @@ __global__ __launch_bounds__(256) void k_solve(const float *__restrict__ partial
     }
     ip = block_sum(ip, sh);
     if (tid == 0) ip_part[blockIdx.x] = ip;
+    // the mode-0 MTTKRP that may have been skipped has already run (stream order)
+    if (skip_reset && blockIdx.x == 0 && tid == 0) *skip_reset = 0;
     // Ms now holds U block (sync inside block_sum)
     double *gp = gram_part + (long)blockIdx.x * RR;
     for (int e = tid; e < RR; e += 256) {
@@ -267,6 +271,245 @@ __global__ void k_sum_partials(const double *__restrict__ part, int n, double *_
 __global__ void k_set_int(int *p, int v) { *p = v; }
 
 // ---------------------------------------------------------------------------
+// Extrapolation step (Tensorlab cpd_als.m:234-339 line search with the step length from
+// the learned MLP, the cpd_lsb.m:30 rule, or a grid for training labels).  After the plain
+// sweep U_start -> U, U_a = U_start + alpha (U - U_start) is evaluated exactly:
+// <MTTKRP_0(U_a), U_a,0> and the Grams of U_a give ||X - [[U_a]]||^2 in FP64.  It replaces
+// U only if it is better; the mode-0 MTTKRP of U_a then serves the next sweep's first update.
+
+struct FacSet {
+    float *p[4];
+    long I[4];
+    int order;
+    int R;
+};
+
+struct LsState {
+    int tries;
+    int accepts;
+    float last_alpha;
+    float alpha_cur;
+    int accept;
+    int skip;
+};
+
+__device__ inline void fac_locate(const FacSet &f, long e, int &m, long &off) {
+    m = 0;
+    off = e;
+    while (m < f.order - 1 && off >= f.I[m] * f.R) {
+        off -= f.I[m] * f.R;
+        ++m;
+    }
+}
+
+__global__ void k_ls_init(LsState *st) {
+    st->tries = 0;
+    st->accepts = 0;
+    st->last_alpha = 1.0f;
+    st->alpha_cur = 1.0f;
+    st->accept = 0;
+    st->skip = 0;
+}
+
+__global__ void k_copy_set(FacSet src, FacSet dst) {
+    long total = 0;
+    for (int m = 0; m < src.order; ++m) total += src.I[m] * src.R;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+        int m;
+        long off;
+        fac_locate(src, e, m, off);
+        dst.p[m][off] = src.p[m][off];
+    }
+}
+
+// features (see TDX_LS_FEATURES in tdx.h) and the step length; one workgroup
+__global__ __launch_bounds__(256) void k_ls_features(FacSet U, FacSet U1, const double *__restrict__ fit_hist,
+                                                     const int *__restrict__ counter, LsState *__restrict__ st,
+                                                     int mode, float grid_alpha, const float *__restrict__ mlp,
+                                                     int width, float *__restrict__ feat_hist,
+                                                     float *__restrict__ alpha_hist, double *__restrict__ plain_hist) {
+    __shared__ double sh[256];
+    __shared__ float feat[TDX_LS_FEATURES];
+    __shared__ float msh[256];
+    const int tid = threadIdx.x;
+    const int c = *counter;  // sweeps recorded so far; this sweep is c - 1
+    const int k = c - 1;
+    double chg[4] = {0, 0, 0, 0};
+    for (int m = 0; m < U.order; ++m) {
+        double d2 = 0.0, u2 = 0.0;
+        const long n = U.I[m] * U.R;
+        for (long e = tid; e < n; e += 256) {
+            const double u = U.p[m][e], d = u - (double)U1.p[m][e];
+            d2 += d * d;
+            u2 += u * u;
+        }
+        d2 = block_sum(d2, sh);
+        u2 = block_sum(u2, sh);
+        chg[m] = sqrt(d2 / fmax(u2, 1e-300));
+    }
+    if (tid == 0) {
+        const double r0 = fit_hist[k];
+        feat[0] = (float)log10(fmax(r0, 1e-30));
+        for (int j = 1; j <= 3; ++j) {
+            float v = 0.0f;
+            if (k - j >= 0) v = (float)log10(fmax(fit_hist[k - j], 1e-30) / fmax(fit_hist[k - j + 1], 1e-30));
+            feat[j] = v;
+        }
+        for (int m = 0; m < 4; ++m) feat[4 + m] = (m < U.order) ? (float)log10(chg[m] + 1e-12) : 0.0f;
+        feat[8] = (float)log10((double)k + 1.0);
+        feat[9] = (U.order == 4) ? 1.0f : 0.0f;
+        feat[10] = logf(fmaxf(st->last_alpha, 1.0f));
+        feat[11] = (st->tries > 0) ? (float)st->accepts / (float)st->tries : 0.0f;
+        if (plain_hist) plain_hist[k] = r0;
+    }
+    __syncthreads();
+    float alpha;
+    if (mode == 1) {
+        MlpShape sh_{TDX_LS_FEATURES, width, 1};
+        const float beta = mlp_forward_one(mlp, sh_, feat, msh);
+        alpha = expf(fminf(fmaxf(beta, 0.0f), 3.4657359f));  // [1, 32]
+    } else if (mode == 2) {
+        alpha = (float)cbrt((double)k + 1.0);
+    } else {
+        alpha = grid_alpha;
+    }
+    if (tid == 0) {
+        st->alpha_cur = alpha;
+        if (alpha_hist) alpha_hist[k] = alpha;
+    }
+    if (feat_hist && tid < TDX_LS_FEATURES) feat_hist[(long)k * TDX_LS_FEATURES + tid] = feat[tid];
+}
+
+// U_a = U_start + alpha (U - U_start), written column-major and as panels (mode-0 panel
+// scaled by the FP8 slice scales when given)
+__global__ void k_extrapolate(FacSet U, FacSet U1, FacSet Ua, float *pa0, float *pa1, float *pa2, float *pa3, int Rp,
+                              const LsState *__restrict__ st, const float *__restrict__ scale0) {
+    float *pa[4] = {pa0, pa1, pa2, pa3};
+    const float alpha = st->alpha_cur;
+    long total = 0, prow[4];
+    for (int m = 0; m < U.order; ++m) {
+        prow[m] = (U.I[m] + 63) / 64 * 64;
+        total += prow[m] * Rp;
+    }
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+        int m = 0;
+        long off = e;
+        while (m < U.order - 1 && off >= prow[m] * Rp) {
+            off -= prow[m] * Rp;
+            ++m;
+        }
+        const long i = ((off >> 3) / Rp) * 8 + (off & 7);
+        const long r = (off >> 3) % Rp;
+        float v = 0.0f;
+        if (i < U.I[m] && r < U.R) {
+            const long idx = i + r * U.I[m];
+            const float u1 = U1.p[m][idx];
+            v = fmaf(alpha, U.p[m][idx] - u1, u1);
+            Ua.p[m][idx] = v;
+            if (m == 0 && scale0) v *= scale0[i];
+        }
+        pa[m][off] = v;
+    }
+}
+
+// Grams of a factor set, one workgroup per mode
+__global__ __launch_bounds__(256) void k_gram_set(FacSet U, double *G0, double *G1, double *G2, double *G3) {
+    double *Gs[4] = {G0, G1, G2, G3};
+    const int m = blockIdx.x;
+    const long I = U.I[m];
+    const int R = U.R;
+    const float *u = U.p[m];
+    for (int e = threadIdx.x; e < R * R; e += 256) {
+        const int a = e % R, b = e / R;
+        double s = 0.0;
+        for (long i = 0; i < I; ++i) s += (double)u[i + a * I] * (double)u[i + b * I];
+        Gs[m][e] = s;
+    }
+}
+
+// per-block <M_0(U_a), U_a,0> with M_0 the FP64 sum of the split partials
+__global__ __launch_bounds__(256) void k_ls_ip(const float *__restrict__ partial, int nsplit, long rows_pad, int Rp,
+                                               long In, int R, const float *__restrict__ Ua0,
+                                               double *__restrict__ ip_part) {
+    __shared__ double sh[256];
+    const long row0 = (long)blockIdx.x * kSolveRows;
+    double ip = 0.0;
+    for (int e = threadIdx.x; e < kSolveRows * Rp; e += 256) {
+        const long i = row0 + e / Rp;
+        const int b = e % Rp;
+        if (i >= In || b >= R) continue;
+        double m = 0.0;
+        for (int sp = 0; sp < nsplit; ++sp) m += (double)partial[((long)sp * rows_pad + i) * Rp + b];
+        ip += m * (double)Ua0[i + (long)b * In];
+    }
+    ip = block_sum(ip, sh);
+    if (threadIdx.x == 0) ip_part[blockIdx.x] = ip;
+}
+
+// exact relative error of U_a; accept (line_search 1, 2) or record (3)
+__global__ __launch_bounds__(256) void k_ls_decide(const double *__restrict__ ip_part, int nblk, const double *Ga0,
+                                                   const double *Ga1, const double *Ga2, const double *Ga3, int order,
+                                                   int R, const double *__restrict__ normx2, double *fit_hist,
+                                                   const int *__restrict__ counter, int mode, int g, int ngrid,
+                                                   double *__restrict__ grid_hist, int *__restrict__ accepted_hist,
+                                                   LsState *__restrict__ st) {
+    __shared__ double sh[256];
+    const double *Ga[4] = {Ga0, Ga1, Ga2, Ga3};
+    double hx = 0.0;
+    for (int e = threadIdx.x; e < R * R; e += 256) {
+        double v = 1.0;
+        for (int m = 0; m < order; ++m) v *= Ga[m][e];
+        hx += v;
+    }
+    hx = block_sum(hx, sh);
+    double ip = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += 256) ip += ip_part[b];
+    ip = block_sum(ip, sh);
+    if (threadIdx.x != 0) return;
+    const double nx2 = *normx2;
+    const double rel_a = sqrt(fabs(nx2 - 2.0 * ip + hx) / nx2);
+    const int k = *counter - 1;
+    if (mode == 3) {
+        if (grid_hist) grid_hist[(long)k * ngrid + g] = rel_a;
+        st->accept = 0;
+        st->skip = 0;
+        return;
+    }
+    const int acc = (isfinite(rel_a) && rel_a < fit_hist[k]) ? 1 : 0;
+    st->tries += 1;
+    if (acc) {
+        st->accepts += 1;
+        st->last_alpha = st->alpha_cur;
+        fit_hist[k] = rel_a;
+    } else {
+        st->last_alpha = 1.0f;
+    }
+    st->accept = acc;
+    st->skip = acc;
+    if (accepted_hist) accepted_hist[k] = acc;
+}
+
+// on acceptance: U <- U_a, panels <- panels_a, G <- G_a
+__global__ void k_ls_commit(FacSet U, FacSet Ua, float *p0, float *p1, float *p2, float *p3, const float *pa0,
+                            const float *pa1, const float *pa2, const float *pa3, int Rp, double *G0, double *G1,
+                            double *G2, double *G3, const double *Ga0, const double *Ga1, const double *Ga2,
+                            const double *Ga3, const LsState *__restrict__ st) {
+    if (!st->accept) return;
+    float *pd[4] = {p0, p1, p2, p3};
+    const float *ps[4] = {pa0, pa1, pa2, pa3};
+    double *Gd[4] = {G0, G1, G2, G3};
+    const double *Gs[4] = {Ga0, Ga1, Ga2, Ga3};
+    const long gstride = (long)gridDim.x * blockDim.x, g0 = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    for (int m = 0; m < U.order; ++m) {
+        const long n = U.I[m] * U.R;
+        for (long e = g0; e < n; e += gstride) U.p[m][e] = Ua.p[m][e];
+        const long np = (U.I[m] + 63) / 64 * 64 * Rp;
+        for (long e = g0; e < np; e += gstride) pd[m][e] = ps[m][e];
+        for (long e = g0; e < (long)U.R * U.R; e += gstride) Gd[m][e] = Gs[m][e];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // host driver
 
 struct CpPlan {
@@ -292,9 +535,38 @@ struct CpPlan {
     hipStream_t side = nullptr;
     hipEvent_t ev_fork[4], ev_join[4];
     bool events = false;
+    // extrapolation step
+    int ls_mode = 0;
+    const float *mlp = nullptr;
+    int mlp_width = 64;
+    int ngrid = 0;
+    double grid[8];
+    float *U1[4] = {nullptr, nullptr, nullptr, nullptr};
+    float *Ua[4] = {nullptr, nullptr, nullptr, nullptr};
+    float *panel_a[4] = {nullptr, nullptr, nullptr, nullptr};
+    double *Ga[4] = {nullptr, nullptr, nullptr, nullptr};
+    LsState *ls = nullptr;
+    double *ip_part_a = nullptr;
+    float *feat_hist = nullptr, *alpha_hist = nullptr;
+    int *accepted_hist = nullptr;
+    double *grid_hist = nullptr, *plain_hist = nullptr;
+    std::vector<void *> owned;
 
     ~CpPlan() { release(); }
+    FacSet set_of(float *const *p) const {
+        FacSet f{};
+        f.order = order;
+        f.R = (int)R;
+        for (int m = 0; m < 4; ++m) {
+            f.p[m] = m < order ? p[m] : nullptr;
+            f.I[m] = m < order ? dims[m] : 0;
+        }
+        return f;
+    }
     void release() {
+        for (void *p : owned)
+            if (p) (void)hipFree(p);
+        owned.clear();
         for (int m = 0; m < 4; ++m) {
             if (panel[m]) (void)hipFree(panel[m]);
             if (G[m]) (void)hipFree(G[m]);
@@ -347,6 +619,28 @@ static int plan_setup(CpPlan &P, int order, const long *dims, long R, float *con
     TDX_HIP(hipMalloc(&P.status, sizeof(int)));
     TDX_HIP(hipMalloc(&P.normx2, sizeof(double)));
     TDX_HIP(hipMalloc(&P.scratch, sizeof(double) * 4096));
+    if (P.ls_mode) {
+        auto alloc = [&](void **p, size_t bytes) -> int {
+            TDX_HIP(hipMalloc(p, bytes));
+            P.owned.push_back(*p);
+            return TDX_OK;
+        };
+        int rc = 0;
+        for (int m = 0; m < order && !rc; ++m) {
+            rc = alloc((void **)&P.U1[m], sizeof(float) * dims[m] * R);
+            if (!rc) rc = alloc((void **)&P.Ua[m], sizeof(float) * dims[m] * R);
+            if (!rc) rc = alloc((void **)&P.panel_a[m], sizeof(float) * panel_floats(dims[m], P.Rp));
+            if (!rc) rc = alloc((void **)&P.Ga[m], sizeof(double) * R * R);
+        }
+        if (!rc) rc = alloc((void **)&P.ls, sizeof(LsState));
+        if (!rc) rc = alloc((void **)&P.ip_part_a, sizeof(double) * nblk_max);
+        if (!rc) rc = alloc((void **)&P.feat_hist, sizeof(float) * P.max_sweeps_hist * TDX_LS_FEATURES);
+        if (!rc) rc = alloc((void **)&P.alpha_hist, sizeof(float) * P.max_sweeps_hist);
+        if (!rc) rc = alloc((void **)&P.accepted_hist, sizeof(int) * P.max_sweeps_hist);
+        if (!rc) rc = alloc((void **)&P.plain_hist, sizeof(double) * P.max_sweeps_hist);
+        if (!rc && P.ngrid > 0) rc = alloc((void **)&P.grid_hist, sizeof(double) * P.max_sweeps_hist * P.ngrid);
+        if (rc) return rc;
+    }
     TDX_HIP(hipStreamCreateWithFlags(&P.side, hipStreamNonBlocking));
     for (int m = 0; m < 4; ++m) {
         TDX_HIP(hipEventCreateWithFlags(&P.ev_fork[m], hipEventDisableTiming));
@@ -382,6 +676,8 @@ static int storage_setup(CpPlan &P, Storage &S, const void *X, int dtype, const 
 
 // (re)packs panels and Grams from the current factors
 static int refresh_factors(CpPlan &P, const Storage &S, hipStream_t s) {
+    // partials of a previous storage phase are never reused across a storage change
+    if (P.ls_mode) k_set_int<<<1, 1, 0, s>>>(&P.ls->skip, 0);
     for (int m = 0; m < P.order; ++m) {
         const float *sc = (S.dtype == TDX_FP8_E4M3 && m == 0) ? S.scales : nullptr;
         int st = pack_panel(P.U[m], P.dims[m], P.R, P.Rp, sc, P.panel[m], s);
@@ -406,8 +702,14 @@ static int norm2(const void *X, int dtype, const float *scales, long I, long n, 
 }
 
 // enqueue one sweep (all modes) on stream s
+static int enqueue_ls(CpPlan &P, const Storage &S, hipStream_t s);
+
 static int enqueue_sweep(CpPlan &P, const Storage &S, hipStream_t s, bool fit) {
     const int R = (int)P.R;
+    if (P.ls_mode) {
+        k_copy_set<<<256, 256, 0, s>>>(P.set_of(P.U), P.set_of(P.U1));
+        TDX_LAUNCH_CHECK();
+    }
     for (int n = 0; n < P.order; ++n) {
         // fork: V^{-1} on the side stream while the MTTKRP streams X
         TDX_HIP(hipEventRecord(P.ev_fork[n], s));
@@ -418,19 +720,56 @@ static int enqueue_sweep(CpPlan &P, const Storage &S, hipStream_t s, bool fit) {
         TDX_HIP(hipEventRecord(P.ev_join[n], P.side));
         const float *panels[4] = {P.panel[0], P.panel[1], P.panel[2], P.panel[3]};
         const float *row_scale = (S.dtype == TDX_FP8_E4M3 && n == 0) ? S.scales : nullptr;
-        int st = mttkrp_launch(S.mp[n], S.X, panels, row_scale, P.partial, s);
+        const int *skip = (P.ls_mode && n == 0) ? &P.ls->skip : nullptr;
+        int st = mttkrp_launch(S.mp[n], S.X, panels, row_scale, P.partial, s, skip);
         if (st) return st;
         TDX_HIP(hipStreamWaitEvent(s, P.ev_join[n], 0));
         const long In = P.dims[n];
         const int nblk = (int)ceil_div(In, kSolveRows);
         const float *pscale = (S.dtype == TDX_FP8_E4M3 && n == 0) ? S.scales : nullptr;
         k_solve<<<nblk, 256, 0, s>>>(P.partial, S.mp[n].nsplit, S.mp[n].rows_pad, P.Rp, In, R, P.Vinv[n], P.U[n],
-                                     P.panel[n], pscale, P.ip_part, P.gram_part);
+                                     P.panel[n], pscale, P.ip_part, P.gram_part, skip ? &P.ls->skip : nullptr);
         TDX_LAUNCH_CHECK();
         const bool do_fit = fit && (n == P.order - 1);
         k_gram_reduce<<<1, 256, 0, s>>>(P.gram_part, nblk, R, P.G[n], P.ip_part, P.G[0], P.G[1],
                                         P.order > 2 ? P.G[2] : P.G[0], P.order > 3 ? P.G[3] : P.G[0], P.order,
                                         do_fit ? 1 : 0, P.normx2, P.fit_hist, P.counter);
+        TDX_LAUNCH_CHECK();
+    }
+    if (P.ls_mode) return enqueue_ls(P, S, s);
+    return TDX_OK;
+}
+
+// one candidate U_a (or one per grid point): features/alpha, U_a and its panels and
+// Grams, mode-0 MTTKRP of U_a, exact relative error, decision, commit
+static int enqueue_ls(CpPlan &P, const Storage &S, hipStream_t s) {
+    const int R = (int)P.R;
+    const int ncand = (P.ls_mode == 3) ? P.ngrid : 1;
+    const float *sc0 = (S.dtype == TDX_FP8_E4M3) ? S.scales : nullptr;
+    const long I0 = P.dims[0];
+    const int nblk0 = (int)ceil_div(I0, kSolveRows);
+    for (int g = 0; g < ncand; ++g) {
+        const float ga = (P.ls_mode == 3) ? (float)P.grid[g] : 1.0f;
+        k_ls_features<<<1, 256, 0, s>>>(P.set_of(P.U), P.set_of(P.U1), P.fit_hist, P.counter, P.ls, P.ls_mode, ga,
+                                        P.mlp, P.mlp_width, P.feat_hist, P.alpha_hist, P.plain_hist);
+        k_extrapolate<<<256, 256, 0, s>>>(P.set_of(P.U), P.set_of(P.U1), P.set_of(P.Ua), P.panel_a[0], P.panel_a[1],
+                                          P.panel_a[2], P.panel_a[3], P.Rp, P.ls, sc0);
+        k_gram_set<<<P.order, 256, 0, s>>>(P.set_of(P.Ua), P.Ga[0], P.Ga[1], P.Ga[2], P.Ga[3]);
+        TDX_LAUNCH_CHECK();
+        const float *pa[4] = {P.panel_a[0], P.panel_a[1], P.panel_a[2], P.panel_a[3]};
+        const float *row_scale = (S.dtype == TDX_FP8_E4M3) ? S.scales : nullptr;
+        int st = mttkrp_launch(S.mp[0], S.X, pa, row_scale, P.partial, s, nullptr);
+        if (st) return st;
+        k_ls_ip<<<nblk0, 256, 0, s>>>(P.partial, S.mp[0].nsplit, S.mp[0].rows_pad, P.Rp, I0, R, P.Ua[0],
+                                      P.ip_part_a);
+        const double *G3 = P.order > 3 ? P.Ga[3] : P.Ga[0];
+        k_ls_decide<<<1, 256, 0, s>>>(P.ip_part_a, nblk0, P.Ga[0], P.Ga[1], P.Ga[2], G3, P.order, R, P.normx2,
+                                      P.fit_hist, P.counter, P.ls_mode, g, P.ngrid, P.grid_hist, P.accepted_hist,
+                                      P.ls);
+        if (P.ls_mode != 3)
+            k_ls_commit<<<64, 256, 0, s>>>(P.set_of(P.U), P.set_of(P.Ua), P.panel[0], P.panel[1], P.panel[2],
+                                           P.panel[3], P.panel_a[0], P.panel_a[1], P.panel_a[2], P.panel_a[3], P.Rp,
+                                           P.G[0], P.G[1], P.G[2], P.G[3], P.Ga[0], P.Ga[1], P.Ga[2], P.Ga[3], P.ls);
         TDX_LAUNCH_CHECK();
     }
     return TDX_OK;
@@ -462,7 +801,7 @@ static int capture_sweep(CpPlan &P, const Storage &S, hipStream_t s, bool fit, S
 
 // Runs `sweeps` sweeps on storage S; records per-sweep ms if sweep_ms != NULL.
 static int run_sweeps(CpPlan &P, const Storage &S, hipStream_t s, int sweeps, bool fit, bool graph, float *sweep_ms,
-                      double tol, double *fit_host_last, int *done) {
+                      double tol, double target, int *done) {
     *done = 0;
     if (sweeps <= 0) return TDX_OK;
     SweepGraph G;
@@ -491,14 +830,15 @@ static int run_sweeps(CpPlan &P, const Storage &S, hipStream_t s, int sweeps, bo
             if (e != hipSuccess) { st = record_hip(e, "hipEventRecord", __FILE__, __LINE__); break; }
         }
         *done = it + 1;
-        if (tol > 0.0 && fit) {
+        if ((tol > 0.0 || target > 0.0) && fit) {
             int c = 0;
             double f = 0.0;
             hipError_t e = hipStreamSynchronize(s);
             if (e == hipSuccess) e = hipMemcpy(&c, P.counter, sizeof(int), hipMemcpyDeviceToHost);
             if (e == hipSuccess && c > 0) e = hipMemcpy(&f, P.fit_hist + (c - 1), sizeof(double), hipMemcpyDeviceToHost);
             if (e != hipSuccess) { st = record_hip(e, "fit readback", __FILE__, __LINE__); break; }
-            if (prev >= 0.0 && fabs(prev - f) < tol) break;
+            if (tol > 0.0 && prev >= 0.0 && fabs(prev - f) < tol) break;
+            if (target > 0.0 && f <= target) break;
             prev = f;
         }
     }
@@ -507,7 +847,6 @@ static int run_sweeps(CpPlan &P, const Storage &S, hipStream_t s, int sweeps, bo
         for (int it = 0; it < *done; ++it) TDX_HIP(hipEventElapsedTime(&sweep_ms[it], evs[it], evs[it + 1]));
     }
     for (auto &e : evs) (void)hipEventDestroy(e);
-    (void)fit_host_last;
     return st;
 }
 
@@ -527,8 +866,9 @@ void tdx_cp_default_options(tdx_cp_options *o) {
     o->seed = 1234;
     o->tol = 0.0;
     o->record_fit = 1;
-    o->learned = 0;
+    o->line_search = 0;
     o->use_graph = 1;
+    o->mlp_width = 64;
 }
 
 int tdx_cp_als_ex(const float *X, int order, const long *dims, long rank, float *const *factors,
@@ -565,8 +905,22 @@ int tdx_cp_als_ex(const float *X, int order, const long *dims, long rank, float 
     } guard{owned};
     CpPlan P;
     const int total = opt.max_sweeps + (opt.storage == TDX_F32 ? 0 : opt.refine_sweeps);
+    if (opt.line_search < 0 || opt.line_search > 3) return record(TDX_ERR_ARG);
+    P.ls_mode = opt.line_search;
+    if (P.ls_mode == 1) {
+        if (!opt.mlp_params || opt.mlp_width <= 0 || opt.mlp_width > 128) return record(TDX_ERR_ARG);
+        P.mlp = opt.mlp_params;
+        P.mlp_width = opt.mlp_width;
+    }
+    if (P.ls_mode == 3) {
+        if (opt.ngrid <= 0 || opt.ngrid > 8) return record(TDX_ERR_ARG);
+        P.ngrid = opt.ngrid;
+        for (int g = 0; g < opt.ngrid; ++g) P.grid[g] = opt.grid[g];
+    }
+    if (P.ls_mode) opt.record_fit = 1;
     int st = plan_setup(P, order, dims, rank, factors, total);
     if (st) return st;
+    if (P.ls_mode) k_ls_init<<<1, 1, 0, s>>>(P.ls);
     // initialisation: modes 1..d-1 from the seeded generator unless the caller supplied them
     if (opt.init == 0) {
         for (int m = 1; m < order; ++m) {
@@ -605,13 +959,13 @@ int tdx_cp_als_ex(const float *X, int order, const long *dims, long rank, float 
     (void)hipEventRecord(t0, s);
     if (!st)
         st = run_sweeps(P, Smain, s, opt.max_sweeps, opt.record_fit != 0, opt.use_graph != 0, sweep_ms, opt.tol,
-                        nullptr, &done_main);
+                        opt.storage == TDX_F32 ? opt.target_relerr : 0.0, &done_main);
     if (!st && opt.storage != TDX_F32 && opt.refine_sweeps > 0) {
         // FP32 refinement: the mode-0 panel must be unscaled again (rebuilt by the first solve)
         st = refresh_factors(P, S32, s);
         if (!st)
             st = run_sweeps(P, S32, s, opt.refine_sweeps, opt.record_fit != 0, opt.use_graph != 0,
-                            sweep_ms ? sweep_ms + done_main : nullptr, 0.0, nullptr, &done_ref);
+                            sweep_ms ? sweep_ms + done_main : nullptr, 0.0, 0.0, &done_ref);
     }
     (void)hipEventRecord(t1, s);
     hipError_t e = hipStreamSynchronize(s);
@@ -632,6 +986,19 @@ int tdx_cp_als_ex(const float *X, int order, const long *dims, long rank, float 
         TDX_HIP(hipMemcpy(h.data(), P.fit_hist, sizeof(double) * count, hipMemcpyDeviceToHost));
         if (fit_hist) memcpy(fit_hist, h.data(), sizeof(double) * count);
         last = h[count - 1];
+    }
+    if (opt.trace && P.ls_mode && count > 0) {
+        tdx_cp_trace *tr = opt.trace;
+        if (tr->alpha) TDX_HIP(hipMemcpy(tr->alpha, P.alpha_hist, sizeof(float) * count, hipMemcpyDeviceToHost));
+        if (tr->accepted)
+            TDX_HIP(hipMemcpy(tr->accepted, P.accepted_hist, sizeof(int) * count, hipMemcpyDeviceToHost));
+        if (tr->features)
+            TDX_HIP(hipMemcpy(tr->features, P.feat_hist, sizeof(float) * count * TDX_LS_FEATURES,
+                              hipMemcpyDeviceToHost));
+        if (tr->plain_relerr)
+            TDX_HIP(hipMemcpy(tr->plain_relerr, P.plain_hist, sizeof(double) * count, hipMemcpyDeviceToHost));
+        if (tr->grid_relerr && P.grid_hist)
+            TDX_HIP(hipMemcpy(tr->grid_relerr, P.grid_hist, sizeof(double) * count * P.ngrid, hipMemcpyDeviceToHost));
     }
     int result = status ? TDX_WARN_SINGULAR : TDX_OK;
     if (opt.record_fit && count > 0 && !isfinite(last)) result = TDX_ERR_NONFINITE;

@@ -38,6 +38,7 @@ struct KArgs {
     const float *panel[4];
     const float *row_scale;
     float *partial;
+    const int *skip;  // when non-null and *skip != 0 the launch does nothing (partials reused)
     long stride[4];
     long dims[4];
     long In, If, FB, ksteps, kper, rows_pad;
@@ -82,6 +83,7 @@ __global__ __launch_bounds__(256, 2) void k_mttkrp(KArgs a) {
     constexpr int T = (ST == 0) ? 4 : 8;
     constexpr int RW = 16 * T;
     __shared__ float red[RW * 64];
+    if (a.skip && *a.skip) return;
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lm = lane & 15, lg = lane >> 4;
@@ -270,6 +272,7 @@ __global__ __launch_bounds__(256, 2) void k_mttkrp(KArgs a) {
 // extent, rank > 64): one workgroup per (output row, split), one lane per column.
 template <int ST>
 __global__ __launch_bounds__(64) void k_mttkrp_generic(KArgs a, long kelems) {
+    if (a.skip && *a.skip) return;
     const long row = blockIdx.x % a.In;
     const long split = blockIdx.x / a.In;
     const long k0 = split * a.kper, k1 = min(k0 + a.kper, kelems);
@@ -414,8 +417,9 @@ static void launch_fast(const MttkrpPlan &p, const KArgs &a, hipStream_t s) {
 }
 
 int mttkrp_launch(const MttkrpPlan &p, const void *X, const float *const *panels, const float *row_scale,
-                  float *partial, hipStream_t s) {
+                  float *partial, hipStream_t s, const int *skip) {
     KArgs a{};
+    a.skip = skip;
     a.X = X;
     for (int m = 0; m < 4; ++m) a.panel[m] = (m < p.order) ? panels[m] : nullptr;
     a.row_scale = row_scale;

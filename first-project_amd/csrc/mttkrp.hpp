@@ -37,7 +37,7 @@ int mttkrp_plan(MttkrpPlan &p, int order, const long *dims, int mode, int dtype,
 // panels[m] are factor panels (panels[mode] unused); for FP8 storage panels[0] must be
 // pre-scaled by the slice scales when mode != 0, and row_scale = scales when mode == 0.
 int mttkrp_launch(const MttkrpPlan &p, const void *X, const float *const *panels,
-                  const float *row_scale, float *partial, hipStream_t s);
+                  const float *row_scale, float *partial, hipStream_t s, const int *skip = nullptr);
 
 // panel packing (optionally scaling row i by scale[i])
 int pack_panel(const float *U, long I, long R, int Rp, const float *scale, float *panel, hipStream_t s);

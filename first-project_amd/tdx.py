@@ -30,10 +30,20 @@ class TdxError(RuntimeError):
         super().__init__(f"{fn}: status {status} ({msg})")
 
 
+LS_FEATURES = 12
+LS_NONE, LS_LEARNED, LS_LSB, LS_GRID = 0, 1, 2, 3
+
+
+class CpTrace(ctypes.Structure):
+    _fields_ = [("alpha", _P), ("accepted", _P), ("features", _P), ("grid_relerr", _P), ("plain_relerr", _P)]
+
+
 class CpOptions(ctypes.Structure):
     _fields_ = [("max_sweeps", ctypes.c_int), ("refine_sweeps", ctypes.c_int), ("storage", ctypes.c_int),
                 ("init", ctypes.c_int), ("seed", ctypes.c_uint64), ("tol", ctypes.c_double),
-                ("record_fit", ctypes.c_int), ("learned", ctypes.c_int), ("use_graph", ctypes.c_int)]
+                ("record_fit", ctypes.c_int), ("line_search", ctypes.c_int), ("use_graph", ctypes.c_int),
+                ("mlp_params", _P), ("mlp_width", ctypes.c_int), ("ngrid", ctypes.c_int),
+                ("grid", ctypes.c_double * 8), ("trace", ctypes.POINTER(CpTrace)), ("target_relerr", ctypes.c_double)]
 
 
 class CpReport(ctypes.Structure):
@@ -86,10 +96,16 @@ _SIGS = {
     "tdx_tucker_tensorcore": [_P, _P, _P, _P, _P, _L, _L, _L, _L, _L, _L],
     "tdx_tucker_hooi": [_P, ctypes.c_int, ctypes.POINTER(_L), ctypes.POINTER(_L), ctypes.c_int, ctypes.c_int,
                         _P, ctypes.POINTER(_P), _P],
+    "tdx_mlp_init": [_P, _L, _L, _L, ctypes.c_uint64],
+    "tdx_mlp_forward": [_P, _P, _L, _L, _L, _L, _P],
+    "tdx_mlp_backward": [_P, _P, _P, _L, _L, _L, _L, _P, _P],
+    "tdx_mlp_adam": [_P, _P, _P, _P, _L, _L, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float],
+    "tdx_mlp_train": [_P, _P, _P, _L, _L, _L, _L, ctypes.c_int, _L, ctypes.c_float, ctypes.c_uint64, _P],
 }
 
 # every symbol include/tdx.h declares (checked against the built library by the tests)
-HEADER_SYMBOLS = sorted(list(_SIGS) + ["tdx_get_stream", "tdx_status_string", "tdx_cp_default_options"])
+HEADER_SYMBOLS = sorted(list(_SIGS) + ["tdx_get_stream", "tdx_status_string", "tdx_cp_default_options",
+                                        "tdx_mlp_param_count"])
 
 
 def lib():
@@ -109,6 +125,8 @@ def lib():
         L.tdx_get_stream.restype = _P
         L.tdx_cp_default_options.argtypes = [ctypes.POINTER(CpOptions)]
         L.tdx_cp_default_options.restype = None
+        L.tdx_mlp_param_count.restype = _L
+        L.tdx_mlp_param_count.argtypes = [_L, _L, _L]
         _LIB = L
     return _LIB
 
@@ -252,16 +270,33 @@ def default_cp_options():
     return o
 
 
-def cp_als_ex(X, dims, rank, factors, opts=None, want_fit=True, want_ms=False):
-    """Returns (status, report, fit_hist list, sweep_ms list)."""
+def cp_als_ex(X, dims, rank, factors, opts=None, want_fit=True, want_ms=False, trace=None):
+    """Returns (status, report, fit_hist, sweep_ms).  With trace=dict() and a line search
+    the dict receives per-sweep arrays alpha, accepted, features, plain_relerr (and
+    grid_relerr for the grid mode)."""
     import numpy as np
     o = opts if opts is not None else default_cp_options()
     total = o.max_sweeps + (o.refine_sweeps if o.storage != TDX_F32 else 0)
     fit = np.zeros(max(total, 1), np.float64)
     ms = np.zeros(max(total, 1), np.float32)
     rep = CpReport()
+    tr = None
+    if trace is not None and o.line_search:
+        n = max(total, 1)
+        trace["alpha"] = np.zeros(n, np.float32)
+        trace["accepted"] = np.zeros(n, np.int32)
+        trace["features"] = np.zeros((n, LS_FEATURES), np.float32)
+        trace["plain_relerr"] = np.zeros(n, np.float64)
+        trace["grid_relerr"] = np.zeros((n, max(o.ngrid, 1)), np.float64)
+        tr = CpTrace(trace["alpha"].ctypes.data, trace["accepted"].ctypes.data, trace["features"].ctypes.data,
+                     trace["grid_relerr"].ctypes.data, trace["plain_relerr"].ctypes.data)
+        o.trace = ctypes.pointer(tr)
     st = _call_status("tdx_cp_als_ex", ptr(X), len(dims), dims_arr(dims), rank, ptr_arr(factors), ctypes.byref(o),
                       ctypes.byref(rep), fit.ctypes.data if want_fit else None, ms.ctypes.data if want_ms else None)
+    if tr is not None:
+        o.trace = None
+        for k in trace:
+            trace[k] = trace[k][:rep.sweeps_done]
     if st not in (TDX_OK, TDX_WARN_SINGULAR):
         raise TdxError(st, "tdx_cp_als_ex")
     return st, rep, fit[:rep.sweeps_done], ms[:rep.sweeps_done]
@@ -297,6 +332,34 @@ def tucker_hooi(X, dims, ranks, iters, core, U, bf16_ttm=False):
     _call("tdx_tucker_hooi", ptr(X), len(dims), dims_arr(dims), dims_arr(ranks), iters, 1 if bf16_ttm else 0,
           ptr(core), ptr_arr(U), hist.ctypes.data)
     return hist[:iters]
+
+
+def mlp_param_count(nin, width, nout):
+    return lib().tdx_mlp_param_count(nin, width, nout)
+
+
+def mlp_init(params, nin, width, nout, seed):
+    _call("tdx_mlp_init", ptr(params), nin, width, nout, seed)
+
+
+def mlp_forward(params, x, batch, nin, width, nout, y):
+    _call("tdx_mlp_forward", ptr(params), ptr(x), batch, nin, width, nout, ptr(y))
+
+
+def mlp_backward(params, x, t, batch, nin, width, nout, grad, loss=None):
+    _call("tdx_mlp_backward", ptr(params), ptr(x), ptr(t), batch, nin, width, nout, ptr(grad), ptr(loss))
+
+
+def mlp_adam(params, grad, m, v, step, lr, b1=0.9, b2=0.999, eps=1e-8):
+    _call("tdx_mlp_adam", ptr(params), ptr(grad), ptr(m), ptr(v), params.numel(), step, lr, b1, b2, eps)
+
+
+def mlp_train(params, X, T, nsamples, nin, width, nout, epochs, batch, lr, seed):
+    import numpy as np
+    hist = np.zeros(max(epochs, 1), np.float32)
+    _call("tdx_mlp_train", ptr(params), ptr(X), ptr(T), nsamples, nin, width, nout, epochs, batch, lr, seed,
+          hist.ctypes.data)
+    return hist[:epochs]
 
 
 def synchronize():

@@ -134,9 +134,34 @@ typedef struct {
     uint64_t seed;           /* init seed (reference: 1234) */
     double tol;              /* stop when |fit change| < tol (0: run all sweeps) */
     int record_fit;          /* compute the fast-fit relative error every sweep */
-    int learned;             /* 1: use the learned extrapolation step (tdx_mlp_*), 0: plain ALS */
+    int line_search;         /* extrapolation after every sweep, U_a = U_start + alpha (U - U_start),
+                                accepted only if its exact relative error is lower (Tensorlab
+                                cpd_als.m:234-339): 0 none (plain ALS), 1 alpha from the MLP
+                                (mlp_params), 2 alpha = (sweep+1)^(1/3) (cpd_lsb.m:30),
+                                3 no step, record the relative error at every grid alpha */
     int use_graph;           /* capture each sweep in a hipGraph (default 1) */
+    const float *mlp_params; /* device MLP parameters, nin = TDX_LS_FEATURES, nout = 1 */
+    int mlp_width;
+    int ngrid;               /* line_search == 3: number of grid points (<= 8) */
+    double grid[8];
+    struct tdx_cp_trace *trace; /* optional per-sweep host outputs */
+    double target_relerr;    /* stop once the relative error is <= this (0: off) */
 } tdx_cp_options;
+
+/* features of the extrapolation MLP, computed on the device after every sweep k:
+ * [0] log10 relerr_k, [1..3] log10(relerr_{k-j}/relerr_{k-j+1}), [4..7] log10 of the relative
+ * factor change ||U_m - U_start,m|| / ||U_m|| per mode (0 beyond the order), [8] log10(k+1),
+ * [9] 1 for order 4, [10] log of the last accepted alpha (0 if none), [11] accept rate.
+ * MLP output beta gives alpha = exp(clamp(beta, 0, log 32)). */
+#define TDX_LS_FEATURES 12
+
+typedef struct tdx_cp_trace {
+    float *alpha;            /* [sweeps] proposed alpha (line_search 1, 2) */
+    int *accepted;           /* [sweeps] 1 if the extrapolated point was kept */
+    float *features;         /* [sweeps * TDX_LS_FEATURES] */
+    double *grid_relerr;     /* [sweeps * ngrid] relative error at each grid alpha (line_search 3) */
+    double *plain_relerr;    /* [sweeps] relative error of the plain ALS update of every sweep */
+} tdx_cp_trace;
 
 typedef struct {
     int sweeps_done;
@@ -160,6 +185,25 @@ int tdx_cp_als4(const float *X, float *A, float *B, float *C, float *D, long a, 
  * with FP32 accumulation, FP64 solve, 2 FP32 refinement sweeps (the reference uses FP16 and
  * no refinement); caller buffers are never freed (unlike src/tensorcore_cp.cu:136-139). */
 int tdx_cp_tensorcore(const float *X, float *A, float *B, float *C, long a, long b, long c, long r);
+
+/* ---- learned ALS extrapolation: the MLP (no reference counterpart; the step it
+ *      replaces is the line search of Tensorlab cpd_als.m:165-166,234-339) ---- */
+/* parameter count of y = W3 gelu(W2 gelu(W1 x + b1) + b2) + b3, or -1 for a bad shape
+ * (nin <= 64, width <= 128, nout <= 16); parameters are one FP32 device buffer laid out
+ * [W1 (width x nin, row-major) | b1 | W2 (width x width) | b2 | W3 (nout x width) | b3] */
+long tdx_mlp_param_count(long nin, long width, long nout);
+int tdx_mlp_init(float *params, long nin, long width, long nout, uint64_t seed);      /* Glorot uniform, zero bias */
+int tdx_mlp_forward(const float *params, const float *x, long batch, long nin, long width, long nout, float *y);
+/* gradient of loss = mean((y - t)^2)/2 over batch*nout into grad (same layout as params);
+ * loss (device float, may be NULL) receives the loss */
+int tdx_mlp_backward(const float *params, const float *x, const float *t, long batch, long nin, long width,
+                     long nout, float *grad, float *loss);
+/* one Adam step (Kingma & Ba 2015), step >= 1 */
+int tdx_mlp_adam(float *params, const float *grad, float *m, float *v, long n, long step, float lr, float b1,
+                 float b2, float eps);
+/* minibatch Adam training on device arrays X (nsamples x nin), T (nsamples x nout) */
+int tdx_mlp_train(float *params, const float *X, const float *T, long nsamples, long nin, long width, long nout,
+                  int epochs, long batch, float lr, uint64_t seed, float *loss_hist);
 
 /* ---- Tucker ---- */
 /* tucker_hosvd: src/tucker_hosvd.cu:3-103 */
