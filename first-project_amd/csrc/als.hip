@@ -39,72 +39,16 @@ __global__ __launch_bounds__(256) void k_vinv(const double *__restrict__ G0, con
     __shared__ double sh[256];
     __shared__ double cs[4 * (kMaxR / 2 + 1)];
     __shared__ int fail;
-    const int tid = threadIdx.x, nt = blockDim.x, RR = R * R;
     const double *G[4] = {G0, G1, G2, G3};
-    for (int e = tid; e < RR; e += nt) {
+    for (int e = threadIdx.x; e < R * R; e += blockDim.x) {
         double v = 1.0;
         for (int m = 0; m < order; ++m)
             if (m != mode) v *= G[m][e];
         V[e] = v;
-        L[e] = v;
     }
-    if (tid == 0) fail = 0;
     __syncthreads();
-    // right-looking Cholesky on the lower triangle of L
-    for (int j = 0; j < R; ++j) {
-        if (tid == 0) {
-            double d = L[j + j * R];
-            if (!(d > 0.0) || !isfinite(d)) fail = 1;
-            L[j + j * R] = sqrt(fmax(d, 0.0));
-        }
-        __syncthreads();
-        if (fail) break;
-        const double d = L[j + j * R];
-        for (int i = j + 1 + tid; i < R; i += nt) L[i + j * R] /= d;
-        __syncthreads();
-        const int m = R - j - 1;
-        for (int e = tid; e < m * m; e += nt) {
-            int i = j + 1 + e % m, k = j + 1 + e / m;
-            if (k <= i) L[i + k * R] -= L[i + j * R] * L[k + j * R];
-        }
-        __syncthreads();
-    }
-    if (!fail) {
-        // W = L^{-1} (lower): column c by forward substitution, one thread per column
-        for (int c = tid; c < R; c += nt) {
-            for (int i = 0; i < c; ++i) W[i + c * R] = 0.0;
-            for (int i = c; i < R; ++i) {
-                double s = (i == c) ? 1.0 : 0.0;
-                for (int k = c; k < i; ++k) s -= L[i + k * R] * W[k + c * R];
-                W[i + c * R] = s / L[i + i * R];
-            }
-        }
-        __syncthreads();
-        // V^{-1} = W^T W
-        for (int e = tid; e < RR; e += nt) {
-            int a = e % R, b = e / R, k0 = a > b ? a : b;
-            double s = 0.0;
-            for (int k = k0; k < R; ++k) s += W[k + a * R] * W[k + b * R];
-            Vinv[e] = s;
-        }
-        return;
-    }
-    // pseudo-inverse fallback: V = Q diag(w) Q^T
-    for (int e = tid; e < RR; e += nt) L[e] = V[e];
-    __syncthreads();
-    jacobi_eig_block(L, W, R, cs, sh);
-    double wmax = 0.0;
-    for (int a = 0; a < R; ++a) wmax = fmax(wmax, fabs(L[a + a * R]));
-    for (int e = tid; e < RR; e += nt) {
-        int a = e % R, b = e / R;
-        double s = 0.0;
-        for (int k = 0; k < R; ++k) {
-            double lam = L[k + k * R];
-            if (fabs(lam) > 1e-12 * wmax) s += W[a + k * R] * W[b + k * R] / lam;
-        }
-        Vinv[e] = s;
-    }
-    if (tid == 0) atomicOr(status, 1);
+    const int fb = block_spd_inverse(V, L, W, Vinv, R, cs, sh, &fail);
+    if (fb && threadIdx.x == 0) atomicOr(status, 1);
 }
 
 // U_n rows = (sum of split partials) V^{-1}.  One workgroup per 64 rows, 256 threads,

@@ -1,0 +1,447 @@
+// batched.hip -- CP-ALS of thousands of small, uneven 3rd-order tensors in one persistent
+// kernel per sweep (no reference counterpart: the reference decomposes one tensor per call,
+// src/cp_als.cu:3).
+//
+// Each workgroup pulls tensors from a size-sorted work queue (one atomic counter, reset by
+// a one-thread kernel at the start of every sweep so the sweep can be replayed as a graph)
+// and performs the whole Gauss-Seidel sweep of its tensor locally:
+//   for n in 0..2:  M = MTTKRP_n streamed slab by slab (X(:,:,k0:k0+kc) in LDS, coalesced),
+//                   V = hadamard of the other Grams, FP64 Cholesky inverse (Jacobi
+//                   pseudo-inverse fallback), U_n = M V^{-1}, G_n = U_n^T U_n (FP64)
+//   relerr from the fast-fit identity, appended to the tensor's history.
+// Every tensor element is read from HBM once per mode, exactly like the single-tensor path.
+// Tunables (block size, slab capacity) are template parameters picked by the autotuner.
+#include "common.hpp"
+#include "linalg.hpp"
+#include <algorithm>
+#include <map>
+#include <stdio.h>
+#include <string>
+#include <vector>
+
+namespace tdx {
+
+static constexpr int kBMaxDim = 128;
+static constexpr int kBMaxR = 16;
+static constexpr int kSlabCols = 128;  // columns per slab (per-column scratch Ys)
+
+struct BDesc {
+    long xoff;
+    long uoff[3];
+    int I, J, K, R;
+};
+
+template <int BS, int SLAB>
+__global__ __launch_bounds__(BS) void k_cp_batched_sweep(const float *__restrict__ X, float *__restrict__ U,
+                                                         const BDesc *__restrict__ desc, const int *__restrict__ sched,
+                                                         int ntensors, int *__restrict__ queue,
+                                                         const double *__restrict__ normx2,
+                                                         double *__restrict__ fit_hist, int max_sweeps,
+                                                         const int *__restrict__ sweep_ctr, int *__restrict__ status) {
+    __shared__ float slab[SLAB];
+    __shared__ float Us[3][kBMaxDim * kBMaxR];
+    __shared__ float Ms[kBMaxDim * kBMaxR];
+    __shared__ float Ys[kSlabCols * kBMaxR > kBMaxDim * kBMaxR ? kSlabCols * kBMaxR : kBMaxDim * kBMaxR];
+    __shared__ double Gs[3][kBMaxR * kBMaxR];
+    __shared__ double V[kBMaxR * kBMaxR], L[kBMaxR * kBMaxR], W[kBMaxR * kBMaxR], Vi[kBMaxR * kBMaxR];
+    __shared__ double sh[BS];
+    __shared__ double cs[4 * (kBMaxR / 2 + 1)];
+    __shared__ int fail, slot;
+    const int tid = threadIdx.x;
+    const int sweep = *sweep_ctr;
+    for (;;) {
+        if (tid == 0) slot = atomicAdd(queue, 1);
+        __syncthreads();
+        const int sl = slot;
+        if (sl >= ntensors) break;  // uniform: every wave exits once the queue is drained
+        const int t = sched[sl];
+        const BDesc d = desc[t];
+        const int dims[3] = {d.I, d.J, d.K};
+        const int R = d.R;
+        const float *x = X + d.xoff;
+        // factors to LDS, Grams in FP64
+        for (int m = 0; m < 3; ++m)
+            for (int e = tid; e < dims[m] * R; e += BS) Us[m][e] = U[d.uoff[m] + e];
+        __syncthreads();
+        for (int m = 0; m < 3; ++m)
+            for (int e = tid; e < R * R; e += BS) {
+                const int a = e % R, b = e / R;
+                double s = 0.0;
+                for (int i = 0; i < dims[m]; ++i) s += (double)Us[m][i + a * dims[m]] * (double)Us[m][i + b * dims[m]];
+                Gs[m][e] = s;
+            }
+        __syncthreads();
+        // slabs are runs of consecutive columns c = j + k*J of the mode-1 unfolding
+        // (contiguous in memory); at most kSlabCols columns so the per-column scratch fits
+        const int ncol = d.J * d.K;
+        const int CC = min(SLAB / d.I, kSlabCols);
+        double ip_last = 0.0;
+        for (int n = 0; n < 3; ++n) {
+            const int In = dims[n];
+            for (int e = tid; e < In * R; e += BS) Ms[e] = 0.0f;
+            // -------- MTTKRP, slab by slab --------
+            for (int c0 = 0; c0 < ncol; c0 += CC) {
+                const int cc = min(CC, ncol - c0);
+                for (int e = tid; e < cc * d.I; e += BS) slab[e] = x[(long)c0 * d.I + e];
+                if (n == 0) {
+                    // Khatri-Rao rows of the slab's columns: Ys[q*16 + r] = B[j_q,r] C[k_q,r]
+                    for (int e = tid; e < cc * R; e += BS) {
+                        const int q = e / R, r = e % R, c = c0 + q;
+                        Ys[q * kBMaxR + r] = Us[1][c % d.J + r * d.J] * Us[2][c / d.J + r * d.K];
+                    }
+                }
+                __syncthreads();
+                if (n == 0) {
+                    for (int e = tid; e < d.I * R; e += BS) {
+                        const int i = e % d.I, r = e / d.I;
+                        float acc = 0.0f;
+                        for (int q = 0; q < cc; ++q) acc = fmaf(slab[i + q * d.I], Ys[q * kBMaxR + r], acc);
+                        Ms[e] += acc;
+                    }
+                } else if (n == 1) {
+                    // M[j,r] += sum over the slab's columns with this j of C[k,r] sum_i S[i,c] A[i,r]
+                    for (int e = tid; e < d.J * R; e += BS) {
+                        const int j = e % d.J, r = e / d.J;
+                        float acc = 0.0f;
+                        for (int q = (j - c0 % d.J + d.J) % d.J; q < cc; q += d.J) {
+                            const float *sc = slab + q * d.I;
+                            float a2 = 0.0f;
+                            for (int i = 0; i < d.I; ++i) a2 = fmaf(sc[i], Us[0][i + r * d.I], a2);
+                            acc = fmaf(a2, Us[2][(c0 + q) / d.J + r * d.K], acc);
+                        }
+                        Ms[e] += acc;
+                    }
+                } else {
+                    // Ys[q,r] = B[j_q,r] sum_i S[i,q] A[i,r]; then M[k,r] += sum of Ys over k's columns
+                    for (int e = tid; e < cc * R; e += BS) {
+                        const int q = e % cc, r = e / cc, c = c0 + q;
+                        const float *sc = slab + q * d.I;
+                        float a2 = 0.0f;
+                        for (int i = 0; i < d.I; ++i) a2 = fmaf(sc[i], Us[0][i + r * d.I], a2);
+                        Ys[q * kBMaxR + r] = a2 * Us[1][c % d.J + r * d.J];
+                    }
+                    __syncthreads();
+                    const int kf = c0 / d.J, kl = (c0 + cc - 1) / d.J;
+                    for (int e = tid; e < (kl - kf + 1) * R; e += BS) {
+                        const int k = kf + e / R, r = e % R;
+                        const int qa = max(0, k * d.J - c0), qb = min(cc, (k + 1) * d.J - c0);
+                        float acc = 0.0f;
+                        for (int q = qa; q < qb; ++q) acc += Ys[q * kBMaxR + r];
+                        Ms[k + r * d.K] += acc;
+                    }
+                }
+                __syncthreads();
+            }
+            // -------- normal equations in FP64 --------
+            for (int e = tid; e < R * R; e += BS) {
+                double v = 1.0;
+                for (int m = 0; m < 3; ++m)
+                    if (m != n) v *= Gs[m][e];
+                V[e] = v;
+            }
+            __syncthreads();
+            if (block_spd_inverse(V, L, W, Vi, R, cs, sh, &fail) && tid == 0) atomicOr(status, 1);
+            // U_n = M V^{-1}; <M, U_n> for the fit after the last mode
+            double ip = 0.0;
+            for (int e = tid; e < In * R; e += BS) {
+                const int i = e % In, b = e / In;
+                double s = 0.0;
+                for (int a = 0; a < R; ++a) s += (double)Ms[i + a * In] * Vi[a + b * R];
+                const float u = (float)s;
+                Ys[e] = u;
+                ip += (double)Ms[e] * (double)u;
+            }
+            __syncthreads();
+            for (int e = tid; e < In * R; e += BS) {
+                Us[n][e] = Ys[e];
+                U[d.uoff[n] + e] = Ys[e];
+            }
+            if (n == 2) ip_last = block_sum(ip, sh);
+            __syncthreads();
+            for (int e = tid; e < R * R; e += BS) {
+                const int a = e % R, b = e / R;
+                double s = 0.0;
+                for (int i = 0; i < In; ++i) s += (double)Us[n][i + a * In] * (double)Us[n][i + b * In];
+                Gs[n][e] = s;
+            }
+            __syncthreads();
+        }
+        // fit: ||X - Xhat||^2 = ||X||^2 - 2 <M_2, U_2> + sum(G0 o G1 o G2)
+        double hx = 0.0;
+        for (int e = tid; e < R * R; e += BS) hx += Gs[0][e] * Gs[1][e] * Gs[2][e];
+        hx = block_sum(hx, sh);
+        if (tid == 0 && sweep < max_sweeps) {
+            const double nx2 = normx2[t];
+            fit_hist[(long)t * max_sweeps + sweep] = sqrt(fabs(nx2 - 2.0 * ip_last + hx) / nx2);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_batched_prologue(int *queue) { *queue = 0; }
+__global__ void k_batched_epilogue(int *sweep_ctr) { *sweep_ctr += 1; }
+
+__global__ void k_batched_norm2(const float *__restrict__ X, const BDesc *__restrict__ desc, int n,
+                                double *__restrict__ normx2) {
+    __shared__ double sh[256];
+    for (int t = blockIdx.x; t < n; t += gridDim.x) {
+        const BDesc d = desc[t];
+        const long N = (long)d.I * d.J * d.K;
+        double s = 0.0, c = 0.0;
+        for (long e = threadIdx.x; e < N; e += 256) {
+            const double v = X[d.xoff + e];
+            const double y = v * v - c, tt = s + y;
+            c = (tt - s) - y;
+            s = tt;
+        }
+        s = block_sum(s, sh);
+        if (threadIdx.x == 0) normx2[t] = s;
+    }
+}
+
+// U_m of tensor t <- U(0,1) of Philox stream (seed + t, m)
+__global__ void k_batched_init(float *__restrict__ U, const BDesc *__restrict__ desc, int n, uint64_t seed) {
+    for (int t = blockIdx.x; t < n; t += gridDim.x) {
+        const BDesc d = desc[t];
+        const int dims[3] = {d.I, d.J, d.K};
+        for (int m = 0; m < 3; ++m)
+            for (long e = threadIdx.x; e < (long)dims[m] * d.R; e += blockDim.x)
+                U[d.uoff[m] + e] = word_to_uniform(philox_word(seed + (uint64_t)t, (uint32_t)m, (uint64_t)e));
+    }
+}
+
+// ---------------------------------------------------------------------------
+typedef void (*BatchedKernel)(const float *, float *, const BDesc *, const int *, int, int *, const double *,
+                              double *, int, const int *, int *);
+
+struct Variant {
+    int bs, slab;
+    BatchedKernel fn;
+    const char *name;
+};
+
+static const Variant kVariants[] = {
+    {128, 4096, k_cp_batched_sweep<128, 4096>, "bs128_slab4k"},
+    {256, 4096, k_cp_batched_sweep<256, 4096>, "bs256_slab4k"},
+    {256, 8192, k_cp_batched_sweep<256, 8192>, "bs256_slab8k"},
+    {512, 8192, k_cp_batched_sweep<512, 8192>, "bs512_slab8k"},
+    {256, 12288, k_cp_batched_sweep<256, 12288>, "bs256_slab12k"},
+};
+static constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+
+static int launch_variant(int v, int grid, hipStream_t s, const float *X, float *U, const BDesc *desc,
+                          const int *sched, int n, int *queue, const double *normx2, double *fit, int max_sweeps,
+                          int *sweep_ctr, int *status) {
+    k_batched_prologue<<<1, 1, 0, s>>>(queue);
+    const Variant &V = kVariants[v];
+    hipLaunchKernelGGL(V.fn, dim3(grid), dim3(V.bs), 0, s, X, U, desc, sched, n, queue, normx2, fit, max_sweeps,
+                       sweep_ctr, status);
+    k_batched_epilogue<<<1, 1, 0, s>>>(sweep_ctr);
+    TDX_LAUNCH_CHECK();
+    return TDX_OK;
+}
+
+static int grid_for_variant(int v) {
+    int dev = 0, cus = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    int per_cu = 2;
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)kVariants[v].fn, kVariants[v].bs, 0);
+    if (per_cu < 1) per_cu = 1;
+    return cus * per_cu;
+}
+
+}  // namespace tdx
+
+using namespace tdx;
+
+extern "C" {
+
+int tdx_cp_batched_variants(void) { return kNumVariants; }
+
+const char *tdx_cp_batched_variant_name(int v) {
+    return (v >= 0 && v < kNumVariants) ? kVariants[v].name : "";
+}
+
+// CP-ALS of `ntensors` 3rd-order tensors packed in X (tensor t at xoff[t], column-major
+// dims[3t..3t+2]) with ranks[t] <= 16 and extents <= 128; factors packed in U (mode m of
+// tensor t at uoff[3t+m], I_m x R column-major).  init = 0 draws U(0,1) from Philox stream
+// (seed + t, m); 1 uses the caller's U.  variant < 0 picks variant 1.  fit_hist (host, may
+// be NULL) receives ntensors x sweeps relative errors; sweep_ms (host, may be NULL) the
+// device time of every sweep.  The per-sweep launches are captured once in a hipGraph.
+int tdx_cp_als_batched(const float *X, float *U, const long *dims, const long *ranks, const long *xoff,
+                       const long *uoff, long ntensors, int sweeps, int init, uint64_t seed, int variant,
+                       double *fit_hist, float *sweep_ms) {
+    if (!X || !U || !dims || !ranks || !xoff || !uoff || ntensors <= 0 || sweeps < 0) return record(TDX_ERR_ARG);
+    if (variant >= kNumVariants) return record(TDX_ERR_ARG);
+    if (variant < 0) variant = 1;
+    std::vector<BDesc> h(ntensors);
+    std::vector<std::pair<long, int>> order(ntensors);
+    for (long t = 0; t < ntensors; ++t) {
+        BDesc &d = h[t];
+        d.I = (int)dims[3 * t];
+        d.J = (int)dims[3 * t + 1];
+        d.K = (int)dims[3 * t + 2];
+        d.R = (int)ranks[t];
+        if (d.I <= 0 || d.J <= 0 || d.K <= 0 || d.I > kBMaxDim || d.J > kBMaxDim || d.K > kBMaxDim || d.R <= 0 ||
+            d.R > kBMaxR)
+            return record(TDX_ERR_UNSUPPORTED);
+        d.xoff = xoff[t];
+        for (int m = 0; m < 3; ++m) d.uoff[m] = uoff[3 * t + m];
+        order[t] = {-(long)d.I * d.J * d.K * d.R, (int)t};  // largest work first
+    }
+    std::sort(order.begin(), order.end());
+    std::vector<int> sched(ntensors);
+    for (long t = 0; t < ntensors; ++t) sched[t] = order[t].second;
+
+    hipStream_t s = stream();
+    hipStream_t owned = nullptr;
+    if (!s) {
+        TDX_HIP(hipStreamCreate(&owned));
+        s = owned;
+    }
+    BDesc *d_desc = nullptr;
+    int *d_sched = nullptr, *d_int = nullptr;
+    double *d_nx = nullptr, *d_fit = nullptr;
+    struct Guard {
+        hipStream_t st;
+        void *p[5];
+        ~Guard() {
+            if (st) (void)hipStreamSynchronize(st);
+            for (void *q : p)
+                if (q) (void)hipFree(q);
+            if (st) (void)hipStreamDestroy(st);
+        }
+    } guard{owned, {nullptr, nullptr, nullptr, nullptr, nullptr}};
+    TDX_HIP(hipMalloc(&d_desc, sizeof(BDesc) * ntensors));
+    guard.p[0] = d_desc;
+    TDX_HIP(hipMalloc(&d_sched, sizeof(int) * ntensors));
+    guard.p[1] = d_sched;
+    TDX_HIP(hipMalloc(&d_int, sizeof(int) * 4));
+    guard.p[2] = d_int;
+    TDX_HIP(hipMalloc(&d_nx, sizeof(double) * ntensors));
+    guard.p[3] = d_nx;
+    const int S = sweeps > 0 ? sweeps : 1;
+    TDX_HIP(hipMalloc(&d_fit, sizeof(double) * ntensors * S));
+    guard.p[4] = d_fit;
+    TDX_HIP(hipMemcpyAsync(d_desc, h.data(), sizeof(BDesc) * ntensors, hipMemcpyHostToDevice, s));
+    TDX_HIP(hipMemcpyAsync(d_sched, sched.data(), sizeof(int) * ntensors, hipMemcpyHostToDevice, s));
+    int *queue = d_int, *sweep_ctr = d_int + 1, *status = d_int + 2;
+    k_batched_prologue<<<1, 1, 0, s>>>(sweep_ctr);
+    k_batched_prologue<<<1, 1, 0, s>>>(status);
+    if (init == 0) k_batched_init<<<1024, 256, 0, s>>>(U, d_desc, (int)ntensors, seed);
+    k_batched_norm2<<<1024, 256, 0, s>>>(X, d_desc, (int)ntensors, d_nx);
+    TDX_LAUNCH_CHECK();
+    const int grid = grid_for_variant(variant);
+    // capture one sweep
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    TDX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int st = launch_variant(variant, grid, s, X, U, d_desc, d_sched, (int)ntensors, queue, d_nx, d_fit, S,
+                            sweep_ctr, status);
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (st) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess) return record_hip(e, "hipStreamEndCapture", __FILE__, __LINE__);
+    e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return record_hip(e, "hipGraphInstantiate", __FILE__, __LINE__);
+    }
+    std::vector<hipEvent_t> evs(sweeps + 1);
+    for (auto &ev : evs) (void)hipEventCreate(&ev);
+    (void)hipEventRecord(evs[0], s);
+    for (int it = 0; it < sweeps && e == hipSuccess; ++it) {
+        e = hipGraphLaunch(ge, s);
+        if (e == hipSuccess) e = hipEventRecord(evs[it + 1], s);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e == hipSuccess && sweep_ms)
+        for (int it = 0; it < sweeps; ++it) (void)hipEventElapsedTime(&sweep_ms[it], evs[it], evs[it + 1]);
+    for (auto &ev : evs) (void)hipEventDestroy(ev);
+    (void)hipGraphExecDestroy(ge);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return record_hip(e, "batched sweeps", __FILE__, __LINE__);
+    if (fit_hist && sweeps > 0)
+        TDX_HIP(hipMemcpy(fit_hist, d_fit, sizeof(double) * ntensors * S, hipMemcpyDeviceToHost));
+    int hs = 0;
+    TDX_HIP(hipMemcpy(&hs, status, sizeof(int), hipMemcpyDeviceToHost));
+    return record(hs ? TDX_WARN_SINGULAR : TDX_OK);
+}
+
+// Autotuner: times `trial_sweeps` sweeps of every variant on copies of the initial factors
+// (the caller's U is left as given) and returns the fastest variant in *best; the timings
+// of all variants go to ms_per_sweep (host, tdx_cp_batched_variants() entries).  Results
+// are cached in `cache_path` (a text file of "key variant ms" lines; NULL: no cache) under a
+// key built from the batch shape and the device name.
+int tdx_cp_batched_autotune(const float *X, const float *U, const long *dims, const long *ranks, const long *xoff,
+                            const long *uoff, long ntensors, long u_floats, int trial_sweeps, const char *cache_path,
+                            int *best, float *ms_per_sweep) {
+    if (!X || !U || !best || ntensors <= 0 || u_floats <= 0 || trial_sweeps <= 0) return record(TDX_ERR_ARG);
+    // cache key: device name + tensor count + total elements + rank histogram
+    hipDeviceProp_t prop;
+    int dev = 0;
+    TDX_HIP(hipGetDevice(&dev));
+    TDX_HIP(hipGetDeviceProperties(&prop, dev));
+    long tot = 0, rh[17] = {0};
+    for (long t = 0; t < ntensors; ++t) {
+        tot += dims[3 * t] * dims[3 * t + 1] * dims[3 * t + 2];
+        if (ranks[t] >= 0 && ranks[t] <= 16) rh[ranks[t]]++;
+    }
+    char key[512];
+    snprintf(key, sizeof(key), "%s|n%ld|e%ld|r4:%ld,r8:%ld,r16:%ld", prop.gcnArchName, ntensors, tot, rh[4], rh[8],
+             rh[16]);
+    if (cache_path) {
+        FILE *f = fopen(cache_path, "r");
+        if (f) {
+            char line[640];
+            while (fgets(line, sizeof(line), f)) {
+                char k[512];
+                int v = -1;
+                float ms = 0;
+                if (sscanf(line, "%511s %d %f", k, &v, &ms) == 3 && std::string(k) == key && v >= 0 &&
+                    v < kNumVariants) {
+                    *best = v;
+                    fclose(f);
+                    return record(TDX_OK);
+                }
+            }
+            fclose(f);
+        }
+    }
+    float *Uw = nullptr;
+    TDX_HIP(hipMalloc(&Uw, sizeof(float) * u_floats));
+    float best_ms = 1e30f;
+    int bv = 1, rc = TDX_OK;
+    std::vector<float> ms(trial_sweeps);
+    for (int v = 0; v < kNumVariants && !rc; ++v) {
+        rc = (hipMemcpy(Uw, U, sizeof(float) * u_floats, hipMemcpyDeviceToDevice) == hipSuccess) ? TDX_OK
+                                                                                              : record(TDX_ERR_HIP);
+        if (rc) break;
+        rc = tdx_cp_als_batched(X, Uw, dims, ranks, xoff, uoff, ntensors, trial_sweeps, 1, 0, v, nullptr, ms.data());
+        if (rc == TDX_WARN_SINGULAR) rc = TDX_OK;
+        float m = 0;
+        for (float x : ms) m += x;
+        m /= (float)trial_sweeps;
+        if (ms_per_sweep) ms_per_sweep[v] = m;
+        if (m < best_ms) {
+            best_ms = m;
+            bv = v;
+        }
+    }
+    (void)hipFree(Uw);
+    if (rc) return rc;
+    *best = bv;
+    if (cache_path) {
+        FILE *f = fopen(cache_path, "a");
+        if (f) {
+            fprintf(f, "%s %d %.5f\n", key, bv, best_ms);
+            fclose(f);
+        }
+    }
+    return record(TDX_OK);
+}
+
+}  // extern "C"

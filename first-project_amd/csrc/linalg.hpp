@@ -93,4 +93,70 @@ __device__ inline void jacobi_eig_block(double *A, double *Q, int n, double *cs,
     }
 }
 
+// Vinv = V^{-1} for a symmetric R x R V (all arrays column-major, R*R doubles, any memory
+// space visible to the whole workgroup; V is preserved).  Right-looking Cholesky on L,
+// W = L^{-1}, Vinv = W^T W.  If V is not numerically SPD, a pseudo-inverse from the Jacobi
+// eigendecomposition (eigenvalues below 1e-12 * max dropped) is returned instead and the
+// function returns 1 (in every thread).  cs needs 4*(R/2+1) doubles, sh blockDim doubles.
+__device__ inline int block_spd_inverse(const double *V, double *L, double *W, double *Vinv, int R, double *cs,
+                                        double *sh, int *fail) {
+    const int tid = threadIdx.x, nt = blockDim.x, RR = R * R;
+    for (int e = tid; e < RR; e += nt) L[e] = V[e];
+    if (tid == 0) *fail = 0;
+    __syncthreads();
+    for (int j = 0; j < R; ++j) {
+        if (tid == 0) {
+            const double d = L[j + j * R];
+            if (!(d > 0.0) || !isfinite(d)) *fail = 1;
+            L[j + j * R] = sqrt(fmax(d, 0.0));
+        }
+        __syncthreads();
+        if (*fail) break;
+        const double d = L[j + j * R];
+        for (int i = j + 1 + tid; i < R; i += nt) L[i + j * R] /= d;
+        __syncthreads();
+        const int m = R - j - 1;
+        for (int e = tid; e < m * m; e += nt) {
+            const int i = j + 1 + e % m, k = j + 1 + e / m;
+            if (k <= i) L[i + k * R] -= L[i + j * R] * L[k + j * R];
+        }
+        __syncthreads();
+    }
+    if (!*fail) {
+        for (int c = tid; c < R; c += nt) {
+            for (int i = 0; i < c; ++i) W[i + c * R] = 0.0;
+            for (int i = c; i < R; ++i) {
+                double s = (i == c) ? 1.0 : 0.0;
+                for (int k = c; k < i; ++k) s -= L[i + k * R] * W[k + c * R];
+                W[i + c * R] = s / L[i + i * R];
+            }
+        }
+        __syncthreads();
+        for (int e = tid; e < RR; e += nt) {
+            const int a = e % R, b = e / R, k0 = a > b ? a : b;
+            double s = 0.0;
+            for (int k = k0; k < R; ++k) s += W[k + a * R] * W[k + b * R];
+            Vinv[e] = s;
+        }
+        __syncthreads();
+        return 0;
+    }
+    for (int e = tid; e < RR; e += nt) L[e] = V[e];
+    __syncthreads();
+    jacobi_eig_block(L, W, R, cs, sh);
+    double wmax = 0.0;
+    for (int a = 0; a < R; ++a) wmax = fmax(wmax, fabs(L[a + a * R]));
+    for (int e = tid; e < RR; e += nt) {
+        const int a = e % R, b = e / R;
+        double s = 0.0;
+        for (int k = 0; k < R; ++k) {
+            const double lam = L[k + k * R];
+            if (fabs(lam) > 1e-12 * wmax) s += W[a + k * R] * W[b + k * R] / lam;
+        }
+        Vinv[e] = s;
+    }
+    __syncthreads();
+    return 1;
+}
+
 }  // namespace tdx

@@ -96,6 +96,10 @@ _SIGS = {
     "tdx_tucker_tensorcore": [_P, _P, _P, _P, _P, _L, _L, _L, _L, _L, _L],
     "tdx_tucker_hooi": [_P, ctypes.c_int, ctypes.POINTER(_L), ctypes.POINTER(_L), ctypes.c_int, ctypes.c_int,
                         _P, ctypes.POINTER(_P), _P],
+    "tdx_cp_als_batched": [_P, _P, _P, _P, _P, _P, _L, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_int,
+                           _P, _P],
+    "tdx_cp_batched_autotune": [_P, _P, _P, _P, _P, _P, _L, _L, ctypes.c_int, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int),
+                                _P],
     "tdx_mlp_init": [_P, _L, _L, _L, ctypes.c_uint64],
     "tdx_mlp_forward": [_P, _P, _L, _L, _L, _L, _P],
     "tdx_mlp_backward": [_P, _P, _P, _L, _L, _L, _L, _P, _P],
@@ -105,7 +109,8 @@ _SIGS = {
 
 # every symbol include/tdx.h declares (checked against the built library by the tests)
 HEADER_SYMBOLS = sorted(list(_SIGS) + ["tdx_get_stream", "tdx_status_string", "tdx_cp_default_options",
-                                        "tdx_mlp_param_count"])
+                                        "tdx_mlp_param_count", "tdx_cp_batched_variants",
+                                        "tdx_cp_batched_variant_name"])
 
 
 def lib():
@@ -127,6 +132,10 @@ def lib():
         L.tdx_cp_default_options.restype = None
         L.tdx_mlp_param_count.restype = _L
         L.tdx_mlp_param_count.argtypes = [_L, _L, _L]
+        L.tdx_cp_batched_variants.restype = ctypes.c_int
+        L.tdx_cp_batched_variants.argtypes = []
+        L.tdx_cp_batched_variant_name.restype = ctypes.c_char_p
+        L.tdx_cp_batched_variant_name.argtypes = [ctypes.c_int]
         _LIB = L
     return _LIB
 
@@ -332,6 +341,54 @@ def tucker_hooi(X, dims, ranks, iters, core, U, bf16_ttm=False):
     _call("tdx_tucker_hooi", ptr(X), len(dims), dims_arr(dims), dims_arr(ranks), iters, 1 if bf16_ttm else 0,
           ptr(core), ptr_arr(U), hist.ctypes.data)
     return hist[:iters]
+
+
+class Batch:
+    """Packed layout of many small 3rd-order tensors: shapes (n x 3), ranks (n), and the
+    element offsets of each tensor and factor in the packed buffers."""
+
+    def __init__(self, shapes, ranks):
+        import numpy as np
+        self.shapes = np.asarray(shapes, np.int64).reshape(-1, 3)
+        self.ranks = np.asarray(ranks, np.int64)
+        n = len(self.ranks)
+        sizes = np.prod(self.shapes, axis=1)
+        self.xoff = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+        self.x_floats = int(sizes.sum())
+        usz = (self.shapes * self.ranks[:, None]).ravel()
+        self.uoff = np.concatenate([[0], np.cumsum(usz)[:-1]]).astype(np.int64)
+        self.u_floats = int(usz.sum())
+        self.n = n
+
+    def _args(self):
+        return (self.shapes.ravel().ctypes.data, self.ranks.ctypes.data, self.xoff.ctypes.data,
+                self.uoff.ctypes.data)
+
+
+def cp_als_batched(X, U, batch, sweeps, init=0, seed=0, variant=-1, want_fit=True, want_ms=True):
+    import numpy as np
+    fit = np.zeros((batch.n, max(sweeps, 1)), np.float64)
+    ms = np.zeros(max(sweeps, 1), np.float32)
+    shapes = np.ascontiguousarray(batch.shapes.ravel())
+    st = _call_status("tdx_cp_als_batched", ptr(X), ptr(U), shapes.ctypes.data, batch.ranks.ctypes.data,
+                      batch.xoff.ctypes.data, batch.uoff.ctypes.data, batch.n, sweeps, init, seed, variant,
+                      fit.ctypes.data if want_fit else None, ms.ctypes.data if want_ms else None)
+    if st not in (TDX_OK, TDX_WARN_SINGULAR):
+        raise TdxError(st, "tdx_cp_als_batched")
+    return st, fit[:, :sweeps], ms[:sweeps]
+
+
+def cp_batched_autotune(X, U, batch, trial_sweeps=2, cache_path=None):
+    import numpy as np
+    best = ctypes.c_int()
+    nv = lib().tdx_cp_batched_variants()
+    ms = np.zeros(nv, np.float32)
+    shapes = np.ascontiguousarray(batch.shapes.ravel())
+    _call("tdx_cp_batched_autotune", ptr(X), ptr(U), shapes.ctypes.data, batch.ranks.ctypes.data,
+          batch.xoff.ctypes.data, batch.uoff.ctypes.data, batch.n, batch.u_floats, trial_sweeps,
+          cache_path.encode() if cache_path else None, ctypes.byref(best), ms.ctypes.data)
+    names = [lib().tdx_cp_batched_variant_name(v).decode() for v in range(nv)]
+    return best.value, dict(zip(names, ms.tolist()))
 
 
 def mlp_param_count(nin, width, nout):

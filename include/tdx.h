@@ -186,6 +186,28 @@ int tdx_cp_als4(const float *X, float *A, float *B, float *C, float *D, long a, 
  * no refinement); caller buffers are never freed (unlike src/tensorcore_cp.cu:136-139). */
 int tdx_cp_tensorcore(const float *X, float *A, float *B, float *C, long a, long b, long c, long r);
 
+/* ---- batched CP-ALS (no reference counterpart: the reference decomposes one tensor per
+ *      call, src/cp_als.cu:3) ---- */
+/* CP-ALS of `ntensors` 3rd-order tensors packed in X (tensor t at element offset xoff[t],
+ * column-major, extents dims[3t..3t+2] <= 128) with ranks[t] <= 16; factors packed in U (mode
+ * m of tensor t at uoff[3t+m], I_m x R column-major).  One persistent kernel per sweep pulls
+ * tensors from a size-sorted queue; the sweep is captured in a hipGraph.  init = 0 draws
+ * U(0,1) from Philox stream (seed + t, m), 1 uses the caller's U.  variant: kernel variant
+ * (see tdx_cp_batched_autotune), < 0 for the default.  fit_hist (host, ntensors x sweeps,
+ * may be NULL) receives the fast-fit relative errors; sweep_ms (host, may be NULL) the device
+ * time of every sweep. */
+int tdx_cp_als_batched(const float *X, float *U, const long *dims, const long *ranks, const long *xoff,
+                       const long *uoff, long ntensors, int sweeps, int init, uint64_t seed, int variant,
+                       double *fit_hist, float *sweep_ms);
+int tdx_cp_batched_variants(void);
+const char *tdx_cp_batched_variant_name(int variant);
+/* times trial_sweeps sweeps of every variant on a copy of U (U is not modified), returns the
+ * fastest in *best and all timings in ms_per_sweep (host, may be NULL); the choice is cached
+ * in cache_path ("key variant ms" lines, key = device + batch shape; NULL: no cache) */
+int tdx_cp_batched_autotune(const float *X, const float *U, const long *dims, const long *ranks, const long *xoff,
+                            const long *uoff, long ntensors, long u_floats, int trial_sweeps, const char *cache_path,
+                            int *best, float *ms_per_sweep);
+
 /* ---- learned ALS extrapolation: the MLP (no reference counterpart; the step it
  *      replaces is the line search of Tensorlab cpd_als.m:165-166,234-339) ---- */
 /* parameter count of y = W3 gelu(W2 gelu(W1 x + b1) + b2) + b3, or -1 for a bad shape
