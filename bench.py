@@ -61,6 +61,71 @@ def kernels_only(args, X, dims, R, U):
     print(json.dumps({"kernels_only_ms": res, "copy_floats": 1 << 28}))
 
 
+def extras(args, log):
+    """Secondary configurations: C1 (128^3 plain ALS, 50 sweeps), C3 (128^4 Tucker HOSVD + 10
+    HOOI), C4 (4096-tensor batched ALS, 30 sweeps, vs a per-tensor launch loop on a subsample)."""
+    import numpy as np
+    import torch
+
+    import tdx
+    import workloads as W
+    out = {}
+    # C1
+    log("C1: 128^3 rank 16, 50 plain FP32 sweeps")
+    X, dims, R = W.c1()
+    U = W.uniform_init(dims, R)
+    o = tdx.default_cp_options()
+    o.max_sweeps, o.init = 50, 1
+    st, rep, fit, ms = tdx.cp_als_ex(X, dims, R, U, o, want_ms=True)
+    out["c1"] = {"ms_per_sweep": round(float(np.mean(ms)), 4), "sweeps": 50, "relerr_final": float(fit[-1]),
+                 "us_per_sweep_median": round(float(np.median(ms)) * 1e3, 1)}
+    del X
+    # C3
+    log("C3: Tucker 128^4, core 32^4, HOSVD + 10 HOOI")
+    X, dims, ranks = W.c3()
+    Ud = [torch.empty(d * r, device="cuda") for d, r in zip(dims, ranks)]
+    core = torch.empty(int(np.prod(ranks)), device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist = tdx.tucker_hooi(X, dims, ranks, 10, core, Ud)
+    torch.cuda.synchronize()
+    out["c3"] = {"seconds_hosvd_plus_10_hooi": round(time.perf_counter() - t0, 4),
+                 "relerr_per_iter": [float(h) for h in hist], "noise": 5e-3}
+    del X
+    # C4
+    log("C4: 4096 tensors, batched ALS")
+    X, b = W.c4()
+    U = torch.empty(b.u_floats, device="cuda")
+    tdx.cp_als_batched(X, U, b, 0, init=0, seed=1)
+    cache = os.path.join(ROOT, "gpurun_out", "batched_tuning.txt")
+    os.makedirs(os.path.dirname(cache), exist_ok=True)
+    best, times = tdx.cp_batched_autotune(X, U, b, trial_sweeps=2, cache_path=cache)
+    st, fit, ms = tdx.cp_als_batched(X, U, b, 30, init=0, seed=1, variant=best)
+    loop_n = 64
+    idx = np.linspace(0, b.n - 1, loop_n).astype(int)
+    loop_s = 0.0
+    for t in idx:
+        dims = tuple(int(v) for v in b.shapes[t])
+        R = int(b.ranks[t])
+        Ut = [torch.empty(d * R, device="cuda") for d in dims]
+        xt = X[int(b.xoff[t]):int(b.xoff[t]) + int(np.prod(dims))]
+        oo = tdx.default_cp_options()
+        oo.max_sweeps, oo.record_fit = 30, 1
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tdx.cp_als_ex(xt, dims, R, Ut, oo)
+        torch.cuda.synchronize()
+        loop_s += time.perf_counter() - t0
+    out["c4"] = {"tensors": b.n, "elements": b.x_floats, "sweeps": 30, "batched_s": round(float(np.sum(ms)) / 1e3, 5),
+                 "batched_ms_per_sweep": round(float(np.mean(ms)), 4), "variant": best, "autotune_ms": times,
+                 "batched_GBps_per_sweep": round(3 * 4 * b.x_floats / (float(np.mean(ms)) * 1e-3) / 1e9, 1),
+                 "per_tensor_loop_s_extrapolated": round(loop_s * b.n / loop_n, 3),
+                 "per_tensor_loop_sample": loop_n,
+                 "median_relerr_final": float(np.median(fit[:, -1]))}
+    del X, U
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +135,7 @@ def main():
     ap.add_argument("--rank", type=int, default=64)
     ap.add_argument("--mttkrp-iters", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the secondary configurations")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the MTTKRP launches (3 storages x 3 modes) and a 1 GiB plain copy, for counter runs")
     args = ap.parse_args()
@@ -203,6 +269,8 @@ def main():
                "sample": f"1 FP64 ALS sweep (3 MTTKRPs + Cholesky solves) of the {n}^3 rank-{R} tensor on "
                          f"{threads} OpenMP threads = {cpu_sweep:.2f} s, x {sweeps_needed} sweeps to target"}
 
+    extra = None if args.no_extra else extras(args, log)
+
     bf = mt["bf16"]
     alg_b = 2 * N + 4 * R * 2 * n + 4 * n * R
     avg_b_ms = float(np.mean(bf["ms_per_mode"]))
@@ -235,6 +303,7 @@ def main():
                      "traffic": None, "kernel": "k_mttkrp BF16 storage (mean of the 3 modes)",
                      "bytes_per_launch": alg_b},
         "cpu_baseline": cpu,
+        "other_configs": extra,
     }
     print(json.dumps(out))
 
