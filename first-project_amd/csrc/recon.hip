@@ -158,7 +158,11 @@ __global__ void k_ratio(const double *__restrict__ part, int nparts, double *__r
 //   Y[p + a*P + q*P*Ra] = sum_{i < In} U[a*lda + i*ldi] * X[p + i*P + q*P*In]
 // (reference ttm, src/ttm.cu:21,29,37, with lda = 1, ldi = d; transposed factor with
 //  lda = In, ldi = 1; Tucker expansion core -> tensor with lda = 1, ldi = I).
-__global__ __launch_bounds__(256) void k_ttm_generic(const float *__restrict__ X, long P, long In, long Q,
+__device__ __forceinline__ float ld_elem(const float *p, long i) { return p[i]; }
+__device__ __forceinline__ float ld_elem(const uint16_t *p, long i) { return __uint_as_float((uint32_t)p[i] << 16); }
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_ttm_generic(const T *__restrict__ X, long P, long In, long Q,
                                                      const float *__restrict__ U, long lda, long ldi, long Ra,
                                                      float *__restrict__ Y) {
     const long n = P * Ra * Q;
@@ -166,16 +170,17 @@ __global__ __launch_bounds__(256) void k_ttm_generic(const float *__restrict__ X
         const long p = e % P;
         const long a = (e / P) % Ra;
         const long q = e / (P * Ra);
-        const float *x = X + p + q * P * In;
+        const T *x = X + p + q * P * In;
         const float *u = U + a * lda;
         float acc = 0.0f;
-        for (long i = 0; i < In; ++i) acc = fmaf(u[i * ldi], x[i * P], acc);
+        for (long i = 0; i < In; ++i) acc = fmaf(u[i * ldi], ld_elem(x, i * P), acc);
         Y[e] = acc;
     }
 }
 
-int ttm_generic(const float *X, int order, const long *dims, int mode, const float *U, long lda, long ldi, long Ra,
-                float *Y, hipStream_t s) {
+// X is FP32 (xtype TDX_F32) or BF16 bit patterns (TDX_BF16)
+int ttm_generic_t(const void *X, int xtype, int order, const long *dims, int mode, const float *U, long lda, long ldi,
+                  long Ra, float *Y, hipStream_t s) {
     long P = 1, Q = 1;
     for (int m = 0; m < mode; ++m) P *= dims[m];
     for (int m = mode + 1; m < order; ++m) Q *= dims[m];
@@ -183,9 +188,17 @@ int ttm_generic(const float *X, int order, const long *dims, int mode, const flo
     long g = ceil_div(n, 256);
     if (g > 65536) g = 65536;
     if (g < 1) g = 1;
-    k_ttm_generic<<<(int)g, 256, 0, s>>>(X, P, dims[mode], Q, U, lda, ldi, Ra, Y);
+    if (xtype == TDX_BF16)
+        k_ttm_generic<uint16_t><<<(int)g, 256, 0, s>>>((const uint16_t *)X, P, dims[mode], Q, U, lda, ldi, Ra, Y);
+    else
+        k_ttm_generic<float><<<(int)g, 256, 0, s>>>((const float *)X, P, dims[mode], Q, U, lda, ldi, Ra, Y);
     TDX_LAUNCH_CHECK();
     return TDX_OK;
+}
+
+int ttm_generic(const float *X, int order, const long *dims, int mode, const float *U, long lda, long ldi, long Ra,
+                float *Y, hipStream_t s) {
+    return ttm_generic_t(X, TDX_F32, order, dims, mode, U, lda, ldi, Ra, Y, s);
 }
 
 static int cp_fibres_launch(int mode, int order, const long *dims, const float *const *U, long R, float *out,

@@ -21,13 +21,19 @@ namespace tdx {
 
 int ttm_generic(const float *X, int order, const long *dims, int mode, const float *U, long lda, long ldi, long Ra,
                 float *Y, hipStream_t s);
+int ttm_generic_t(const void *X, int xtype, int order, const long *dims, int mode, const float *U, long lda, long ldi,
+                  long Ra, float *Y, hipStream_t s);
 
 static constexpr int kGramMaxN = 128;
 static constexpr int kGramChunk = 32;
 
 // partial Gram of the (P, In, Q) view over fibre chunks: thread (ty, tx) owns an 8x8 block
 // of the In x In tile (In <= 128).  Fibre k of a chunk is (p, q) with p fastest.
-__global__ __launch_bounds__(256) void k_gram_partial(const float *__restrict__ X, long P, long In, long Q,
+__device__ __forceinline__ float ldx(const float *p, long i) { return p[i]; }
+__device__ __forceinline__ float ldx(const uint16_t *p, long i) { return __uint_as_float((uint32_t)p[i] << 16); }
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gram_partial(const T *__restrict__ X, long P, long In, long Q,
                                                       long kper, float *__restrict__ part) {
     __shared__ float xs[kGramChunk][kGramMaxN + 1];
     const int tid = threadIdx.x;
@@ -49,7 +55,7 @@ __global__ __launch_bounds__(256) void k_gram_partial(const float *__restrict__ 
             float v = 0.0f;
             if (k < k1 && a < In) {
                 const long p = k % P, q = k / P;
-                v = X[p + a * P + q * P * In];
+                v = ldx(X, p + a * P + q * P * In);
             }
             xs[kk][a] = v;
         }
@@ -169,7 +175,8 @@ struct TuckerWs {
     }
 };
 
-static int unfold_gram(const float *X, int order, const long *dims, int mode, TuckerWs &W, double *G, hipStream_t s) {
+static int unfold_gram(const void *X, int xtype, int order, const long *dims, int mode, TuckerWs &W, double *G,
+                       hipStream_t s) {
     long P = 1, Q = 1;
     for (int m = 0; m < mode; ++m) P *= dims[m];
     for (int m = mode + 1; m < order; ++m) Q *= dims[m];
@@ -181,7 +188,10 @@ static int unfold_gram(const float *X, int order, const long *dims, int mode, Tu
     if (nblk < 1) nblk = 1;
     long kper = ceil_div(ceil_div(K, nblk), kGramChunk) * kGramChunk;
     nblk = ceil_div(K, kper);
-    k_gram_partial<<<(int)nblk, 256, 0, s>>>(X, P, In, Q, kper, W.gram_part);
+    if (xtype == TDX_BF16)
+        k_gram_partial<uint16_t><<<(int)nblk, 256, 0, s>>>((const uint16_t *)X, P, In, Q, kper, W.gram_part);
+    else
+        k_gram_partial<float><<<(int)nblk, 256, 0, s>>>((const float *)X, P, In, Q, kper, W.gram_part);
     k_gram_sum<<<(int)ceil_div(In * In, 256), 256, 0, s>>>(W.gram_part, (int)nblk, In, G);
     TDX_LAUNCH_CHECK();
     return TDX_OK;
@@ -222,24 +232,24 @@ static int ws_setup(TuckerWs &W, int order, const long *dims, const long *ranks)
 
 // Y = X x_{m in modes} U_m^T applied in the listed order; returns the result pointer
 // (one of the workspace buffers) and the resulting dims in cur
-static int ttm_chain(const float *X, int order, const long *dims, const long *ranks, float *const *U,
+static int ttm_chain(const void *X, int xtype, int order, const long *dims, const long *ranks, float *const *U,
                      const int *modes, int nmodes, TuckerWs &W, long *cur, const float **out, hipStream_t s) {
     for (int m = 0; m < order; ++m) cur[m] = dims[m];
-    const float *src = X;
+    const void *src = X;
     for (int t = 0; t < nmodes; ++t) {
         const int m = modes[t];
         float *dst = (t & 1) ? W.y1 : W.y0;
-        int st = ttm_generic(src, order, cur, m, U[m], cur[m], 1, ranks[m], dst, s);
+        int st = ttm_generic_t(src, t == 0 ? xtype : TDX_F32, order, cur, m, U[m], cur[m], 1, ranks[m], dst, s);
         if (st) return st;
         cur[m] = ranks[m];
         src = dst;
     }
-    *out = src;
+    *out = (const float *)src;
     return TDX_OK;
 }
 
 static int tucker_run(const float *X, int order, const long *dims, const long *ranks, int iters, float *core,
-                      float *const *U, double *relerr_hist) {
+                      float *const *U, double *relerr_hist, int bf16) {
     if (!X || !core || !U || order < 2 || order > 4) return record(TDX_ERR_ARG);
     for (int m = 0; m < order; ++m) {
         if (dims[m] <= 0 || ranks[m] <= 0 || ranks[m] > dims[m] || !U[m]) return record(TDX_ERR_ARG);
@@ -249,14 +259,31 @@ static int tucker_run(const float *X, int order, const long *dims, const long *r
     TuckerWs W;
     int st = ws_setup(W, order, dims, ranks);
     if (st) return st;
+    long N = 1;
+    for (int m = 0; m < order; ++m) N *= dims[m];
+    // BF16 storage: the Grams of X and the first TTM of every chain stream a BF16 copy
+    const void *Xs = X;
+    int xtype = TDX_F32;
+    uint16_t *Xb = nullptr;
+    struct FreeB {
+        uint16_t *&p;
+        ~FreeB() {
+            if (p) (void)hipFree(p);
+        }
+    } freeb{Xb};
+    if (bf16) {
+        TDX_HIP(hipMalloc(&Xb, sizeof(uint16_t) * N));
+        st = tdx_quantize_bf16(X, Xb, N);
+        if (st) return st;
+        Xs = Xb;
+        xtype = TDX_BF16;
+    }
     // HOSVD init
     for (int m = 0; m < order; ++m) {
-        st = unfold_gram(X, order, dims, m, W, W.G, s);
+        st = unfold_gram(Xs, xtype, order, dims, m, W, W.G, s);
         if (!st) st = top_eigvecs(W.G, dims[m], ranks[m], W, U[m], s);
         if (st) return st;
     }
-    long N = 1;
-    for (int m = 0; m < order; ++m) N *= dims[m];
     k_norm2_blocks<float><<<1024, 256, 0, s>>>(X, N, W.scal);
     k_sum_to<<<1, 256, 0, s>>>(W.scal, 1024, W.scal + 1024);
     TDX_LAUNCH_CHECK();
@@ -268,8 +295,8 @@ static int tucker_run(const float *X, int order, const long *dims, const long *r
             int modes[4], nm = 0;
             for (int m = 0; m < order; ++m)
                 if (m != n) modes[nm++] = m;
-            st = ttm_chain(X, order, dims, ranks, U, modes, nm, W, cur, &Y, s);
-            if (!st) st = unfold_gram(Y, order, cur, n, W, W.G, s);
+            st = ttm_chain(Xs, xtype, order, dims, ranks, U, modes, nm, W, cur, &Y, s);
+            if (!st) st = unfold_gram(Y, TDX_F32, order, cur, n, W, W.G, s);
             if (!st) st = top_eigvecs(W.G, dims[n], ranks[n], W, U[n], s);
             if (st) return st;
         }
@@ -289,7 +316,7 @@ static int tucker_run(const float *X, int order, const long *dims, const long *r
     }
     // final core
     int modes[4] = {0, 1, 2, 3};
-    st = ttm_chain(X, order, dims, ranks, U, modes, order - 1, W, cur, &Y, s);
+    st = ttm_chain(Xs, xtype, order, dims, ranks, U, modes, order - 1, W, cur, &Y, s);
     if (st) return st;
     st = ttm_generic(Y, order, cur, order - 1, U[order - 1], dims[order - 1], 1, ranks[order - 1], core, s);
     if (st) return st;
@@ -308,7 +335,7 @@ int tdx_unfold_gram(const float *X, int order, const long *dims, int mode, doubl
     TuckerWs W;
     W.gram_blocks = 1024;
     TDX_HIP(hipMalloc(&W.gram_part, sizeof(float) * (size_t)W.gram_blocks * kGramMaxN * kGramMaxN));
-    int st = unfold_gram(X, order, dims, mode, W, G, stream());
+    int st = unfold_gram(X, TDX_F32, order, dims, mode, W, G, stream());
     if (!st) TDX_HIP(hipStreamSynchronize(stream()));
     return record(st);
 }
@@ -316,27 +343,30 @@ int tdx_unfold_gram(const float *X, int order, const long *dims, int mode, doubl
 int tdx_tucker_hooi(const float *X, int order, const long *dims, const long *ranks, int iters, int bf16_ttm,
                     float *core, float *const *U, double *relerr_hist) {
     if (!dims || !ranks || iters < 0) return record(TDX_ERR_ARG);
-    (void)bf16_ttm;
-    return record(tucker_run(X, order, dims, ranks, iters, core, U, relerr_hist));
+    return record(tucker_run(X, order, dims, ranks, iters, core, U, relerr_hist, bf16_ttm));
 }
 
 int tdx_tucker_hosvd(const float *X, float *core, float *U1, float *U2, float *U3, long a, long b, long c, long r1,
                      long r2, long r3) {
     const long dims[3] = {a, b, c}, ranks[3] = {r1, r2, r3};
     float *U[3] = {U1, U2, U3};
-    return record(tucker_run(X, 3, dims, ranks, 0, core, U, nullptr));
+    return record(tucker_run(X, 3, dims, ranks, 0, core, U, nullptr, 0));
 }
 
 int tdx_tucker_hosvd4(const float *X, float *core, float *U1, float *U2, float *U3, float *U4, long a, long b, long c,
                       long d, long r1, long r2, long r3, long r4) {
     const long dims[4] = {a, b, c, d}, ranks[4] = {r1, r2, r3, r4};
     float *U[4] = {U1, U2, U3, U4};
-    return record(tucker_run(X, 4, dims, ranks, 0, core, U, nullptr));
+    return record(tucker_run(X, 4, dims, ranks, 0, core, U, nullptr, 0));
 }
 
+// reference tucker_tensorcore (src/tucker_tensorcore.cu): reduced-precision Grams and TTM
+// chain; here the Grams and the first TTM stream a BF16 copy, everything accumulates in FP32
 int tdx_tucker_tensorcore(const float *X, float *core, float *U1, float *U2, float *U3, long a, long b, long c,
                           long r1, long r2, long r3) {
-    return tdx_tucker_hosvd(X, core, U1, U2, U3, a, b, c, r1, r2, r3);
+    const long dims[3] = {a, b, c}, ranks[3] = {r1, r2, r3};
+    float *U[3] = {U1, U2, U3};
+    return record(tucker_run(X, 3, dims, ranks, 0, core, U, nullptr, 1));
 }
 
 }  // extern "C"

@@ -76,3 +76,24 @@ def test_ttm_golden_and_random():
         out = empty({1: d * b * c, 2: a * d * c, 3: a * b * d}[f])
         tdx.ttm(X, dev(np.asarray(g["U"][f - 1], np.float32)), out, f, a, b, c, d)
         assert np.allclose(host(out), np.asarray(g["XU"][f - 1]), rtol=1e-6, atol=1e-7)
+
+
+def test_bf16_storage_hooi_and_tensorcore_entry():
+    """BF16 Grams and first TTMs (tucker_tensorcore): relative error within 1e-3 of the FP64
+    oracle's (BF16 rounding of X is 2^-9 relative per element)."""
+    dims, ranks = (32, 24, 16, 12), (6, 5, 4, 3)
+    X = tucker_tensor(dims, ranks, 5e-3, seed=2)
+    U0, _ = O.tucker_hosvd(X, dims, ranks)
+    _, co, hist_o = O.tucker_hooi(X, dims, ranks, 2, U0)
+    Ud = [empty(d * r) for d, r in zip(dims, ranks)]
+    core = empty(int(np.prod(ranks)))
+    hist = tdx.tucker_hooi(dev(X), dims, ranks, 2, core, Ud, bf16_ttm=True)
+    assert np.max(np.abs(hist - hist_o)) < 1e-3
+    d3, r3 = (20, 16, 12), (4, 3, 5)
+    X3 = tucker_tensor(d3, r3, 1e-3, seed=3)
+    Uo, co3 = O.tucker_hosvd(X3, d3, r3)
+    U3 = [empty(d * r) for d, r in zip(d3, r3)]
+    c3 = empty(int(np.prod(r3)))
+    tdx.tucker_tensorcore(dev(X3), c3, *U3, *d3, *r3)
+    for u, uo, d, r in zip(U3, Uo, d3, r3):
+        assert relnorm(host(u).reshape((d, r), order="F"), uo) < 1e-2
