@@ -77,11 +77,56 @@ __device__ __forceinline__ uint32_t half_of(const uint4 &v, int t) {
     return (t & 1) ? (w >> 16) : (w & 0xffffu);
 }
 
+// element type and per-lane load of one K-step operand slice
+template <int ST> struct XElem;
+template <> struct XElem<0> { typedef float T; typedef float4 V; };    // 4 FP32 per 16 B
+template <> struct XElem<1> { typedef uint16_t T; typedef uint4 V; };  // 8 BF16 per 16 B
+template <> struct XElem<2> { typedef uint8_t T; typedef uint2 V; };   // 8 FP8 per 8 B
+
+template <typename V> __device__ __forceinline__ V vzero();
+template <> __device__ __forceinline__ float4 vzero<float4>() { return float4{0.f, 0.f, 0.f, 0.f}; }
+template <> __device__ __forceinline__ uint4 vzero<uint4>() { return uint4{0u, 0u, 0u, 0u}; }
+template <> __device__ __forceinline__ uint2 vzero<uint2>() { return uint2{0u, 0u}; }
+
+// Loads of one K step.  Addresses are clamped into the tensor and out-of-range lanes are
+// zeroed with a select, so the loads are unconditional (no per-load branches).
+template <int ST, bool CA, int T>
+struct StepLoads {
+    typedef typename XElem<ST>::T ET;
+    typedef typename XElem<ST>::V VT;
+    static constexpr int NL = (ST == 0) ? 4 : 8;  // loads per lane per K step
+    VT v[NL];
+    __device__ __forceinline__ void issue(const ET *__restrict__ X, long slow_off, long fj, long If, long s1,
+                                          const long *roff, const bool *rvalid) {
+        if constexpr (CA) {
+            // lane rows fixed (roff[0]); NL consecutive fast indices fj .. fj+NL-1
+#pragma unroll
+            for (int q = 0; q < NL; ++q) {
+                const long j = fj + q;
+                const bool ok = rvalid[0] && (j < If);
+                const ET *p = X + slow_off + (ok ? j : 0) * s1 + roff[0];
+                const VT w = *reinterpret_cast<const VT *>(p);
+                v[q] = ok ? w : vzero<VT>();
+            }
+        } else {
+            const bool fok = fj < If;
+#pragma unroll
+            for (int t = 0; t < NL; ++t) {
+                const bool ok = rvalid[t] && fok;
+                const ET *p = X + slow_off + (fok ? fj : 0) + roff[t];
+                const VT w = *reinterpret_cast<const VT *>(p);
+                v[t] = ok ? w : vzero<VT>();
+            }
+        }
+    }
+};
+
 template <int ST, bool CA, int NB>
-__global__ __launch_bounds__(256, 2) void k_mttkrp(KArgs a) {
+__global__ __launch_bounds__(256, (ST != 0 && NB >= 3) ? 1 : 2) void k_mttkrp(KArgs a) {
     constexpr int KS = (ST == 0) ? 16 : 32;
     constexpr int T = (ST == 0) ? 4 : 8;
     constexpr int RW = 16 * T;
+    typedef typename XElem<ST>::T ET;
     __shared__ float red[RW * 64];
     if (a.skip && *a.skip) return;
 
@@ -93,6 +138,30 @@ __global__ __launch_bounds__(256, 2) void k_mttkrp(KArgs a) {
     const long kbeg = split * a.kper;
     const long kend = min(kbeg + a.kper, a.ksteps);
     const int Rp = a.Rp;
+    const ET *__restrict__ X = reinterpret_cast<const ET *>(a.X);
+    const long In = a.In, If = a.If, FB = a.FB;
+    const long s1 = a.stride[1];
+    const long smode = a.stride[a.mode];
+    const float *__restrict__ pf = a.panel[a.fmode];
+
+    // per-lane row offsets (elements), fixed for the whole K range
+    long roff[T];
+    bool rvalid[T];
+    if constexpr (CA) {
+        const long r = row0 + ((ST == 0) ? 4 : 8) * lm;
+        rvalid[0] = r < In;
+        roff[0] = rvalid[0] ? r : 0;
+#pragma unroll
+        for (int t = 1; t < T; ++t) { roff[t] = 0; rvalid[t] = false; }
+    } else {
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            const long o = row0 + 16 * t + lm;
+            rvalid[t] = o < In;
+            roff[t] = (rvalid[t] ? o : 0) * smode;
+        }
+    }
+    const long lane_f = (ST == 0) ? 4 * lg : 8 * lg;  // this lane's K-slot offset inside a step
 
     f32x4 acc[T][NB];
 #pragma unroll
@@ -100,139 +169,147 @@ __global__ __launch_bounds__(256, 2) void k_mttkrp(KArgs a) {
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) acc[t][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    float w[NB];
-    long cur_slow = -1, slow_off = 0;
-    const float *pf = a.panel[a.fmode];
-
-    long fb = (kbeg + wave) % a.FB;
-    long sl = (kbeg + wave) / a.FB;
-    for (long ks = kbeg + wave; ks < kend; ks += 4, fb += 4) {
-        while (fb >= a.FB) { fb -= a.FB; ++sl; }
-        if (sl != cur_slow) {  // wave-uniform
-            cur_slow = sl;
-            long rem = sl;
-            slow_off = 0;
+    // slow-index state: w[r] = prod of slow factor rows, slow_off = their element offset
+    auto slow_state = [&](long sl, float *w, long &off) {
+        long rem = sl;
+        off = 0;
 #pragma unroll
-            for (int nb = 0; nb < NB; ++nb) w[nb] = 1.0f;
-            for (int q = 0; q < a.nslow; ++q) {
-                const int m = a.slow[q];
-                const long idx = rem % a.dims[m];
-                rem /= a.dims[m];
-                slow_off += idx * a.stride[m];
-                const float *pm = a.panel[m] + ((idx >> 3) * Rp) * 8 + (idx & 7);
+        for (int nb = 0; nb < NB; ++nb) w[nb] = 1.0f;
+        for (int q = 0; q < a.nslow; ++q) {
+            const int m = a.slow[q];
+            const long idx = rem % a.dims[m];
+            rem /= a.dims[m];
+            off += idx * a.stride[m];
+            const float *pm = a.panel[m] + ((idx >> 3) * Rp) * 8 + (idx & 7);
 #pragma unroll
-                for (int nb = 0; nb < NB; ++nb) w[nb] *= pm[(16 * nb + lm) * 8];
-            }
+            for (int nb = 0; nb < NB; ++nb) w[nb] *= pm[(16 * nb + lm) * 8];
         }
-        const long f0 = fb * KS;
+    };
 
-        if constexpr (ST == 0) {
-            // ---------------- FP32 storage, v_mfma_f32_16x16x4_f32 ----------------
-            // K slot g of substep s <-> fast index f0 + 4g + s
-            const long fj = f0 + 4 * lg;
-            float kr[NB][4];
+    long ks = kbeg + wave;
+    if (ks < kend) {
+        long fb = ks % FB, sl = ks / FB;
+        float w[NB];
+        long slow_off;
+        slow_state(sl, w, slow_off);
+        StepLoads<ST, CA, T> cur, nxt;
+        cur.issue(X, slow_off, fb * KS + lane_f, If, s1, roff, rvalid);
+        for (;;) {
+            // next step's indices and slow state
+            const long ks_n = ks + 4;
+            long fb_n = fb + 4, sl_n = sl;
+            while (fb_n >= FB) { fb_n -= FB; ++sl_n; }
+            const bool has_next = ks_n < kend;
+            // Khatri-Rao operand of the current step (uses the current w)
+            const long fj = fb * KS + lane_f;
+            if constexpr (ST == 0) {
+                float kr[NB][4];
 #pragma unroll
-            for (int nb = 0; nb < NB; ++nb) {
-                const float4 p = *reinterpret_cast<const float4 *>(
-                    pf + ((fj >> 3) * Rp + 16 * nb + lm) * 8 + (fj & 7));
-                kr[nb][0] = p.x * w[nb];
-                kr[nb][1] = p.y * w[nb];
-                kr[nb][2] = p.z * w[nb];
-                kr[nb][3] = p.w * w[nb];
-            }
-            const float *X = reinterpret_cast<const float *>(a.X);
-            float xv[T][4];  // [row block t][substep s]
-            if constexpr (CA) {
-                const long r = row0 + 4 * lm;
+                for (int nb = 0; nb < NB; ++nb) {
+                    const float4 p = *reinterpret_cast<const float4 *>(pf + ((fj >> 3) * Rp + 16 * nb + lm) * 8 + (fj & 7));
+                    kr[nb][0] = p.x * w[nb];
+                    kr[nb][1] = p.y * w[nb];
+                    kr[nb][2] = p.z * w[nb];
+                    kr[nb][3] = p.w * w[nb];
+                }
+                float wn[NB];
+                long off_n = slow_off;
+                if (has_next) {
+                    if (sl_n != sl) slow_state(sl_n, wn, off_n);
+                    else {
+#pragma unroll
+                        for (int nb = 0; nb < NB; ++nb) wn[nb] = w[nb];
+                    }
+                    nxt.issue(X, off_n, fb_n * KS + lane_f, If, s1, roff, rvalid);
+                }
+                // A operand: xv[t][s] = row block t, K substep s (K slot lg <-> f0 + 4 lg + s)
 #pragma unroll
                 for (int s = 0; s < 4; ++s) {
-                    const long j = fj + s;
-                    float4 v = float4{0.f, 0.f, 0.f, 0.f};
-                    if (r < a.In && j < a.If) v = *reinterpret_cast<const float4 *>(X + slow_off + j * a.stride[1] + r);
-                    xv[0][s] = v.x; xv[1][s] = v.y; xv[2][s] = v.z; xv[3][s] = v.w;
-                }
-            } else {
 #pragma unroll
-                for (int t = 0; t < T; ++t) {
-                    const long o = row0 + 16 * t + lm;
-                    float4 v = float4{0.f, 0.f, 0.f, 0.f};
-                    if (o < a.In && fj < a.If) v = *reinterpret_cast<const float4 *>(X + slow_off + o * a.stride[a.mode] + fj);
-                    xv[t][0] = v.x; xv[t][1] = v.y; xv[t][2] = v.z; xv[t][3] = v.w;
-                }
-            }
+                    for (int t = 0; t < T; ++t) {
+                        float xv;
+                        if constexpr (CA) {
+                            const float4 &q = cur.v[s];
+                            xv = (t == 0) ? q.x : (t == 1) ? q.y : (t == 2) ? q.z : q.w;
+                        } else {
+                            const float4 &q = cur.v[t];
+                            xv = (s == 0) ? q.x : (s == 1) ? q.y : (s == 2) ? q.z : q.w;
+                        }
 #pragma unroll
-            for (int s = 0; s < 4; ++s)
-#pragma unroll
-                for (int t = 0; t < T; ++t)
-#pragma unroll
-                    for (int nb = 0; nb < NB; ++nb)
-                        acc[t][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(xv[t][s], kr[nb][s], acc[t][nb], 0, 0, 0);
-        } else {
-            // -------- BF16 / FP8 storage, v_mfma_f32_16x16x32_bf16 --------
-            // K slot group g holds fast indices f0 + 8g .. f0 + 8g + 7
-            const long fj = f0 + 8 * lg;
-            bf16x8 krb[NB];
-#pragma unroll
-            for (int nb = 0; nb < NB; ++nb) {
-                const float *pp = pf + ((fj >> 3) * Rp + 16 * nb + lm) * 8;
-                const float4 p0 = *reinterpret_cast<const float4 *>(pp);
-                const float4 p1 = *reinterpret_cast<const float4 *>(pp + 4);
-                const float ww = w[nb];
-                uint4 u;
-                u.x = pack_bf16(p0.x * ww, p0.y * ww);
-                u.y = pack_bf16(p0.z * ww, p0.w * ww);
-                u.z = pack_bf16(p1.x * ww, p1.y * ww);
-                u.w = pack_bf16(p1.z * ww, p1.w * ww);
-                krb[nb] = __builtin_bit_cast(bf16x8, u);
-            }
-            uint4 xa[T];
-            if constexpr (CA) {
-                // lane loads rows row0+8lm .. +7 for fast indices fj + s, s = 0..7
-                const long r = row0 + 8 * lm;
-                uint4 v[8];
-#pragma unroll
-                for (int s = 0; s < 8; ++s) {
-                    const long j = fj + s;
-                    v[s] = uint4{0u, 0u, 0u, 0u};
-                    if (r < a.In && j < a.If) {
-                        const long off = slow_off + j * a.stride[1] + r;
-                        if constexpr (ST == 1)
-                            v[s] = *reinterpret_cast<const uint4 *>(reinterpret_cast<const uint16_t *>(a.X) + off);
-                        else
-                            v[s] = fp8x8_to_bf16x8(*reinterpret_cast<const uint2 *>(reinterpret_cast<const uint8_t *>(a.X) + off));
+                        for (int nb = 0; nb < NB; ++nb)
+                            acc[t][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(xv, kr[nb][s], acc[t][nb], 0, 0, 0);
                     }
                 }
-                // in-register 8x8 transpose of 16-bit elements: xa[t] = (v[0][t], ..., v[7][t])
+                if (!has_next) break;
 #pragma unroll
-                for (int t = 0; t < 8; ++t) {
-                    uint4 o;
-                    o.x = half_of(v[0], t) | (half_of(v[1], t) << 16);
-                    o.y = half_of(v[2], t) | (half_of(v[3], t) << 16);
-                    o.z = half_of(v[4], t) | (half_of(v[5], t) << 16);
-                    o.w = half_of(v[6], t) | (half_of(v[7], t) << 16);
-                    xa[t] = o;
-                }
+                for (int nb = 0; nb < NB; ++nb) w[nb] = wn[nb];
+                slow_off = off_n;
             } else {
+                bf16x8 krb[NB];
 #pragma unroll
-                for (int t = 0; t < T; ++t) {
-                    const long o = row0 + 16 * t + lm;
-                    xa[t] = uint4{0u, 0u, 0u, 0u};
-                    if (o < a.In && fj < a.If) {
-                        const long off = slow_off + o * a.stride[a.mode] + fj;
-                        if constexpr (ST == 1)
-                            xa[t] = *reinterpret_cast<const uint4 *>(reinterpret_cast<const uint16_t *>(a.X) + off);
-                        else
-                            xa[t] = fp8x8_to_bf16x8(*reinterpret_cast<const uint2 *>(reinterpret_cast<const uint8_t *>(a.X) + off));
+                for (int nb = 0; nb < NB; ++nb) {
+                    const float *pp = pf + ((fj >> 3) * Rp + 16 * nb + lm) * 8;
+                    const float4 p0 = *reinterpret_cast<const float4 *>(pp);
+                    const float4 p1 = *reinterpret_cast<const float4 *>(pp + 4);
+                    const float ww = w[nb];
+                    uint4 u;
+                    u.x = pack_bf16(p0.x * ww, p0.y * ww);
+                    u.y = pack_bf16(p0.z * ww, p0.w * ww);
+                    u.z = pack_bf16(p1.x * ww, p1.y * ww);
+                    u.w = pack_bf16(p1.z * ww, p1.w * ww);
+                    krb[nb] = __builtin_bit_cast(bf16x8, u);
+                }
+                float wn[NB];
+                long off_n = slow_off;
+                if (has_next) {
+                    if (sl_n != sl) slow_state(sl_n, wn, off_n);
+                    else {
+#pragma unroll
+                        for (int nb = 0; nb < NB; ++nb) wn[nb] = w[nb];
+                    }
+                    nxt.issue(X, off_n, fb_n * KS + lane_f, If, s1, roff, rvalid);
+                }
+                if constexpr (CA) {
+                    uint4 v[8];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) {
+                        if constexpr (ST == 1) v[q] = cur.v[q];
+                        else v[q] = fp8x8_to_bf16x8(cur.v[q]);
+                    }
+                    // in-register 8x8 transpose of 16-bit elements: xa[t] = (v[0][t], ..., v[7][t])
+#pragma unroll
+                    for (int t = 0; t < 8; ++t) {
+                        uint4 o;
+                        o.x = half_of(v[0], t) | (half_of(v[1], t) << 16);
+                        o.y = half_of(v[2], t) | (half_of(v[3], t) << 16);
+                        o.z = half_of(v[4], t) | (half_of(v[5], t) << 16);
+                        o.w = half_of(v[6], t) | (half_of(v[7], t) << 16);
+                        const bf16x8 av = __builtin_bit_cast(bf16x8, o);
+#pragma unroll
+                        for (int nb = 0; nb < NB; ++nb)
+                            acc[t][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, krb[nb], acc[t][nb], 0, 0, 0);
+                    }
+                } else {
+#pragma unroll
+                    for (int t = 0; t < T; ++t) {
+                        uint4 xa;
+                        if constexpr (ST == 1) xa = cur.v[t];
+                        else xa = fp8x8_to_bf16x8(cur.v[t]);
+                        const bf16x8 av = __builtin_bit_cast(bf16x8, xa);
+#pragma unroll
+                        for (int nb = 0; nb < NB; ++nb)
+                            acc[t][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, krb[nb], acc[t][nb], 0, 0, 0);
                     }
                 }
-            }
+                if (!has_next) break;
 #pragma unroll
-            for (int t = 0; t < T; ++t) {
-                const bf16x8 av = __builtin_bit_cast(bf16x8, xa[t]);
-#pragma unroll
-                for (int nb = 0; nb < NB; ++nb)
-                    acc[t][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, krb[nb], acc[t][nb], 0, 0, 0);
+                for (int nb = 0; nb < NB; ++nb) w[nb] = wn[nb];
+                slow_off = off_n;
             }
+            cur = nxt;
+            ks = ks_n;
+            fb = fb_n;
+            sl = sl_n;
         }
     }
 
