@@ -301,9 +301,16 @@ static int tucker_run(const float *X, int order, const long *dims, const long *r
             if (st) return st;
         }
         if (relerr_hist) {
-            // core from the last chain: Y x_{d-1} U_{d-1}^T; relerr^2 = 1 - ||core||^2/||X||^2
+            // core from the last chain: Y x_{d-1} U_{d-1}^T; relerr^2 = 1 - ||core||^2/||X||^2.
+            // With BF16 storage the error is measured against the FP32 tensor: the core is
+            // recomputed from a chain that starts on X itself.
             long nc = 1;
             for (int m = 0; m < order; ++m) nc *= ranks[m];
+            if (bf16) {
+                int fm[4] = {0, 1, 2, 3};
+                st = ttm_chain(X, TDX_F32, order, dims, ranks, U, fm, order - 1, W, cur, &Y, s);
+                if (st) return st;
+            }
             st = ttm_generic(Y, order, cur, order - 1, U[order - 1], dims[order - 1], 1, ranks[order - 1], core, s);
             if (st) return st;
             k_sumsq<<<1, 256, 0, s>>>(core, nc, W.scal + 1025);
@@ -314,9 +321,9 @@ static int tucker_run(const float *X, int order, const long *dims, const long *r
             relerr_hist[it] = sqrt(fmax(nx2 - h[1], 0.0) / nx2);
         }
     }
-    // final core
+    // final core, always from the FP32 tensor
     int modes[4] = {0, 1, 2, 3};
-    st = ttm_chain(Xs, xtype, order, dims, ranks, U, modes, order - 1, W, cur, &Y, s);
+    st = ttm_chain(X, TDX_F32, order, dims, ranks, U, modes, order - 1, W, cur, &Y, s);
     if (st) return st;
     st = ttm_generic(Y, order, cur, order - 1, U[order - 1], dims[order - 1], 1, ranks[order - 1], core, s);
     if (st) return st;

@@ -2,7 +2,7 @@
 library's Philox streams (identical to the oracle's generator).
 
   C1  128^3, rank 16, U(0,1) factors (seed 0), 1% noise, U(0,1) init (seed 1)
-  C2  1024^3, rank 64, N(0,1) unit-column factors (seed 0), 0.1% noise, U(0,1) init (seed 1)
+  C2  1024^3, rank 64, N(0,1) unit-column factors (seed 0), 0.1% noise, N(0,1) init (seed 1)
   C3  128^4 Tucker, 32^4 N(0,1) core (seed 0), orthonormal factors from QR of N(0,1) 128x32
       blocks (seeds 1..4), 0.5% noise (seed 5)
   C4  4096 tensors, extents U{16..96}, ranks {4,8,16}, U(0,1) factors (seed t), 1% noise
@@ -51,6 +51,10 @@ def cp_tensor(dims, R, kind, eta, factor_seed=0, noise_seed=2):
 
 def uniform_init(dims, R, seed=1):
     return [_uniform(d * R, seed, m) for m, d in enumerate(dims)]
+
+
+def normal_init(dims, R, seed=1):
+    return [_normal(d * R, seed, m) for m, d in enumerate(dims)]
 
 
 def c1():

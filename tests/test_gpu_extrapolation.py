@@ -21,8 +21,10 @@ def test_grid_mode_is_plain_and_alpha_one_is_exact():
     fit_g, _, tr = E.run(X, DIMS, R, U, tdx.LS_GRID, 12, grid=[1.0, 1.5, 3.0])
     assert np.array_equal(fit_p, fit_g)
     assert np.array_equal(tr["plain_relerr"], fit_p)
-    # alpha = 1 reproduces the plain update: same relative error up to FP32 MTTKRP rounding
-    assert np.max(np.abs(tr["grid_relerr"][:, 0] - fit_p)) < 2e-6
+    # alpha = 1 reproduces the plain update up to the FP32 rounding of U1 + 1*(U - U1) and of
+    # the two different MTTKRPs behind the two fit identities (largest at sweep 0, where the
+    # swamp factors have large cancelling columns)
+    assert np.max(np.abs(tr["grid_relerr"][:, 0] - fit_p)) < 1e-5
 
 
 def test_lsb_accept_rule_and_exact_evaluation():

@@ -79,8 +79,9 @@ def test_ttm_golden_and_random():
 
 
 def test_bf16_storage_hooi_and_tensorcore_entry():
-    """BF16 Grams and first TTMs (tucker_tensorcore): relative error within 1e-3 of the FP64
-    oracle's (BF16 rounding of X is 2^-9 relative per element)."""
+    """BF16 Grams and first TTMs (tucker_tensorcore, HOOI with bf16_ttm): the subspaces come
+    from the BF16 copy, the core and the relative error from the FP32 tensor, so the error
+    matches the FP64 oracle's to second order in the BF16 rounding."""
     dims, ranks = (32, 24, 16, 12), (6, 5, 4, 3)
     X = tucker_tensor(dims, ranks, 5e-3, seed=2)
     U0, _ = O.tucker_hosvd(X, dims, ranks)
@@ -88,7 +89,7 @@ def test_bf16_storage_hooi_and_tensorcore_entry():
     Ud = [empty(d * r) for d, r in zip(dims, ranks)]
     core = empty(int(np.prod(ranks)))
     hist = tdx.tucker_hooi(dev(X), dims, ranks, 2, core, Ud, bf16_ttm=True)
-    assert np.max(np.abs(hist - hist_o)) < 1e-3
+    assert np.max(np.abs(hist - hist_o)) < 2e-5
     d3, r3 = (20, 16, 12), (4, 3, 5)
     X3 = tucker_tensor(d3, r3, 1e-3, seed=3)
     Uo, co3 = O.tucker_hosvd(X3, d3, r3)
