@@ -83,42 +83,48 @@ template <> struct XElem<0> { typedef float T; typedef float4 V; };    // 4 FP32
 template <> struct XElem<1> { typedef uint16_t T; typedef uint4 V; };  // 8 BF16 per 16 B
 template <> struct XElem<2> { typedef uint8_t T; typedef uint2 V; };   // 8 FP8 per 8 B
 
-template <typename V> __device__ __forceinline__ V vzero();
-template <> __device__ __forceinline__ float4 vzero<float4>() { return float4{0.f, 0.f, 0.f, 0.f}; }
-template <> __device__ __forceinline__ uint4 vzero<uint4>() { return uint4{0u, 0u, 0u, 0u}; }
-template <> __device__ __forceinline__ uint2 vzero<uint2>() { return uint2{0u, 0u}; }
+// keep the loaded bits where mask is all-ones, zero them where it is 0
+__device__ __forceinline__ float4 vmask(float4 v, uint32_t m) {
+    return float4{__uint_as_float(__float_as_uint(v.x) & m), __uint_as_float(__float_as_uint(v.y) & m),
+                  __uint_as_float(__float_as_uint(v.z) & m), __uint_as_float(__float_as_uint(v.w) & m)};
+}
+__device__ __forceinline__ uint4 vmask(uint4 v, uint32_t m) { return uint4{v.x & m, v.y & m, v.z & m, v.w & m}; }
+__device__ __forceinline__ uint2 vmask(uint2 v, uint32_t m) { return uint2{v.x & m, v.y & m}; }
 
-// Loads of one K step.  Addresses are clamped into the tensor and out-of-range lanes are
-// zeroed with a select, so the loads are unconditional (no per-load branches).
+// Loads of one K step.  Addresses are clamped into the tensor so every load is issued
+// unconditionally; which loads were in range is kept as a bit mask and out-of-range values
+// are zeroed only when the step is consumed (zeroing at issue time would make the wave wait
+// for the loads right after issuing them).
 template <int ST, bool CA, int T>
 struct StepLoads {
     typedef typename XElem<ST>::T ET;
     typedef typename XElem<ST>::V VT;
     static constexpr int NL = (ST == 0) ? 4 : 8;  // loads per lane per K step
     VT v[NL];
+    uint32_t ok;  // bit q: load q was in range
     __device__ __forceinline__ void issue(const ET *__restrict__ X, long slow_off, long fj, long If, long s1,
                                           const long *roff, const bool *rvalid) {
+        ok = 0;
         if constexpr (CA) {
             // lane rows fixed (roff[0]); NL consecutive fast indices fj .. fj+NL-1
 #pragma unroll
             for (int q = 0; q < NL; ++q) {
                 const long j = fj + q;
-                const bool ok = rvalid[0] && (j < If);
-                const ET *p = X + slow_off + (ok ? j : 0) * s1 + roff[0];
-                const VT w = *reinterpret_cast<const VT *>(p);
-                v[q] = ok ? w : vzero<VT>();
+                const bool in = rvalid[0] && (j < If);
+                v[q] = *reinterpret_cast<const VT *>(X + slow_off + (in ? j : 0) * s1 + roff[0]);
+                ok |= (uint32_t)in << q;
             }
         } else {
             const bool fok = fj < If;
 #pragma unroll
             for (int t = 0; t < NL; ++t) {
-                const bool ok = rvalid[t] && fok;
-                const ET *p = X + slow_off + (fok ? fj : 0) + roff[t];
-                const VT w = *reinterpret_cast<const VT *>(p);
-                v[t] = ok ? w : vzero<VT>();
+                v[t] = *reinterpret_cast<const VT *>(X + slow_off + (fok ? fj : 0) + roff[t]);
+                ok |= (uint32_t)(rvalid[t] && fok) << t;
             }
         }
     }
+    // load q with out-of-range lanes zeroed
+    __device__ __forceinline__ VT get(int q) const { return vmask(v[q], ((ok >> q) & 1u) ? 0xffffffffu : 0u); }
 };
 
 template <int ST, bool CA, int NB>
@@ -130,7 +136,8 @@ __global__ __launch_bounds__(256, (ST != 0 && NB >= 3) ? 1 : 2) void k_mttkrp(KA
     __shared__ float red[RW * 64];
     if (a.skip && *a.skip) return;
 
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
     const int lm = lane & 15, lg = lane >> 4;
     const int rowtile = blockIdx.x % a.rowtiles;
     const long split = blockIdx.x / a.rowtiles;
@@ -169,150 +176,172 @@ __global__ __launch_bounds__(256, (ST != 0 && NB >= 3) ? 1 : 2) void k_mttkrp(KA
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) acc[t][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    // slow-index state: w[r] = prod of slow factor rows, slow_off = their element offset
+    // Slow-mode weights w[r] = prod of the slow factor rows for every slow index this
+    // workgroup touches, precomputed into LDS (the reduction buffer, free until the end), so
+    // the K loop issues only tensor and panel loads.
+    const long sl_first = kbeg / FB;
+    const long sl_count = (kend > kbeg) ? (kend - 1) / FB - sl_first + 1 : 0;
+    float *wtab = red;  // sl_count * Rp floats, guaranteed to fit by the planner
+    for (long e = threadIdx.x; e < sl_count * Rp; e += blockDim.x) {
+        long rem = sl_first + e / Rp;
+        const int r = (int)(e % Rp);
+        float w = 1.0f;
+        for (int q = 0; q < a.nslow; ++q) {
+            const int m = a.slow[q];
+            const long idx = rem % a.dims[m];
+            rem /= a.dims[m];
+            w *= a.panel[m][((idx >> 3) * Rp + r) * 8 + (idx & 7)];
+        }
+        wtab[e] = w;
+    }
+    __syncthreads();
     auto slow_state = [&](long sl, float *w, long &off) {
         long rem = sl;
         off = 0;
-#pragma unroll
-        for (int nb = 0; nb < NB; ++nb) w[nb] = 1.0f;
         for (int q = 0; q < a.nslow; ++q) {
             const int m = a.slow[q];
             const long idx = rem % a.dims[m];
             rem /= a.dims[m];
             off += idx * a.stride[m];
-            const float *pm = a.panel[m] + ((idx >> 3) * Rp) * 8 + (idx & 7);
-#pragma unroll
-            for (int nb = 0; nb < NB; ++nb) w[nb] *= pm[(16 * nb + lm) * 8];
         }
+        const float *wt = wtab + (sl - sl_first) * Rp + lm;
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) w[nb] = wt[16 * nb];
     };
 
-    long ks = kbeg + wave;
-    if (ks < kend) {
-        long fb = ks % FB, sl = ks / FB;
+    // Software pipeline: PD register slots, each holding one K step's tensor loads, its
+    // fast-block index and its slow-mode weights.  Loads for step i + PD - 1 are issued
+    // before step i is multiplied, so PD - 1 steps are always in flight per wave.
+    constexpr int PD = (ST != 0 && NB >= 3) ? 3 : 2;
+    // the slot's factor-panel values are loaded together with its tensor loads, so that
+    // waiting for one slot never waits for loads issued for later slots (vmcnt is in order)
+    constexpr int PV = (ST == 0) ? 1 : 2;  // float4 panel values per column block
+    struct Slot {
+        StepLoads<ST, CA, T> v;
+        float4 pan[NB][PV];
         float w[NB];
-        long slow_off;
-        slow_state(sl, w, slow_off);
-        StepLoads<ST, CA, T> cur, nxt;
-        cur.issue(X, slow_off, fb * KS + lane_f, If, s1, roff, rvalid);
-        for (;;) {
-            // next step's indices and slow state
-            const long ks_n = ks + 4;
-            long fb_n = fb + 4, sl_n = sl;
-            while (fb_n >= FB) { fb_n -= FB; ++sl_n; }
-            const bool has_next = ks_n < kend;
-            // Khatri-Rao operand of the current step (uses the current w)
-            const long fj = fb * KS + lane_f;
-            if constexpr (ST == 0) {
-                float kr[NB][4];
+    };
+    const long ks0 = kbeg + wave;
+    const long nsteps = (ks0 < kend) ? (kend - ks0 + 3) / 4 : 0;
+    long f_fb = ks0 % FB, f_sl = ks0 / FB, f_off = 0;
+    float f_w[NB];
+    if (nsteps > 0) slow_state(f_sl, f_w, f_off);
+    long issued = 0;
+    auto issue = [&](Slot &sl) {
+        sl.v.issue(X, f_off, f_fb * KS + lane_f, If, s1, roff, rvalid);
+        {
+            const long fj = f_fb * KS + lane_f;
 #pragma unroll
-                for (int nb = 0; nb < NB; ++nb) {
-                    const float4 p = *reinterpret_cast<const float4 *>(pf + ((fj >> 3) * Rp + 16 * nb + lm) * 8 + (fj & 7));
-                    kr[nb][0] = p.x * w[nb];
-                    kr[nb][1] = p.y * w[nb];
-                    kr[nb][2] = p.z * w[nb];
-                    kr[nb][3] = p.w * w[nb];
-                }
-                float wn[NB];
-                long off_n = slow_off;
-                if (has_next) {
-                    if (sl_n != sl) slow_state(sl_n, wn, off_n);
-                    else {
+            for (int nb = 0; nb < NB; ++nb) {
+                const float *pp = pf + ((fj >> 3) * Rp + 16 * nb + lm) * 8 + (fj & 7);
 #pragma unroll
-                        for (int nb = 0; nb < NB; ++nb) wn[nb] = w[nb];
-                    }
-                    nxt.issue(X, off_n, fb_n * KS + lane_f, If, s1, roff, rvalid);
-                }
-                // A operand: xv[t][s] = row block t, K substep s (K slot lg <-> f0 + 4 lg + s)
-#pragma unroll
-                for (int s = 0; s < 4; ++s) {
-#pragma unroll
-                    for (int t = 0; t < T; ++t) {
-                        float xv;
-                        if constexpr (CA) {
-                            const float4 &q = cur.v[s];
-                            xv = (t == 0) ? q.x : (t == 1) ? q.y : (t == 2) ? q.z : q.w;
-                        } else {
-                            const float4 &q = cur.v[t];
-                            xv = (s == 0) ? q.x : (s == 1) ? q.y : (s == 2) ? q.z : q.w;
-                        }
-#pragma unroll
-                        for (int nb = 0; nb < NB; ++nb)
-                            acc[t][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(xv, kr[nb][s], acc[t][nb], 0, 0, 0);
-                    }
-                }
-                if (!has_next) break;
-#pragma unroll
-                for (int nb = 0; nb < NB; ++nb) w[nb] = wn[nb];
-                slow_off = off_n;
-            } else {
-                bf16x8 krb[NB];
-#pragma unroll
-                for (int nb = 0; nb < NB; ++nb) {
-                    const float *pp = pf + ((fj >> 3) * Rp + 16 * nb + lm) * 8;
-                    const float4 p0 = *reinterpret_cast<const float4 *>(pp);
-                    const float4 p1 = *reinterpret_cast<const float4 *>(pp + 4);
-                    const float ww = w[nb];
-                    uint4 u;
-                    u.x = pack_bf16(p0.x * ww, p0.y * ww);
-                    u.y = pack_bf16(p0.z * ww, p0.w * ww);
-                    u.z = pack_bf16(p1.x * ww, p1.y * ww);
-                    u.w = pack_bf16(p1.z * ww, p1.w * ww);
-                    krb[nb] = __builtin_bit_cast(bf16x8, u);
-                }
-                float wn[NB];
-                long off_n = slow_off;
-                if (has_next) {
-                    if (sl_n != sl) slow_state(sl_n, wn, off_n);
-                    else {
-#pragma unroll
-                        for (int nb = 0; nb < NB; ++nb) wn[nb] = w[nb];
-                    }
-                    nxt.issue(X, off_n, fb_n * KS + lane_f, If, s1, roff, rvalid);
-                }
-                if constexpr (CA) {
-                    uint4 v[8];
-#pragma unroll
-                    for (int q = 0; q < 8; ++q) {
-                        if constexpr (ST == 1) v[q] = cur.v[q];
-                        else v[q] = fp8x8_to_bf16x8(cur.v[q]);
-                    }
-                    // in-register 8x8 transpose of 16-bit elements: xa[t] = (v[0][t], ..., v[7][t])
-#pragma unroll
-                    for (int t = 0; t < 8; ++t) {
-                        uint4 o;
-                        o.x = half_of(v[0], t) | (half_of(v[1], t) << 16);
-                        o.y = half_of(v[2], t) | (half_of(v[3], t) << 16);
-                        o.z = half_of(v[4], t) | (half_of(v[5], t) << 16);
-                        o.w = half_of(v[6], t) | (half_of(v[7], t) << 16);
-                        const bf16x8 av = __builtin_bit_cast(bf16x8, o);
-#pragma unroll
-                        for (int nb = 0; nb < NB; ++nb)
-                            acc[t][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, krb[nb], acc[t][nb], 0, 0, 0);
-                    }
-                } else {
-#pragma unroll
-                    for (int t = 0; t < T; ++t) {
-                        uint4 xa;
-                        if constexpr (ST == 1) xa = cur.v[t];
-                        else xa = fp8x8_to_bf16x8(cur.v[t]);
-                        const bf16x8 av = __builtin_bit_cast(bf16x8, xa);
-#pragma unroll
-                        for (int nb = 0; nb < NB; ++nb)
-                            acc[t][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, krb[nb], acc[t][nb], 0, 0, 0);
-                    }
-                }
-                if (!has_next) break;
-#pragma unroll
-                for (int nb = 0; nb < NB; ++nb) w[nb] = wn[nb];
-                slow_off = off_n;
+                for (int q = 0; q < PV; ++q) sl.pan[nb][q] = *reinterpret_cast<const float4 *>(pp + 4 * q);
             }
-            cur = nxt;
-            ks = ks_n;
-            fb = fb_n;
-            sl = sl_n;
+        }
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) sl.w[nb] = f_w[nb];
+        // advance the frontier; past the last step it stays put, so the surplus issues of
+        // the pipeline tail re-read the last step's lines (cache hits, never consumed)
+        if (++issued < nsteps) {
+            f_fb += 4;
+            bool moved = false;
+            while (f_fb >= FB) { f_fb -= FB; ++f_sl; moved = true; }
+            if (moved) slow_state(f_sl, f_w, f_off);
+        }
+    };
+    auto compute = [&](const Slot &sl) {
+        if constexpr (ST == 0) {
+            float kr[NB][4];
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) {
+                const float4 p = sl.pan[nb][0];
+                kr[nb][0] = p.x * sl.w[nb];
+                kr[nb][1] = p.y * sl.w[nb];
+                kr[nb][2] = p.z * sl.w[nb];
+                kr[nb][3] = p.w * sl.w[nb];
+            }
+            // A operand: row block t, K substep s (K slot lg <-> f0 + 4 lg + s)
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+#pragma unroll
+                for (int t = 0; t < T; ++t) {
+                    float xv;
+                    if constexpr (CA) {
+                        const float4 q = sl.v.get(s);
+                        xv = (t == 0) ? q.x : (t == 1) ? q.y : (t == 2) ? q.z : q.w;
+                    } else {
+                        const float4 q = sl.v.get(t);
+                        xv = (s == 0) ? q.x : (s == 1) ? q.y : (s == 2) ? q.z : q.w;
+                    }
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb)
+                        acc[t][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(xv, kr[nb][s], acc[t][nb], 0, 0, 0);
+                }
+            }
+        } else {
+            bf16x8 krb[NB];
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) {
+                const float4 p0 = sl.pan[nb][0];
+                const float4 p1 = sl.pan[nb][PV - 1];
+                const float ww = sl.w[nb];
+                uint4 u;
+                u.x = pack_bf16(p0.x * ww, p0.y * ww);
+                u.y = pack_bf16(p0.z * ww, p0.w * ww);
+                u.z = pack_bf16(p1.x * ww, p1.y * ww);
+                u.w = pack_bf16(p1.z * ww, p1.w * ww);
+                krb[nb] = __builtin_bit_cast(bf16x8, u);
+            }
+            if constexpr (CA) {
+                uint4 v[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    if constexpr (ST == 1) v[q] = sl.v.get(q);
+                    else v[q] = fp8x8_to_bf16x8(sl.v.get(q));
+                }
+                // in-register 8x8 transpose of 16-bit elements: xa[t] = (v[0][t], ..., v[7][t])
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    uint4 o;
+                    o.x = half_of(v[0], t) | (half_of(v[1], t) << 16);
+                    o.y = half_of(v[2], t) | (half_of(v[3], t) << 16);
+                    o.z = half_of(v[4], t) | (half_of(v[5], t) << 16);
+                    o.w = half_of(v[6], t) | (half_of(v[7], t) << 16);
+                    const bf16x8 av = __builtin_bit_cast(bf16x8, o);
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb)
+                        acc[t][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, krb[nb], acc[t][nb], 0, 0, 0);
+                }
+            } else {
+#pragma unroll
+                for (int t = 0; t < T; ++t) {
+                    uint4 xa;
+                    if constexpr (ST == 1) xa = sl.v.get(t);
+                    else xa = fp8x8_to_bf16x8(sl.v.get(t));
+                    const bf16x8 av = __builtin_bit_cast(bf16x8, xa);
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb)
+                        acc[t][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, krb[nb], acc[t][nb], 0, 0, 0);
+                }
+            }
+        }
+    };
+    // Every issue is unconditional: a load issued on only some paths makes the compiler's
+    // wait counting assume the shortest queue and drain all loads in flight.
+    Slot ring[PD];
+    if (nsteps > 0) {
+#pragma unroll
+        for (int p = 0; p < PD - 1; ++p) issue(ring[p]);
+        for (long base = 0; base < nsteps; base += PD) {
+#pragma unroll
+            for (int p = 0; p < PD; ++p) {
+                issue(ring[(p + PD - 1) % PD]);
+                if (base + p < nsteps) compute(ring[p]);
+            }
         }
     }
 
+    __syncthreads();  // all waves are done with the weight table that shares the buffer
     // ---- fixed-order reduction of the 4 waves in LDS ----
     // acc[t][nb][v] is MFMA output (row mr = 4*(lane/16)+v, col 16nb + lane%16)
     for (int wv = 0; wv < 4; ++wv) {
@@ -453,6 +482,12 @@ int mttkrp_plan(MttkrpPlan &p, int order, const long *dims, int mode, int dtype,
         long ns = want < maxs ? want : maxs;
         if (ns < 1) ns = 1;
         p.ksteps_per_split = ceil_div(p.ksteps, ns);
+        // the slow-mode weight table of a split (<= kper/FB + 2 slow indices) lives in the
+        // kernel's LDS reduction buffer of rows_per_wg * 64 floats
+        const long FB = ceil_div(dims[fmode], KS);
+        const long cap_sl = (long)p.rows_per_wg * 64 / p.Rp;
+        const long kmax = (cap_sl - 2) * FB;
+        if (p.ksteps_per_split > kmax) p.ksteps_per_split = kmax;
         p.nsplit = (int)ceil_div(p.ksteps, p.ksteps_per_split);
     } else {
         long kelems = 1;
