@@ -14,6 +14,7 @@
 // Jacobi in one workgroup; eigenvector signs are fixed so that the entry of largest
 // magnitude is positive (syevdx leaves the sign unspecified).
 #include "common.hpp"
+#include "eig.hpp"
 #include "linalg.hpp"
 #include <vector>
 
@@ -90,44 +91,13 @@ __global__ void k_gram_sum(const float *__restrict__ part, int nblk, long In, do
     }
 }
 
-// Top-r eigenvectors of the symmetric n x n G (FP64), one workgroup, dynamic LDS holds A.
-// U (n x r, col-major FP32) receives the eigenvectors of the r largest eigenvalues in
-// ascending order; Q is an n x n FP64 global scratch for the eigenvectors.
-__global__ __launch_bounds__(256) void k_top_eigvecs(const double *__restrict__ G, int n, int r, double *Q,
-                                                     float *__restrict__ U, double *__restrict__ evals) {
-    extern __shared__ double A[];
-    __shared__ double sh[256];
-    __shared__ double cs[4 * (kGramMaxN / 2 + 1)];
-    __shared__ int pos[kGramMaxN];
-    const int tid = threadIdx.x;
-    for (int e = tid; e < n * n; e += 256) A[e] = G[e];
-    __syncthreads();
-    jacobi_eig_block(A, Q, n, cs, sh);
-    // rank of each eigenvalue in ascending order (ties broken by index)
-    for (int i = tid; i < n; i += 256) {
-        const double wi = A[i + i * n];
-        int rk = 0;
-        for (int j = 0; j < n; ++j) {
-            const double wj = A[j + j * n];
-            if (wj < wi || (wj == wi && j < i)) ++rk;
-        }
-        pos[i] = rk;
-        if (evals) evals[rk] = wi;
-    }
-    __syncthreads();
-    // column c of U <- eigenvector with rank n - r + c, sign fixed
-    for (int i = tid; i < n; i += 256) {
-        const int c = pos[i] - (n - r);
-        if (c < 0) continue;
-        int am = 0;
-        double best = fabs(Q[0 + i * n]);
-        for (int k = 1; k < n; ++k) {
-            const double v = fabs(Q[k + i * n]);
-            if (v > best) { best = v; am = k; }
-        }
-        const double sg = (Q[am + i * n] < 0.0) ? -1.0 : 1.0;
-        for (int k = 0; k < n; ++k) U[k + (long)c * n] = (float)(sg * Q[k + i * n]);
-    }
+// Top-r eigenvectors of the symmetric n x n G (FP64), one workgroup: U (n x r, col-major
+// FP32) receives the eigenvectors of the r largest eigenvalues in ascending order.
+__global__ __launch_bounds__(kEigThreads) void k_top_eigvecs(const double *__restrict__ G, int n, int r,
+                                                             float *__restrict__ U, double *__restrict__ w,
+                                                             double *__restrict__ scratch) {
+    extern __shared__ double smem[];
+    sym_top_eigvecs_block(G, n, r, U, w, scratch, scratch + (long)n * n, smem);
 }
 
 __global__ void k_sumsq(const float *__restrict__ X, long n, double *__restrict__ out) {
@@ -197,16 +167,17 @@ static int unfold_gram(const void *X, int xtype, int order, const long *dims, in
     return TDX_OK;
 }
 
-static int top_eigvecs(const double *G, long n, long r, TuckerWs &W, float *U, hipStream_t s) {
-    if (n > kGramMaxN || r > n) return record(TDX_ERR_ARG);
-    const size_t lds = sizeof(double) * n * n;
+// scratch: 2 n^2 doubles
+static int top_eigvecs(const double *G, long n, long r, double *scratch, float *U, double *w, hipStream_t s) {
+    if (n < 1 || n > kEigMaxN || r < 1 || r > n) return record(TDX_ERR_ARG);
     static bool attr_set = false;
     if (!attr_set) {
         TDX_HIP(hipFuncSetAttribute((const void *)k_top_eigvecs, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)(sizeof(double) * kGramMaxN * kGramMaxN)));
+                                    (int)(sizeof(double) * eig_lds_doubles(kEigMaxN))));
         attr_set = true;
     }
-    k_top_eigvecs<<<1, 256, lds, s>>>(G, (int)n, (int)r, W.Q, U, nullptr);
+    k_top_eigvecs<<<1, kEigThreads, sizeof(double) * eig_lds_doubles((int)n), s>>>(G, (int)n, (int)r, U, w,
+                                                                                   scratch);
     TDX_LAUNCH_CHECK();
     return TDX_OK;
 }
@@ -215,7 +186,7 @@ static int ws_setup(TuckerWs &W, int order, const long *dims, const long *ranks)
     W.gram_blocks = 1024;
     TDX_HIP(hipMalloc(&W.gram_part, sizeof(float) * (size_t)W.gram_blocks * kGramMaxN * kGramMaxN));
     TDX_HIP(hipMalloc(&W.G, sizeof(double) * kGramMaxN * kGramMaxN));
-    TDX_HIP(hipMalloc(&W.Q, sizeof(double) * kGramMaxN * kGramMaxN));
+    TDX_HIP(hipMalloc(&W.Q, sizeof(double) * 2 * kGramMaxN * kGramMaxN));
     TDX_HIP(hipMalloc(&W.scal, sizeof(double) * 2048));
     // largest intermediate of a TTM chain: the first contraction of the biggest mode
     long N = 1;
@@ -281,7 +252,7 @@ static int tucker_run(const float *X, int order, const long *dims, const long *r
     // HOSVD init
     for (int m = 0; m < order; ++m) {
         st = unfold_gram(Xs, xtype, order, dims, m, W, W.G, s);
-        if (!st) st = top_eigvecs(W.G, dims[m], ranks[m], W, U[m], s);
+        if (!st) st = top_eigvecs(W.G, dims[m], ranks[m], W.Q, U[m], nullptr, s);
         if (st) return st;
     }
     k_norm2_blocks<float><<<1024, 256, 0, s>>>(X, N, W.scal);
@@ -297,7 +268,7 @@ static int tucker_run(const float *X, int order, const long *dims, const long *r
                 if (m != n) modes[nm++] = m;
             st = ttm_chain(Xs, xtype, order, dims, ranks, U, modes, nm, W, cur, &Y, s);
             if (!st) st = unfold_gram(Y, TDX_F32, order, cur, n, W, W.G, s);
-            if (!st) st = top_eigvecs(W.G, dims[n], ranks[n], W, U[n], s);
+            if (!st) st = top_eigvecs(W.G, dims[n], ranks[n], W.Q, U[n], nullptr, s);
             if (st) return st;
         }
         if (relerr_hist) {
@@ -343,6 +314,16 @@ int tdx_unfold_gram(const float *X, int order, const long *dims, int mode, doubl
     W.gram_blocks = 1024;
     TDX_HIP(hipMalloc(&W.gram_part, sizeof(float) * (size_t)W.gram_blocks * kGramMaxN * kGramMaxN));
     int st = unfold_gram(X, TDX_F32, order, dims, mode, W, G, stream());
+    if (!st) TDX_HIP(hipStreamSynchronize(stream()));
+    return record(st);
+}
+
+int tdx_sym_eig_topr(const double *G, long n, long r, float *U, double *w) {
+    TDX_CHECK_ARG(G && U && n >= 1 && n <= kEigMaxN && r >= 1 && r <= n);
+    void *scr = nullptr;
+    int st = workspace(WS_MISC2, sizeof(double) * 2 * n * n, &scr);
+    if (st) return st;
+    st = top_eigvecs(G, n, r, (double *)scr, U, w, stream());
     if (!st) TDX_HIP(hipStreamSynchronize(stream()));
     return record(st);
 }

@@ -75,6 +75,7 @@ _SIGS = {
     "tdx_ttm": [_P, _P, _P, _L, _L, _L, _L, _L],
     "tdx_ttm_t": [_P, ctypes.c_int, ctypes.POINTER(_L), ctypes.c_int, _P, _L, _P],
     "tdx_unfold_gram": [_P, ctypes.c_int, ctypes.POINTER(_L), ctypes.c_int, _P],
+    "tdx_sym_eig_topr": [_P, _L, _L, _P, _P],
     "tdx_rse": [_P, _P, _L, ctypes.POINTER(ctypes.c_float)],
     "tdx_gencp": [_P, _P, _P, _P, _L, _L, _L, _L],
     "tdx_gencp4": [_P, _P, _P, _P, _P, _L, _L, _L, _L, _L],
@@ -333,6 +334,16 @@ def tucker_hosvd4(X, core, U1, U2, U3, U4, a, b, c, d, r1, r2, r3, r4):
 
 def tucker_tensorcore(X, core, U1, U2, U3, a, b, c, r1, r2, r3):
     _call("tdx_tucker_tensorcore", ptr(X), ptr(core), ptr(U1), ptr(U2), ptr(U3), a, b, c, r1, r2, r3)
+
+
+def sym_eig_topr(G, n, r, U, w=None):
+    """U (n x r FP32) <- eigenvectors of the r largest eigenvalues of the FP64 n x n G,
+    ascending; w (FP64, r) <- those eigenvalues."""
+    _call("tdx_sym_eig_topr", ptr(G), n, r, ptr(U), ptr(w))
+
+
+def unfold_gram(X, dims, mode, G):
+    _call("tdx_unfold_gram", ptr(X), len(dims), dims_arr(dims), mode, ptr(G))
 
 
 def tucker_hooi(X, dims, ranks, iters, core, U, bf16_ttm=False):

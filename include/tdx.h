@@ -97,6 +97,11 @@ int tdx_ttm(const float *X, const float *U, float *XU, long flag, long a, long b
 int tdx_ttm_t(const float *X, int order, const long *dims, int mode, const float *U, long r, float *Y);
 /* new: Gram of the mode-n unfolding, G = X(n) X(n)^T (I_n x I_n, FP64 out) */
 int tdx_unfold_gram(const float *X, int order, const long *dims, int mode, double *G);
+/* new (the eigensolve of src/tucker_hosvd.cu:43-70, cusolverDnDsyevdx + copy of the last r
+ * columns): U (n x r FP32, col-major) = eigenvectors of the r largest eigenvalues of the
+ * symmetric FP64 n x n G, ascending, sign fixed so the largest-magnitude entry is positive;
+ * w (n x 1, optional, NULL to skip) receives those r eigenvalues.  n <= 128. */
+int tdx_sym_eig_topr(const double *G, long n, long r, float *U, double *w);
 
 /* ---- reconstruction and error ---- */
 /* rse: src/rse.cu:4-15 -- *error (host) = ||X - X1|| / ||X||.  Unlike the reference, X1

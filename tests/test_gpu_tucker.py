@@ -27,6 +27,47 @@ def tucker_tensor(dims, ranks, eta, seed=0):
     return X.astype(np.float32)
 
 
+def _sym_cases():
+    rng = np.random.default_rng(7)
+    out = []
+    for n, r in ((1, 1), (2, 1), (3, 2), (12, 5), (64, 16), (128, 32), (128, 128)):
+        B = rng.standard_normal((n, n))
+        out.append((f"spd{n}_{r}", B @ B.T, r, True))
+    L = rng.standard_normal((128, 32))
+    out.append(("lowrank", L @ L.T * 1e3 + 1e-3 * np.eye(128), 32, True))
+    out.append(("indefinite", (lambda S: S + S.T)(rng.standard_normal((40, 40))), 7, True))
+    # degenerate top-r sets: any orthonormal basis of the eigenspace is a valid answer
+    out.append(("repeated", np.diag([1.0, 2, 2, 2, 3, 3, 5, 5]), 5, False))
+    out.append(("zero", np.zeros((6, 6)), 3, False))
+    return out
+
+
+@pytest.mark.parametrize("name,G,r,unique", _sym_cases(), ids=[c[0] for c in _sym_cases()])
+def test_sym_eig_topr_against_lapack(name, G, r, unique):
+    """Tridiagonalisation + multisection + inverse iteration against LAPACK's eigh (the
+    reference calls cusolverDnDsyevdx): eigenvalues, residuals, orthonormality, and the
+    vectors themselves (same sign convention) where the top-r eigenvalues are distinct."""
+    import torch
+    n = G.shape[0]
+    U = empty(n * r)
+    w = torch.empty(r, dtype=torch.float64, device="cuda")
+    tdx.sym_eig_topr(dev(np.ascontiguousarray(G.ravel(order="F"), np.float64)), n, r, U, w)
+    Ug = host(U).reshape((n, r), order="F").astype(np.float64)
+    wg = w.cpu().numpy()
+    wl, Vl = np.linalg.eigh(G)
+    wl, Vl = wl[n - r:], Vl[:, n - r:]
+    scale = max(1.0, np.max(np.abs(wl)))
+    assert np.max(np.abs(wg - wl)) <= 1e-13 * scale
+    assert np.max(np.abs(Ug.T @ Ug - np.eye(r))) < 1e-6
+    assert np.max(np.linalg.norm(G @ Ug - Ug * wg, axis=0)) <= 1e-6 * scale
+    if unique:
+        for c in range(r):
+            k = int(np.argmax(np.abs(Vl[:, c])))
+            if Vl[k, c] < 0:
+                Vl[:, c] = -Vl[:, c]
+        assert np.max(np.abs(Ug - Vl)) < 1e-5
+
+
 def test_gentucker_golden():
     g = GOLD["hand_tucker_3x4x2"]
     dims, ranks = g["dims"], g["ranks"]
