@@ -17,11 +17,14 @@ namespace tdx {
 
 static constexpr int kEigMaxN = 128;
 static constexpr int kEigThreads = 256;
-static constexpr int kEigSmall = 8 * kEigMaxN + 64;  // doubles of small vectors
+static constexpr int kEigSmall = 10 * kEigMaxN + 64;  // doubles of small vectors
 // the large LDS area: the matrix (lda = n + 1), later the packed Householder vectors plus a
 // 32-column batch of eigenvectors
+// (at least 512 doubles: the bisection counts, 4 ints per thread, also live there)
 __host__ __device__ constexpr long eig_area_doubles(int n) {
-    return ((long)n * (n + 1) > (long)n * (n - 1) / 2 + 32L * n) ? (long)n * (n + 1) : (long)n * (n - 1) / 2 + 32L * n;
+    const long a = (long)n * (n + 1), b = (long)n * (n - 1) / 2 + 32L * n;
+    const long m = a > b ? a : b;
+    return m > 512 ? m : 512;
 }
 // dynamic LDS doubles needed for order n
 __host__ __device__ constexpr long eig_lds_doubles(int n) { return kEigSmall + eig_area_doubles(n); }
@@ -32,17 +35,48 @@ __device__ __forceinline__ double wave_sum_f64(double v) {
     return v;
 }
 
+// 1/b to full double precision: the hardware estimate refined by two Newton steps
+// (r <- r + r (1 - b r)); b is finite and nonzero wherever it is used here
+__device__ __forceinline__ double rcp_f64(double b) {
+    double r = __builtin_amdgcn_rcp(b);
+    r = fma(fma(-b, r, 1.0), r, r);
+    r = fma(fma(-b, r, 1.0), r, r);
+    return r;
+}
+
 // number of eigenvalues of the (scaled) tridiagonal (d, e2 = e^2) that are < x
 __device__ inline int sturm_count(const double *d, const double *e2, int n, double x, double pivmin) {
     double q = d[0] - x;
     if (fabs(q) <= pivmin) q = -pivmin;
     int c = q < 0.0;
     for (int i = 1; i < n; ++i) {
-        q = (d[i] - x) - e2[i - 1] / q;
+        q = (d[i] - x) - e2[i - 1] * rcp_f64(q);
         if (fabs(q) <= pivmin) q = -pivmin;
         c += q < 0.0;
     }
     return c;
+}
+
+// the same count at NP points at once (independent recurrences, so their latencies overlap)
+template <int NP>
+__device__ inline void sturm_count_n(const double *d, const double *e2, int n, const double *x, double pivmin,
+                                     int *c) {
+    double q[NP];
+#pragma unroll
+    for (int t = 0; t < NP; ++t) {
+        q[t] = d[0] - x[t];
+        if (fabs(q[t]) <= pivmin) q[t] = -pivmin;
+        c[t] = q[t] < 0.0;
+    }
+    for (int i = 1; i < n; ++i) {
+        const double di = d[i], ei = e2[i - 1];
+#pragma unroll
+        for (int t = 0; t < NP; ++t) {
+            q[t] = (di - x[t]) - ei * rcp_f64(q[t]);
+            if (fabs(q[t]) <= pivmin) q[t] = -pivmin;
+            c[t] += q[t] < 0.0;
+        }
+    }
 }
 
 // One inverse-iteration solve (T - lam I) x = y for the tridiagonal (d, e), in place on the
@@ -61,21 +95,21 @@ __device__ inline void tri_solve_shifted(const double *d, const double *e, int n
         double r0, r1, r2, rhs, na, nb, ny;
         if (fabs(alpha) >= fabs(b)) {
             if (alpha == 0.0) alpha = tiny;
-            const double l = b / alpha;
+            const double l = b * rcp_f64(alpha);
             r0 = alpha; r1 = beta; r2 = 0.0;
             rhs = ycur;
             ny = ynext - l * ycur;
             na = a1 - l * beta;
             nb = c1;
         } else {
-            const double l = alpha / b;
+            const double l = alpha * rcp_f64(b);
             r0 = b; r1 = a1; r2 = c1;
             rhs = ynext;
             ny = ycur - l * ynext;
             na = beta - l * a1;
             nb = -l * c1;
         }
-        u0[i * ys] = 1.0 / r0;
+        u0[i * ys] = rcp_f64(r0);
         u1[i * ys] = r1;
         u2[i * ys] = r2;
         y[i * ys] = rhs;
@@ -84,7 +118,7 @@ __device__ inline void tri_solve_shifted(const double *d, const double *e, int n
         beta = nb;
     }
     if (fabs(alpha) < tiny) alpha = (alpha < 0.0) ? -tiny : tiny;
-    u0[(n - 1) * ys] = 1.0 / alpha;
+    u0[(n - 1) * ys] = rcp_f64(alpha);
     y[(n - 1) * ys] = ycur;
     // back substitution with the upper factor (bandwidth 3)
     double x2 = 0.0, x1 = y[(n - 1) * ys] * u0[(n - 1) * ys];
@@ -112,8 +146,9 @@ __device__ inline void sym_top_eigvecs_block(const double *__restrict__ Ag, int 
     double *tt = e2 + kEigMaxN;         // Householder tau
     double *pp = tt + kEigMaxN;         // 2 x kEigMaxN partial products
     double *lam = pp + 2 * kEigMaxN;    // wanted eigenvalues (scaled)
-    double *red = lam + kEigMaxN;       // 64 reduction slots
-    int *cnt = reinterpret_cast<int *>(pp);  // bisection counts (overlays pp)
+    double *vv = lam + kEigMaxN;        // Householder vector of the current step
+    double *ww = vv + kEigMaxN;         // w = p - K v of the current step
+    double *red = ww + kEigMaxN;        // 64 reduction slots
     double *A = smem + kEigSmall;
     const long area = eig_area_doubles(n);
 
@@ -143,42 +178,58 @@ __device__ inline void sym_top_eigvecs_block(const double *__restrict__ Ag, int 
             ee[k] = beta;
             tt[k] = tau;
         }
-        for (int i = tid; i < m; i += kEigThreads) Vh[(long)k * n + i] = (i == 0) ? 1.0 : x[i] * sc;
-        if (tau == 0.0) {
-            __syncthreads();
-            continue;
+        if (tid < m) {
+            const double v = (tid == 0) ? 1.0 : x[tid] * sc;
+            vv[tid] = v;
+            Vh[(long)k * n + tid] = v;
         }
-        // p = tau A22 v (two threads per row), and the partial sums of p^T v
+        __syncthreads();
+        if (tau == 0.0) continue;  // uniform: H_k = I
+        // p = tau A22 v, two threads per row, 8 independent partial sums
         const int i = tid & (kEigMaxN - 1), half = tid >> 7;
-        double p = 0.0, vi = 0.0;
+        double p = 0.0;
         if (i < m) {
             const int mid = (m + 1) >> 1;
             const int j0 = half ? mid : 0, j1 = half ? m : mid;
-            const double *arow = A + (k + 1 + i);
-            for (int j = j0; j < j1; ++j) {
-                const double vj = (j == 0) ? 1.0 : x[j] * sc;
-                p += arow[(long)(k + 1 + j) * lda] * vj;
+            const double *arow = A + (k + 1 + i) + (long)(k + 1) * lda;
+            double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            int j = j0;
+            for (; j + 8 <= j1; j += 8) {
+#pragma unroll
+                for (int u = 0; u < 8; ++u) acc[u] = fma(arow[(long)(j + u) * lda], vv[j + u], acc[u]);
             }
-            p *= tau;
-            vi = (i == 0) ? 1.0 : x[i] * sc;
+            for (; j < j1; ++j) acc[0] = fma(arow[(long)j * lda], vv[j], acc[0]);
+            p = tau * (((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7])));
         }
         pp[half * kEigMaxN + i] = p;
-        const double pv = wave_sum_f64(p * vi);
+        const double pv = wave_sum_f64((i < m) ? p * vv[i] : 0.0);
         if (lane == 0) red[4 + wave] = pv;
         __syncthreads();
         const double K = 0.5 * tau * (red[4] + red[5] + red[6] + red[7]);
-        // A22 -= v w^T + w v^T with w = p - K v; rows lane, lane + 64; columns j = wave mod 4
+        if (tid < m) ww[tid] = pp[tid] + pp[kEigMaxN + tid] - K * vv[tid];
+        __syncthreads();
+        // A22 -= v w^T + w v^T: rows lane, lane + 64; columns j = wave mod 4, 4 at a time
         const int i0 = lane, i1 = lane + 64;
-        const double v0 = (i0 < m) ? ((i0 == 0) ? 1.0 : x[i0] * sc) : 0.0;
-        const double v1 = (i1 < m) ? x[i1] * sc : 0.0;
-        const double w0 = (i0 < m) ? pp[i0] + pp[kEigMaxN + i0] - K * v0 : 0.0;
-        const double w1 = (i1 < m) ? pp[i1] + pp[kEigMaxN + i1] - K * v1 : 0.0;
-        for (int j = wave; j < m; j += 4) {
-            const double vj = (j == 0) ? 1.0 : x[j] * sc;
-            const double wj = pp[j] + pp[kEigMaxN + j] - K * vj;
-            double *acol = A + (k + 1) + (long)(k + 1 + j) * lda;
-            if (i0 < m) acol[i0] -= v0 * wj + w0 * vj;
-            if (i1 < m) acol[i1] -= v1 * wj + w1 * vj;
+        const bool r0ok = i0 < m, r1ok = i1 < m;
+        const double v0 = r0ok ? vv[i0] : 0.0, w0 = r0ok ? ww[i0] : 0.0;
+        const double v1 = r1ok ? vv[i1] : 0.0, w1 = r1ok ? ww[i1] : 0.0;
+        double *a22 = A + (k + 1) + (long)(k + 1) * lda;
+        int j = wave;
+        for (; j + 12 < m; j += 16) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int jj = j + 4 * u;
+                const double vj = vv[jj], wj = ww[jj];
+                double *acol = a22 + (long)jj * lda;
+                if (r0ok) acol[i0] -= v0 * wj + w0 * vj;
+                if (r1ok) acol[i1] -= v1 * wj + w1 * vj;
+            }
+        }
+        for (; j < m; j += 4) {
+            const double vj = vv[j], wj = ww[j];
+            double *acol = a22 + (long)j * lda;
+            if (r0ok) acol[i0] -= v0 * wj + w0 * vj;
+            if (r1ok) acol[i1] -= v1 * wj + w1 * vj;
         }
         __syncthreads();
     }
@@ -242,27 +293,43 @@ __device__ inline void sym_top_eigvecs_block(const double *__restrict__ Ag, int 
     glo -= 4.0 * DBL_EPSILON * tnorm + 2.0 * pivmin;
     ghi += 4.0 * DBL_EPSILON * tnorm + 2.0 * pivmin;
 
-    // ---- 3. multisection for eigenvalues n-r .. n-1 (g threads per eigenvalue) ----
+    // ---- 3. multisection for eigenvalues n-r .. n-1: g threads x NP points per eigenvalue,
+    //         so each pass narrows the interval (g NP + 1)-fold ----
+    constexpr int NP = 4;
     int lg = 0;
     while ((1 << (lg + 1)) < r) ++lg;
     if (r > 1) ++lg;  // 2^lg >= r
     int g = kEigThreads >> lg;
     if (g < 1) g = 1;
+    const int npt = g * NP;
+    const double inv_div = 1.0 / (double)(npt + 1);
     const int grp = tid / g, t = tid % g;
     const bool active = grp < r;
     const int jwant = n - r + grp;
+    int *cnt = reinterpret_cast<int *>(A);  // counts, the matrix area is free now
     double lo = glo, hi = ghi;
     const double atol = DBL_EPSILON * tnorm + 4.0 * pivmin;
     for (int it = 0; it < 200; ++it) {
-        const double x = lo + (hi - lo) * (double)(t + 1) / (double)(g + 1);
-        bool stalled = false;
-        cnt[tid] = active ? sturm_count(dd, e2, n, x, pivmin) : 0;
+        const double h = (hi - lo) * inv_div;
+        double xs[NP];
+        int cs[NP];
+#pragma unroll
+        for (int u = 0; u < NP; ++u) xs[u] = lo + h * (double)(t * NP + u + 1);
+        if (active) {
+            sturm_count_n<NP>(dd, e2, n, xs, pivmin, cs);
+        } else {
+#pragma unroll
+            for (int u = 0; u < NP; ++u) cs[u] = 0;
+        }
+#pragma unroll
+        for (int u = 0; u < NP; ++u) cnt[tid * NP + u] = cs[u];
         __syncthreads();
+        bool stalled = false;
         if (active) {
             double nlo = lo, nhi = hi;
-            for (int u = 0; u < g; ++u) {
-                const double xu = lo + (hi - lo) * (double)(u + 1) / (double)(g + 1);
-                if (cnt[grp * g + u] > jwant) {
+            for (int u = 0; u < npt; ++u) {
+                const double xu = lo + h * (double)(u + 1);
+                if (cnt[grp * npt + u] > jwant) {
                     nhi = xu;
                     break;
                 }
@@ -291,7 +358,7 @@ __device__ inline void sym_top_eigvecs_block(const double *__restrict__ Ag, int 
             const int c = tid;
             const double lc = lam[c0 + c];
             for (int i = 0; i < n; ++i) Y[i * nb + c] = 1.0 + 0.25 * (double)((i * 37 + (c0 + c) * 101 + 11) % 61) / 61.0;
-            for (int itr = 0; itr < 3; ++itr) {
+            for (int itr = 0; itr < 2; ++itr) {
                 tri_solve_shifted(dd, ee, n, lc, tiny, Y + c, U0 + c, U1 + c, U2 + c, nb);
                 double mxv = 0.0;
                 for (int i = 0; i < n; ++i) mxv = fmax(mxv, fabs(Y[i * nb + c]));
@@ -352,8 +419,8 @@ __device__ inline void sym_top_eigvecs_block(const double *__restrict__ Ag, int 
         for (int i = tid; i < n - k - 1; i += kEigThreads) Vp[off + i] = Vh[(long)k * n + i];
     }
     double *Yb = A + nv;
-    const int CB = 32;
-    const int gb = kEigThreads / CB;  // threads per column
+    constexpr int CB = 32;
+    constexpr int gb = kEigThreads / CB;  // threads per column
     for (int c0 = 0; c0 < r; c0 += CB) {
         const int nb = (r - c0 < CB) ? r - c0 : CB;
         for (int e = tid; e < n * nb; e += kEigThreads) Yb[e] = Yg[(long)c0 * n + e];
@@ -364,14 +431,27 @@ __device__ inline void sym_top_eigvecs_block(const double *__restrict__ Ag, int 
             const int m = n - k - 1;
             const long off = (long)k * (n - 1) - (long)k * (k - 1) / 2;
             const double tau = tt[k];
+            // this thread's rows i = tq + gb u of the reflector (at most kEigMaxN / gb of them)
+            constexpr int NR = kEigMaxN / gb;
+            double vr[NR], yr[NR];
+            double *ycol = Yb + (long)c * n + k + 1;
             double s = 0.0;
-            if (on)
-                for (int i = tq; i < m; i += gb) s += Vp[off + i] * Yb[(long)c * n + k + 1 + i];
+#pragma unroll
+            for (int u = 0; u < NR; ++u) {
+                const int i = tq + gb * u;
+                const bool in = on && i < m;
+                vr[u] = in ? Vp[off + i] : 0.0;
+                yr[u] = in ? ycol[i] : 0.0;
+                s = fma(vr[u], yr[u], s);
+            }
 #pragma unroll
             for (int o = 1; o < gb; o <<= 1) s += __shfl_xor(s, o, 64);
             const double f = tau * s;
-            if (on && f != 0.0)
-                for (int i = tq; i < m; i += gb) Yb[(long)c * n + k + 1 + i] -= f * Vp[off + i];
+#pragma unroll
+            for (int u = 0; u < NR; ++u) {
+                const int i = tq + gb * u;
+                if (on && i < m) ycol[i] = yr[u] - f * vr[u];
+            }
             __syncthreads();
         }
         // sign: entry of largest magnitude (lowest index on ties) positive

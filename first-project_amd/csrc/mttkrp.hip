@@ -221,8 +221,13 @@ __global__ __launch_bounds__(256, (ST != 0 && NB >= 3) ? 1 : 2) void k_mttkrp(KA
         float4 pan[NB][PV];
         float w[NB];
     };
-    const long ks0 = kbeg + wave;
-    const long nsteps = (ks0 < kend) ? (kend - ks0 + 3) / 4 : 0;
+    // each wave takes a contiguous quarter of the workgroup's K steps, so the two halves of
+    // a cache line (consecutive K steps of a row in case B) are read by the same wave
+    // back to back
+    const long quarter = (kend - kbeg + 3) / 4;
+    const long ks0 = kbeg + wave * quarter;
+    const long ks1 = min(ks0 + quarter, kend);
+    const long nsteps = (ks0 < ks1) ? ks1 - ks0 : 0;
     long f_fb = ks0 % FB, f_sl = ks0 / FB, f_off = 0;
     float f_w[NB];
     if (nsteps > 0) slow_state(f_sl, f_w, f_off);
@@ -243,9 +248,9 @@ __global__ __launch_bounds__(256, (ST != 0 && NB >= 3) ? 1 : 2) void k_mttkrp(KA
         // advance the frontier; past the last step it stays put, so the surplus issues of
         // the pipeline tail re-read the last step's lines (cache hits, never consumed)
         if (++issued < nsteps) {
-            f_fb += 4;
+            f_fb += 1;
             bool moved = false;
-            while (f_fb >= FB) { f_fb -= FB; ++f_sl; moved = true; }
+            if (f_fb >= FB) { f_fb = 0; ++f_sl; moved = true; }
             if (moved) slow_state(f_sl, f_w, f_off);
         }
     };

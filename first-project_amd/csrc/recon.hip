@@ -160,6 +160,12 @@ __global__ void k_ratio(const double *__restrict__ part, int nparts, double *__r
 //  lda = In, ldi = 1; Tucker expansion core -> tensor with lda = 1, ldi = I).
 __device__ __forceinline__ float ld_elem(const float *p, long i) { return p[i]; }
 __device__ __forceinline__ float ld_elem(const uint16_t *p, long i) { return __uint_as_float((uint32_t)p[i] << 16); }
+// the factor entry as it enters the product: unchanged for FP32 storage, rounded to BF16
+// (nearest-even) for BF16 storage, as on the MFMA path
+__device__ __forceinline__ float coef(float u, const float *) { return u; }
+__device__ __forceinline__ float coef(float u, const uint16_t *) {
+    return __uint_as_float(f32_to_bf16_bits(u) << 16);
+}
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_ttm_generic(const T *__restrict__ X, long P, long In, long Q,
@@ -173,17 +179,24 @@ __global__ __launch_bounds__(256) void k_ttm_generic(const T *__restrict__ X, lo
         const T *x = X + p + q * P * In;
         const float *u = U + a * lda;
         float acc = 0.0f;
-        for (long i = 0; i < In; ++i) acc = fmaf(u[i * ldi], ld_elem(x, i * P), acc);
+        for (long i = 0; i < In; ++i) acc = fmaf(coef(u[i * ldi], x), ld_elem(x, i * P), acc);
         Y[e] = acc;
     }
 }
 
 // X is FP32 (xtype TDX_F32) or BF16 bit patterns (TDX_BF16)
+int ttm_mfma_launch(const void *X, int xtype, long P, long In, long Q, const float *U, long lda, long ldi, long Ra,
+                    float *Y, hipStream_t s);
+
 int ttm_generic_t(const void *X, int xtype, int order, const long *dims, int mode, const float *U, long lda, long ldi,
                   long Ra, float *Y, hipStream_t s) {
     long P = 1, Q = 1;
     for (int m = 0; m < mode; ++m) P *= dims[m];
     for (int m = mode + 1; m < order; ++m) Q *= dims[m];
+    // MFMA kernel for the shapes it covers, the per-output kernel otherwise
+    const int mf = ttm_mfma_launch(X, xtype, P, dims[mode], Q, U, lda, ldi, Ra, Y, s);
+    if (mf < 0) return record(TDX_ERR_HIP);
+    if (mf > 0) return TDX_OK;
     const long n = P * Ra * Q;
     long g = ceil_div(n, 256);
     if (g > 65536) g = 65536;
@@ -308,6 +321,13 @@ int tdx_ttm(const float *X, const float *U, float *XU, long flag, long a, long b
 int tdx_ttm_t(const float *X, int order, const long *dims, int mode, const float *U, long r, float *Y) {
     TDX_CHECK_ARG(X && dims && U && Y && order >= 2 && order <= 4 && mode >= 0 && mode < order && r > 0);
     return record(ttm_generic(X, order, dims, mode, U, dims[mode], 1, r, Y, stream()));
+}
+
+int tdx_ttm_ex(const void *X, int dtype, int order, const long *dims, int mode, const float *U, long r, float *Y) {
+    TDX_CHECK_ARG(X && dims && U && Y && order >= 2 && order <= 4 && mode >= 0 && mode < order && r > 0);
+    TDX_CHECK_ARG(dtype == TDX_F32 || dtype == TDX_BF16);
+    for (int m = 0; m < order; ++m) TDX_CHECK_ARG(dims[m] > 0);
+    return record(ttm_generic_t(X, dtype, order, dims, mode, U, dims[mode], 1, r, Y, stream()));
 }
 
 // Tucker reconstruction core x_1 U_1 ... x_d U_d by successive expansions

@@ -22,6 +22,7 @@ namespace tdx {
 
 int ttm_generic(const float *X, int order, const long *dims, int mode, const float *U, long lda, long ldi, long Ra,
                 float *Y, hipStream_t s);
+int gram_mfma_launch(const void *X, int xtype, long P, long In, long Q, float *part, int max_parts, hipStream_t s);
 int ttm_generic_t(const void *X, int xtype, int order, const long *dims, int mode, const float *U, long lda, long ldi,
                   long Ra, float *Y, hipStream_t s);
 
@@ -84,7 +85,9 @@ __global__ __launch_bounds__(256) void k_gram_partial(const T *__restrict__ X, l
 
 __global__ void k_gram_sum(const float *__restrict__ part, int nblk, long In, double *__restrict__ G) {
     for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < In * In; e += (long)gridDim.x * blockDim.x) {
-        const long a = e % In, b = e / In;
+        // both triangles from the upper one, so G is symmetric bit for bit
+        const long i = e % In, j = e / In;
+        const long a = i < j ? i : j, b = i < j ? j : i;
         double s = 0.0;
         for (int k = 0; k < nblk; ++k) s += (double)part[(long)k * kGramMaxN * kGramMaxN + a * kGramMaxN + b];
         G[e] = s;
@@ -153,6 +156,17 @@ static int unfold_gram(const void *X, int xtype, int order, const long *dims, in
     const long In = dims[mode];
     if (In > kGramMaxN) return record(TDX_ERR_UNSUPPORTED);
     const long K = P * Q;
+    // MFMA path for the shapes it covers (16-byte vectors along the contiguous index)
+    {
+        const int mb = W.gram_blocks < 512 ? W.gram_blocks : 512;
+        const int nb = gram_mfma_launch(X, xtype, P, In, Q, W.gram_part, mb, s);
+        if (nb < 0) return record(TDX_ERR_HIP);
+        if (nb > 0) {
+            k_gram_sum<<<(int)ceil_div(In * In, 256), 256, 0, s>>>(W.gram_part, nb, In, G);
+            TDX_LAUNCH_CHECK();
+            return TDX_OK;
+        }
+    }
     long nblk = ceil_div(K, 2048);
     if (nblk > W.gram_blocks) nblk = W.gram_blocks;
     if (nblk < 1) nblk = 1;
@@ -284,7 +298,12 @@ static int tucker_run(const float *X, int order, const long *dims, const long *r
             }
             st = ttm_generic(Y, order, cur, order - 1, U[order - 1], dims[order - 1], 1, ranks[order - 1], core, s);
             if (st) return st;
-            k_sumsq<<<1, 256, 0, s>>>(core, nc, W.scal + 1025);
+            {
+                // ||core||^2: block partials (the first 1024 slots are free once ||X||^2 is summed)
+                const int nb = (int)(ceil_div(nc, 1024L) < 1024 ? ceil_div(nc, 1024L) : 1024);
+                k_norm2_blocks<float><<<nb, 256, 0, s>>>(core, nc, W.scal);
+                k_sum_to<<<1, 256, 0, s>>>(W.scal, nb, W.scal + 1025);
+            }
             double h[2];
             TDX_HIP(hipMemcpyAsync(h, W.scal + 1024, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
             TDX_HIP(hipStreamSynchronize(s));
@@ -308,14 +327,20 @@ using namespace tdx;
 
 extern "C" {
 
-int tdx_unfold_gram(const float *X, int order, const long *dims, int mode, double *G) {
+int tdx_unfold_gram_ex(const void *X, int dtype, int order, const long *dims, int mode, double *G) {
     TDX_CHECK_ARG(X && dims && G && order >= 2 && order <= 4 && mode >= 0 && mode < order);
+    TDX_CHECK_ARG(dtype == TDX_F32 || dtype == TDX_BF16);
+    for (int m = 0; m < order; ++m) TDX_CHECK_ARG(dims[m] > 0);
     TuckerWs W;
     W.gram_blocks = 1024;
     TDX_HIP(hipMalloc(&W.gram_part, sizeof(float) * (size_t)W.gram_blocks * kGramMaxN * kGramMaxN));
-    int st = unfold_gram(X, TDX_F32, order, dims, mode, W, G, stream());
+    int st = unfold_gram(X, dtype, order, dims, mode, W, G, stream());
     if (!st) TDX_HIP(hipStreamSynchronize(stream()));
     return record(st);
+}
+
+int tdx_unfold_gram(const float *X, int order, const long *dims, int mode, double *G) {
+    return tdx_unfold_gram_ex(X, TDX_F32, order, dims, mode, G);
 }
 
 int tdx_sym_eig_topr(const double *G, long n, long r, float *U, double *w) {

@@ -76,6 +76,8 @@ _SIGS = {
     "tdx_ttm_t": [_P, ctypes.c_int, ctypes.POINTER(_L), ctypes.c_int, _P, _L, _P],
     "tdx_unfold_gram": [_P, ctypes.c_int, ctypes.POINTER(_L), ctypes.c_int, _P],
     "tdx_sym_eig_topr": [_P, _L, _L, _P, _P],
+    "tdx_ttm_ex": [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_L), ctypes.c_int, _P, _L, _P],
+    "tdx_unfold_gram_ex": [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_L), ctypes.c_int, _P],
     "tdx_rse": [_P, _P, _L, ctypes.POINTER(ctypes.c_float)],
     "tdx_gencp": [_P, _P, _P, _P, _L, _L, _L, _L],
     "tdx_gencp4": [_P, _P, _P, _P, _P, _L, _L, _L, _L, _L],
@@ -342,8 +344,13 @@ def sym_eig_topr(G, n, r, U, w=None):
     _call("tdx_sym_eig_topr", ptr(G), n, r, ptr(U), ptr(w))
 
 
-def unfold_gram(X, dims, mode, G):
-    _call("tdx_unfold_gram", ptr(X), len(dims), dims_arr(dims), mode, ptr(G))
+def ttm_ex(X, dims, mode, U, r, Y, dtype=TDX_F32):
+    """Y = X x_mode U^T with U (I_mode x r) column-major; X FP32 or BF16 bits."""
+    _call("tdx_ttm_ex", ptr(X), dtype, len(dims), dims_arr(dims), mode, ptr(U), r, ptr(Y))
+
+
+def unfold_gram(X, dims, mode, G, dtype=TDX_F32):
+    _call("tdx_unfold_gram_ex", ptr(X), dtype, len(dims), dims_arr(dims), mode, ptr(G))
 
 
 def tucker_hooi(X, dims, ranks, iters, core, U, bf16_ttm=False):

@@ -95,8 +95,14 @@ int tdx_bench_mttkrp(const void *X, int dtype, const float *scales, int order, c
 int tdx_ttm(const float *X, const float *U, float *XU, long flag, long a, long b, long c, long d);
 /* new: order-d TTM with a transposed factor, Y = X x_mode U^T, U (I_mode x r) */
 int tdx_ttm_t(const float *X, int order, const long *dims, int mode, const float *U, long r, float *Y);
+/* new: tdx_ttm_t for X stored as TDX_F32 or TDX_BF16 (for BF16, U is rounded to BF16 and the
+ * products accumulate in FP32 on MFMA) */
+int tdx_ttm_ex(const void *X, int dtype, int order, const long *dims, int mode, const float *U, long r, float *Y);
 /* new: Gram of the mode-n unfolding, G = X(n) X(n)^T (I_n x I_n, FP64 out) */
 int tdx_unfold_gram(const float *X, int order, const long *dims, int mode, double *G);
+/* new: the same Gram for X stored as TDX_F32 or TDX_BF16 (FP32 accumulation on MFMA for the
+ * shapes with 16-byte vectors along the contiguous index, FP64 sum of the partials) */
+int tdx_unfold_gram_ex(const void *X, int dtype, int order, const long *dims, int mode, double *G);
 /* new (the eigensolve of src/tucker_hosvd.cu:43-70, cusolverDnDsyevdx + copy of the last r
  * columns): U (n x r FP32, col-major) = eigenvectors of the r largest eigenvalues of the
  * symmetric FP64 n x n G, ascending, sign fixed so the largest-magnitude entry is positive;

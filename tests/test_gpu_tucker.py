@@ -68,6 +68,91 @@ def test_sym_eig_topr_against_lapack(name, G, r, unique):
         assert np.max(np.abs(Ug - Vl)) < 1e-5
 
 
+GRAM_SHAPES = [
+    (64, 48, 32, 16),      # case B everywhere except mode 0 (case A, In % 8 == 0)
+    (128, 40, 24),         # In = 128 in case A; P = 128 / 5120 in case B
+    (32, 96, 128),         # the largest In in case B
+    (20, 16, 12),          # P = 20 / 320: falls back to the LDS kernel in modes 1-2
+    (18, 32, 33),          # In = 18 in case A (not a multiple of 8): fallback
+]
+
+
+@pytest.mark.parametrize("dims", GRAM_SHAPES, ids=["x".join(map(str, d)) for d in GRAM_SHAPES])
+@pytest.mark.parametrize("dtype", ["f32", "bf16"])
+def test_unfold_gram_all_modes(dims, dtype):
+    """G = X(n) X(n)^T against the FP64 numpy Gram of the same (dequantised) values; BF16
+    products are exact in FP32, so both paths are accumulation-error only."""
+    import torch
+    rng = np.random.default_rng(11)
+    X = rng.standard_normal(int(np.prod(dims))).astype(np.float32)
+    Xd = dev(X)
+    if dtype == "bf16":
+        Xq = torch.empty(X.size, dtype=torch.int16, device="cuda")
+        tdx.quantize_bf16(Xd, Xq)
+        Xv = (host(Xq).astype(np.uint16).astype(np.uint32) << 16).view(np.float32)
+        src, dt = Xq, tdx.TDX_BF16
+    else:
+        Xv, src, dt = X, Xd, tdx.TDX_F32
+    T = Xv.astype(np.float64).reshape(dims, order="F")
+    for mode in range(len(dims)):
+        In = dims[mode]
+        G = torch.empty(In * In, dtype=torch.float64, device="cuda")
+        tdx.unfold_gram(src, dims, mode, G, dtype=dt)
+        Gg = G.cpu().numpy().reshape((In, In), order="F")
+        Xn = np.moveaxis(T, mode, 0).reshape(In, -1)
+        Go = Xn @ Xn.T
+        err = np.max(np.abs(Gg - Go)) / np.max(np.abs(Go))
+        assert err < 2e-6, (dims, dtype, mode, err)
+        assert np.array_equal(Gg, Gg.T)
+
+
+TTM_SHAPES = [
+    ((64, 48, 32, 16), 8),    # TR in modes 1-3, P == 1 in mode 0
+    ((128, 40, 24), 32),
+    ((32, 128, 16), 20),      # Ra not a multiple of 16
+    ((16, 24, 8, 4), 40),     # Ra up to 64 (BF16 uses 4 column blocks, FP32 falls back)
+    ((20, 18, 12), 5),        # P = 20 / 360: falls back in modes 1-2; In = 20 in mode 0
+]
+
+
+@pytest.mark.parametrize("dims,r", TTM_SHAPES, ids=["x".join(map(str, d)) + f"_r{r}" for d, r in TTM_SHAPES])
+@pytest.mark.parametrize("dtype", ["f32", "bf16"])
+def test_ttm_transposed_all_modes(dims, r, dtype):
+    """Y = X x_n U^T against FP64 numpy on the same (dequantised) values; for BF16 storage U
+    is rounded to BF16 as the kernel does, so both sides multiply identical values."""
+    import torch
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal(int(np.prod(dims))).astype(np.float32)
+    Xd = dev(X)
+    if dtype == "bf16":
+        Xq = torch.empty(X.size, dtype=torch.int16, device="cuda")
+        tdx.quantize_bf16(Xd, Xq)
+        Xv = (host(Xq).astype(np.uint16).astype(np.uint32) << 16).view(np.float32)
+        src, dt = Xq, tdx.TDX_BF16
+    else:
+        Xv, src, dt = X, Xd, tdx.TDX_F32
+    T = Xv.astype(np.float64).reshape(dims, order="F")
+    for mode in range(len(dims)):
+        In = dims[mode]
+        rr = min(r, In) if dtype == "f32" else r
+        U = rng.standard_normal((In, rr)).astype(np.float32)
+        Ud = dev(np.ascontiguousarray(U.ravel(order="F")))
+        if dtype == "bf16":
+            Uq = torch.empty(U.size, dtype=torch.int16, device="cuda")
+            tdx.quantize_bf16(Ud, Uq)
+            Uv = (host(Uq).astype(np.uint16).astype(np.uint32) << 16).view(np.float32).reshape((In, rr), order="F")
+        else:
+            Uv = U
+        out_dims = list(dims)
+        out_dims[mode] = rr
+        Y = empty(int(np.prod(out_dims)))
+        tdx.ttm_ex(src, dims, mode, Ud, rr, Y, dtype=dt)
+        Yo = np.moveaxis(np.tensordot(T, Uv.astype(np.float64), axes=([mode], [0])), -1, mode)
+        Yg = host(Y).reshape(out_dims, order="F")
+        err = np.max(np.abs(Yg - Yo)) / np.max(np.abs(Yo))
+        assert err < 2e-6, (dims, rr, dtype, mode, err)
+
+
 def test_gentucker_golden():
     g = GOLD["hand_tucker_3x4x2"]
     dims, ranks = g["dims"], g["ranks"]
