@@ -28,6 +28,10 @@ sys.path.insert(0, os.path.join(ROOT, "first-project_amd"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per BF16 MTTKRP launch (mean of the 3 modes) from the two counter passes of
+# `bench.py --kernels-only` (FETCH_SIZE and WRITE_SIZE, each scaled by its reading for a
+# plain 1 GiB copy); counters cannot be read in the timed run itself
+MTTKRP_BF16_TRAFFIC = 2324912872
 TARGET_DELTA = 1e-4
 
 
@@ -132,6 +136,7 @@ def main():
     ap.add_argument("--steps", type=int, default=25, help="BF16 sweeps of the mixed path")
     ap.add_argument("--warmup", type=int, default=2, help="untimed warm-up sweeps per path")
     ap.add_argument("--size", type=int, default=1024)
+    ap.add_argument("--seeds", default="1,2,3,4,5", help="initialisation seeds of the C2 runs")
     ap.add_argument("--rank", type=int, default=64)
     ap.add_argument("--mttkrp-iters", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
@@ -168,17 +173,17 @@ def main():
     X = torch.empty(N, dtype=torch.float32, device=dev)
     tdx.gen_cp_tensor(X, dims, F, R, 1e-3, 2, 0)
 
-    def init():
-        # N(0,1) initial factors from seed 1 (a U(0,1) start stalls on these mixed-sign factors)
+    def init(seed=1):
+        # N(0,1) initial factors (a U(0,1) start stalls on these mixed-sign factors)
         us = []
         for m, d in enumerate(dims):
             t = torch.empty(d * R, dtype=torch.float32, device=dev)
-            tdx.fill_normal(t, 1, m)
+            tdx.fill_normal(t, seed, m)
             us.append(t)
         return us
 
-    def run(storage, sweeps, refine, want_ms=True):
-        U = init()
+    def run(storage, sweeps, refine, seed, want_ms=True):
+        U = init(seed)
         o = tdx.default_cp_options()
         o.max_sweeps, o.refine_sweeps, o.storage = sweeps, refine, storage
         o.init, o.record_fit, o.use_graph = 1, 1, 1
@@ -192,43 +197,59 @@ def main():
     # warm-up (also instantiates graphs, workspaces)
     log("warm-up")
     if args.warmup > 0:
-        run(tdx.TDX_F32, args.warmup, 0)
-        run(tdx.TDX_BF16, args.warmup, 1)
+        run(tdx.TDX_F32, args.warmup, 0, 1)
+        run(tdx.TDX_BF16, args.warmup, 1, 1)
+        run(tdx.TDX_FP8_E4M3, args.warmup, 1, 1)
     torch.cuda.synchronize()
 
-    # --- plain FP32 reference path ---
-    log("plain FP32 CP-ALS, 27 sweeps")
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    st32, rep32, fit32, ms32, U32 = run(tdx.TDX_F32, 27, 0)
-    torch.cuda.synchronize()
-    wall32 = time.perf_counter() - t0
-    ref = tdx.cp_relerr(X, dims, U32, R)
-    target = ref + TARGET_DELTA
-
-    def time_to_target(fit, ms):
+    def time_to_target(fit, ms, target):
         cum = np.cumsum(ms) / 1e3
         hit = np.nonzero(fit <= target)[0]
         if hit.size == 0:
             return None, None
         return float(cum[hit[0]]), int(hit[0] + 1)
 
-    t32, s32 = time_to_target(fit32, ms32)
+    # --- per init seed: plain FP32 (27 sweeps) sets the target, then the mixed paths ---
+    # ALS on this tensor settles on plateaus whose error is quantised by the number of
+    # components it has not resolved (relerr^2 ~ k/64); which plateau a run reaches at sweep 27
+    # depends on the trajectory, so the value is the median over several initialisations.
+    seeds = [int(v) for v in args.seeds.split(",")]
+    per_seed = []
+    U32 = None
+    for seed in seeds:
+        log(f"seed {seed}: plain FP32 27 sweeps, BF16 {args.steps}+2, FP8 {args.steps}+2")
+        torch.cuda.synchronize()
+        st32, rep32, fit32, ms32, U = run(tdx.TDX_F32, 27, 0, seed)
+        ref = tdx.cp_relerr(X, dims, U, R)
+        if U32 is None:
+            U32 = U
+        target = ref + TARGET_DELTA
+        t32, s32 = time_to_target(fit32, ms32, target)
+        stb, repb, fitb, msb, Ub = run(tdx.TDX_BF16, args.steps, 2, seed)
+        relb = tdx.cp_relerr(X, dims, Ub, R)
+        tb, sb = time_to_target(fitb, msb, target)
+        st8, rep8, fit8, ms8, U8 = run(tdx.TDX_FP8_E4M3, args.steps, 2, seed)
+        rel8 = tdx.cp_relerr(X, dims, U8, R)
+        t8, s8 = time_to_target(fit8, ms8, target)
+        per_seed.append({"seed": seed, "target": target,
+                         "fp32": {"s": t32, "sweeps": s32, "relerr": ref, "ms_sweep": float(ms32.mean())},
+                         "bf16": {"s": tb, "sweeps": sb, "relerr": relb, "ms_sweep": float(msb[:args.steps].mean())},
+                         "fp8": {"s": t8, "sweeps": s8, "relerr": rel8, "ms_sweep": float(ms8[:args.steps].mean())}})
+        log(f"  target {target:.6f}: fp32 {t32} s ({s32} sweeps), bf16 {tb} s ({sb}, relerr {relb:.6f}), "
+            f"fp8 {t8} s ({s8}, relerr {rel8:.6f})")
 
-    # --- mixed: BF16 + 2 FP32 refinement sweeps (the value) ---
-    log("mixed BF16 CP-ALS")
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    stb, repb, fitb, msb, Ub = run(tdx.TDX_BF16, args.steps, 2)
-    torch.cuda.synchronize()
-    wallb = time.perf_counter() - t0
-    relb = tdx.cp_relerr(X, dims, Ub, R)
-    tb, sb = time_to_target(fitb, msb)
+    def median_time(key):
+        v = sorted((r[key]["s"] if r[key]["s"] is not None else float("inf")) for r in per_seed)
+        med = float(np.median(v))
+        return None if not np.isfinite(med) else med
 
-    log("mixed FP8 CP-ALS")
-    st8, rep8, fit8, ms8, U8 = run(tdx.TDX_FP8_E4M3, args.steps, 2)
-    rel8 = tdx.cp_relerr(X, dims, U8, R)
-    t8, s8 = time_to_target(fit8, ms8)
+    t32m, tbm, t8m = median_time("fp32"), median_time("bf16"), median_time("fp8")
+    speedups = sorted((r["fp32"]["s"] / r["bf16"]["s"]) if (r["fp32"]["s"] and r["bf16"]["s"]) else 0.0
+                      for r in per_seed)
+    ms32_all = float(np.mean([r["fp32"]["ms_sweep"] for r in per_seed]))
+    msb_all = float(np.mean([r["bf16"]["ms_sweep"] for r in per_seed]))
+    ms8_all = float(np.mean([r["fp8"]["ms_sweep"] for r in per_seed]))
+    s32_med = float(np.median([r["fp32"]["sweeps"] or 27 for r in per_seed]))
 
     # --- MTTKRP per mode and storage ---
     log("MTTKRP micro-benchmarks")
@@ -265,7 +286,7 @@ def main():
         t0 = time.perf_counter()
         O.cp_als(Xh, dims, U0, 1)
         cpu_sweep = time.perf_counter() - t0
-        sweeps_needed = s32 if s32 is not None else 27
+        sweeps_needed = int(round(s32_med))
         cpu = {"value": round(cpu_sweep * sweeps_needed, 3), "unit": "s", "cores": threads, "kind": "port",
                "sample": f"1 FP64 ALS sweep (3 MTTKRPs + Cholesky solves) of the {n}^3 rank-{R} tensor on "
                          f"{threads} OpenMP threads = {cpu_sweep:.2f} s, x {sweeps_needed} sweeps to target"}
@@ -277,32 +298,32 @@ def main():
     avg_b_ms = float(np.mean(bf["ms_per_mode"]))
     out = {
         "metric": "CP-ALS sec to rel-err 1e-4 (mixed/learned vs plain FP32); MTTKRP HBM GB/s",
-        "value": round(tb, 5) if tb is not None else None,
+        "value": round(tbm, 5) if tbm is not None else None,
         "unit": "s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(float(np.mean(msb[:args.steps])), 4),
+        "ms_per_step": round(msb_all, 4),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "bf16",
         "data": "synthetic (Philox low-rank + 0.1% noise, generated in HBM)",
-        "config": {"workload": f"CP-ALS {n}x{n}x{n} rank {R}, BF16 storage + 2 FP32 refinement sweeps",
-                   "dims": list(dims), "rank": R, "noise": 1e-3},
-        "plain_fp32": {"sec_to_target": t32, "sweeps_to_target": s32, "ms_per_sweep": round(float(ms32.mean()), 4),
-                       "relerr_final": ref, "wall_s_27_sweeps": round(wall32, 4)},
-        "mixed_bf16": {"sec_to_target": tb, "sweeps_to_target": sb, "relerr_final": relb,
-                       "ms_per_sweep_bf16": round(float(msb[:args.steps].mean()), 4), "wall_s": round(wallb, 4)},
-        "mixed_fp8": {"sec_to_target": t8, "sweeps_to_target": s8, "relerr_final": rel8,
-                      "ms_per_sweep_fp8": round(float(ms8[:args.steps].mean()), 4)},
-        "speedup_vs_plain": round(t32 / tb, 3) if (t32 and tb) else None,
-        "target_relerr": target,
+        "config": {"workload": f"CP-ALS {n}x{n}x{n} rank {R}, BF16 storage {args.steps} sweeps + 2 FP32 refinement "
+                               f"sweeps; target = plain FP32 27-sweep rel-err + 1e-4; median over init seeds {seeds}",
+                   "dims": list(dims), "rank": R, "noise": 1e-3, "init_seeds": seeds},
+        "plain_fp32": {"median_sec_to_target": t32m, "ms_per_sweep": round(ms32_all, 4)},
+        "mixed_bf16": {"median_sec_to_target": tbm, "ms_per_sweep": round(msb_all, 4)},
+        "mixed_fp8": {"median_sec_to_target": t8m, "ms_per_sweep": round(ms8_all, 4)},
+        "speedup_vs_plain_median": round(float(np.median(speedups)), 3),
+        "per_seed": per_seed,
         "mttkrp": mt,
         "roofline": {"bound": "hbm", "achieved": round(alg_b / (avg_b_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(alg_b / (avg_b_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                     "traffic": None, "kernel": "k_mttkrp BF16 storage (mean of the 3 modes)",
-                     "bytes_per_launch": alg_b},
+                     "traffic": MTTKRP_BF16_TRAFFIC, "kernel": "k_mttkrp BF16 storage (mean of the 3 modes)",
+                     "bytes_per_launch": alg_b,
+                     "traffic_source": "FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per launch (scaled by a 1 GiB copy), "
+                                       "mean of the 3 modes; profiles/mttkrp_1024cubed_r64.md"},
         "cpu_baseline": cpu,
         "other_configs": extra,
     }

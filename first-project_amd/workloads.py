@@ -105,3 +105,29 @@ def c4(ntensors=4096, seed=0, lo=16, hi=96):
         xt = X[int(b.xoff[t]):int(b.xoff[t]) + int(np.prod(dims))]
         tdx.gen_cp_tensor(xt, dims, F, R, 1e-2, 20_000 + t, 0)
     return X, b
+
+
+def run_c3(iters=10):
+    """HOSVD + `iters` HOOI iterations of the C3 tensor; returns (seconds, rel-err per iteration)."""
+    import time
+
+    import torch
+    X, dims, ranks = c3()
+    U = [torch.empty(d * r, device="cuda") for d, r in zip(dims, ranks)]
+    core = torch.empty(int(np.prod(ranks)), device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist = tdx.tucker_hooi(X, dims, ranks, iters, core, U)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, hist
+
+
+if __name__ == "__main__":
+    import argparse
+    ap = argparse.ArgumentParser(description="run one benchmark recipe on cuda:0")
+    ap.add_argument("recipe", choices=["c3"])
+    ap.add_argument("--iters", type=int, default=10)
+    a = ap.parse_args()
+    tdx.lib()
+    sec, hist = run_c3(a.iters)
+    print(f"c3: {sec:.4f} s, rel-err {list(map(float, hist))}")
