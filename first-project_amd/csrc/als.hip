@@ -10,9 +10,10 @@
 // The reference does the same with a KR batched GEMM, an SGEMM on an unfolded copy of X,
 // FP32 LU (getrf/getrs) with a cudaMalloc per call, and a transpose (Sgeam).
 //
-// Kernels per mode: k_vinv (one workgroup, Cholesky + triangular inverse, Jacobi
-// pseudo-inverse fallback) runs on a forked stream concurrently with the MTTKRP, then
-// k_solve (one workgroup per 64 rows) and k_gram_reduce (one workgroup, also the fit).
+// Kernels per mode: k_vinv (one workgroup, symmetric sweep-operator inverse, Jacobi
+// pseudo-inverse fallback), the fused MTTKRP, then
+// k_split_sum (FP64 sum of the split-K partials), k_solve (one workgroup per 16 rows),
+// k_gram_psum (Gram of the new factor) and, once per sweep, k_fit.
 // A sweep is captured once into a hipGraph and replayed.
 #include "mttkrp.hpp"
 #include "linalg.hpp"
@@ -24,7 +25,8 @@
 namespace tdx {
 
 static constexpr int kMaxR = 64;     // rank limit of the LDS-resident solve
-static constexpr int kSolveRows = 64;
+static constexpr int kSolveRows = 16;  // rows per k_solve workgroup
+static constexpr int kGramRows = 64;   // rows per Gram-partial workgroup
 
 // V = hadamard_{m != mode} G_m; Vinv = V^{-1} by Cholesky (lower) and L^{-1}; on failure
 // a pseudo-inverse from the Jacobi eigendecomposition (eigenvalues below 1e-12 * max
@@ -47,20 +49,38 @@ __global__ __launch_bounds__(256) void k_vinv(const double *__restrict__ G0, con
         V[e] = v;
     }
     __syncthreads();
-    const int fb = block_spd_inverse(V, L, W, Vinv, R, cs, sh, &fail);
+    const int fb = block_spd_inverse<kMaxR * kMaxR / 256>(V, L, W, Vinv, R, cs, sh, &fail);
     if (fb && threadIdx.x == 0) atomicOr(status, 1);
 }
 
-// U_n rows = (sum of split partials) V^{-1}.  One workgroup per 64 rows, 256 threads,
-// each thread owns 16 (row, column) elements of the 64 x Rp block.
-// Also writes the panel layout (optionally row-scaled for FP8 storage), the per-block
-// inner product <M, U> (fit) and the per-block Gram partial U_blk^T U_blk.
-__global__ __launch_bounds__(256) void k_solve(const float *__restrict__ partial, int nsplit, long rows_pad, int Rp,
-                                               long In, int R, const double *__restrict__ Vinv,
-                                               float *__restrict__ U, float *__restrict__ panel,
-                                               const float *__restrict__ panel_scale,
+// M = sum of the split-K partials of the MTTKRP, FP64, in split order (deterministic);
+// one thread per element of the rows_pad x Rp block, 8 independent loads in flight
+__global__ __launch_bounds__(256) void k_split_sum(const float *__restrict__ partial, int nsplit, long nel,
+                                                   double *__restrict__ msum) {
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= nel) return;
+    double s = 0.0;
+    int sp = 0;
+    for (; sp + 8 <= nsplit; sp += 8) {
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = partial[(long)(sp + u) * nel + e];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += (double)v[u];
+    }
+    for (; sp < nsplit; ++sp) s += (double)partial[(long)sp * nel + e];
+    msum[e] = s;
+}
+
+// U_n rows = M V^{-1} for kSolveRows rows per workgroup (256 threads, 4 (row, column)
+// elements each).  Also writes the panel layout (optionally row-scaled for FP8 storage), the
+// per-block inner product <M, U> (fit) and the per-block Gram partial U_blk^T U_blk.
+__global__ __launch_bounds__(256) void k_solve(const double *__restrict__ msum, int Rp, long In, int R,
+                                               const double *__restrict__ Vinv, float *__restrict__ U,
+                                               float *__restrict__ panel, const float *__restrict__ panel_scale,
                                                double *__restrict__ ip_part, double *__restrict__ gram_part,
                                                int *__restrict__ skip_reset) {
+    constexpr int EPT = kSolveRows * kMaxR / 256;  // elements per thread
     __shared__ double Vs[kMaxR * kMaxR];
     __shared__ double Ms[kSolveRows * kMaxR];
     __shared__ double sh[256];
@@ -68,30 +88,14 @@ __global__ __launch_bounds__(256) void k_solve(const float *__restrict__ partial
     const long row0 = (long)blockIdx.x * kSolveRows;
     const int RR = R * R;
     for (int e = tid; e < RR; e += 256) Vs[e] = Vinv[e];
-    const int nel = kSolveRows * Rp;  // elements of the block (<= 64*64)
-    // 1. partial sums in fixed split order, FP64
-    double m[16];
-#pragma unroll
-    for (int u = 0; u < 16; ++u) m[u] = 0.0;
-    for (int sp = 0; sp < nsplit; ++sp) {
-        const float *src = partial + ((long)sp * rows_pad + row0) * Rp;
-#pragma unroll
-        for (int u = 0; u < 16; ++u) {
-            const int e = tid + 256 * u;
-            if (e < nel && row0 + e / Rp < rows_pad) m[u] += (double)src[e];
-        }
-    }
-#pragma unroll
-    for (int u = 0; u < 16; ++u) {
-        const int e = tid + 256 * u;
-        if (e < nel) Ms[(e / Rp) * kMaxR + (e % Rp)] = m[u];
-    }
+    const int nel = kSolveRows * Rp;
+    for (int e = tid; e < nel; e += 256) Ms[(e / Rp) * kMaxR + (e % Rp)] = msum[row0 * Rp + e];
     __syncthreads();
-    // 2. U = M V^{-1}, inner product with M
-    double uval[16];
+    // U = M V^{-1}, inner product with M
+    double uval[EPT];
     double ip = 0.0;
 #pragma unroll
-    for (int u = 0; u < 16; ++u) {
+    for (int u = 0; u < EPT; ++u) {
         const int e = tid + 256 * u;
         uval[u] = 0.0;
         if (e < nel) {
@@ -99,18 +103,23 @@ __global__ __launch_bounds__(256) void k_solve(const float *__restrict__ partial
             const long i = row0 + r;
             float uf = 0.0f;
             if (b < R && i < In) {
-                double s = 0.0;
-                for (int a = 0; a < R; ++a) s += Ms[r * kMaxR + a] * Vs[a + b * R];
-                uf = (float)s;
+                double sacc[4] = {0.0, 0.0, 0.0, 0.0};
+                int a = 0;
+                for (; a + 4 <= R; a += 4) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) sacc[q] += Ms[r * kMaxR + a + q] * Vs[a + q + b * R];
+                }
+                for (; a < R; ++a) sacc[0] += Ms[r * kMaxR + a] * Vs[a + b * R];
+                uf = (float)((sacc[0] + sacc[1]) + (sacc[2] + sacc[3]));
                 ip += Ms[r * kMaxR + b] * (double)uf;
             }
             uval[u] = (double)uf;
         }
     }
     __syncthreads();
-    // 3. write outputs; stage U (as rounded FP32 values) for the Gram partial
+    // write outputs; stage U (as rounded FP32 values) for the Gram partial
 #pragma unroll
-    for (int u = 0; u < 16; ++u) {
+    for (int u = 0; u < EPT; ++u) {
         const int e = tid + 256 * u;
         if (e < nel) {
             const int r = e / Rp, b = e % Rp;
@@ -126,36 +135,44 @@ __global__ __launch_bounds__(256) void k_solve(const float *__restrict__ partial
     if (tid == 0) ip_part[blockIdx.x] = ip;
     // the mode-0 MTTKRP that may have been skipped has already run (stream order)
     if (skip_reset && blockIdx.x == 0 && tid == 0) *skip_reset = 0;
-    // Ms now holds U block (sync inside block_sum)
+    // Ms now holds the U block (sync inside block_sum)
     double *gp = gram_part + (long)blockIdx.x * RR;
     for (int e = tid; e < RR; e += 256) {
         const int a = e % R, b = e / R;
         double s = 0.0;
+#pragma unroll
         for (int r = 0; r < kSolveRows; ++r) s += Ms[r * kMaxR + a] * Ms[r * kMaxR + b];
         gp[e] = s;
     }
 }
 
-// G_mode = sum of the block Gram partials (fixed order).  With fit != 0 also computes
-// the fast-fit relative error of Tensorlab cpd_als.m:281-283,
+// G = sum of the block Gram partials, in block order; one thread per entry
+__global__ __launch_bounds__(256) void k_gram_psum(const double *__restrict__ gram_part, int nblk, int RR,
+                                                   double *__restrict__ Gout) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= RR) return;
+    double s = 0.0;
+    int b = 0;
+    for (; b + 4 <= nblk; b += 4) {
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = gram_part[(long)(b + u) * RR + e];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) s += v[u];
+    }
+    for (; b < nblk; ++b) s += gram_part[(long)b * RR + e];
+    Gout[e] = s;
+}
+
+// The fast-fit relative error of Tensorlab cpd_als.m:281-283,
 //   ||X - Xhat||^2 = ||X||^2 - 2 <M, U_n> + sum(hadamard of all Grams),
-// appends it to fit_hist[*counter] and increments *counter.
-__global__ __launch_bounds__(256) void k_gram_reduce(const double *__restrict__ gram_part, int nblk, int R,
-                                                     double *__restrict__ Gout, const double *__restrict__ ip_part,
-                                                     const double *G0, const double *G1, const double *G2,
-                                                     const double *G3, int order, int fit,
-                                                     const double *__restrict__ normx2, double *__restrict__ fit_hist,
-                                                     int *__restrict__ counter) {
+// appended to fit_hist[*counter]; *counter is incremented.
+__global__ __launch_bounds__(256) void k_fit(const double *__restrict__ ip_part, int nblk, int R, const double *G0,
+                                             const double *G1, const double *G2, const double *G3, int order,
+                                             const double *__restrict__ normx2, double *__restrict__ fit_hist,
+                                             int *__restrict__ counter) {
     __shared__ double sh[256];
     const int tid = threadIdx.x, RR = R * R;
-    for (int e = tid; e < RR; e += 256) {
-        double s = 0.0;
-        for (int b = 0; b < nblk; ++b) s += gram_part[(long)b * RR + e];
-        Gout[e] = s;
-    }
-    if (!fit) return;
-    __syncthreads();
-    __threadfence_block();
     const double *G[4] = {G0, G1, G2, G3};
     double hx = 0.0;
     for (int e = tid; e < RR; e += 256) {
@@ -176,15 +193,38 @@ __global__ __launch_bounds__(256) void k_gram_reduce(const double *__restrict__ 
     }
 }
 
-// direct Gram U^T U (FP64) of a column-major factor; one workgroup
-__global__ __launch_bounds__(256) void k_gram_direct(const float *__restrict__ U, long I, int R, double *__restrict__ G) {
+// Gram partials U_blk^T U_blk (FP64) of kGramRows-row blocks of a column-major I x R factor
+__global__ __launch_bounds__(256) void k_gram_rows(const float *__restrict__ U, long I, int R,
+                                                   double *__restrict__ gram_part) {
+    __shared__ double Us[kGramRows * kMaxR];
+    const long row0 = (long)blockIdx.x * kGramRows;
+    for (int e = threadIdx.x; e < kGramRows * R; e += 256) {
+        const int r = e % kGramRows, b = e / kGramRows;
+        const long i = row0 + r;
+        Us[r * kMaxR + b] = (i < I) ? (double)U[i + (long)b * I] : 0.0;
+    }
+    __syncthreads();
     const int RR = R * R;
+    double *gp = gram_part + (long)blockIdx.x * RR;
     for (int e = threadIdx.x; e < RR; e += 256) {
         const int a = e % R, b = e / R;
-        double s = 0.0;
-        for (long i = 0; i < I; ++i) s += (double)U[i + a * I] * (double)U[i + b * I];
-        G[e] = s;
+        double s[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+        for (int r = 0; r < kGramRows; r += 4) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) s[q] += Us[(r + q) * kMaxR + a] * Us[(r + q) * kMaxR + b];
+        }
+        gp[e] = (s[0] + s[1]) + (s[2] + s[3]);
     }
+}
+
+// G = U^T U (FP64) of a column-major I x R factor: row-block partials, then their sum
+static int gram_of(const float *U, long I, int R, double *gram_part, double *G, hipStream_t s) {
+    const int nb = (int)ceil_div(I, kGramRows);
+    k_gram_rows<<<nb, 256, 0, s>>>(U, I, R, gram_part);
+    k_gram_psum<<<(int)ceil_div((long)R * R, 256L), 256, 0, s>>>(gram_part, nb, R * R, G);
+    TDX_LAUNCH_CHECK();
+    return TDX_OK;
 }
 
 // ||X||^2 in FP64 with Kahan compensation per thread; per-block partials then one block
@@ -357,24 +397,9 @@ __global__ void k_extrapolate(FacSet U, FacSet U1, FacSet Ua, float *pa0, float 
 }
 
 // Grams of a factor set, one workgroup per mode
-__global__ __launch_bounds__(256) void k_gram_set(FacSet U, double *G0, double *G1, double *G2, double *G3) {
-    double *Gs[4] = {G0, G1, G2, G3};
-    const int m = blockIdx.x;
-    const long I = U.I[m];
-    const int R = U.R;
-    const float *u = U.p[m];
-    for (int e = threadIdx.x; e < R * R; e += 256) {
-        const int a = e % R, b = e / R;
-        double s = 0.0;
-        for (long i = 0; i < I; ++i) s += (double)u[i + a * I] * (double)u[i + b * I];
-        Gs[m][e] = s;
-    }
-}
-
-// per-block <M_0(U_a), U_a,0> with M_0 the FP64 sum of the split partials
-__global__ __launch_bounds__(256) void k_ls_ip(const float *__restrict__ partial, int nsplit, long rows_pad, int Rp,
-                                               long In, int R, const float *__restrict__ Ua0,
-                                               double *__restrict__ ip_part) {
+// per-block <M_0(U_a), U_a,0> with M_0 the FP64 sum of the split partials (msum)
+__global__ __launch_bounds__(256) void k_ls_ip(const double *__restrict__ msum, int Rp, long In, int R,
+                                               const float *__restrict__ Ua0, double *__restrict__ ip_part) {
     __shared__ double sh[256];
     const long row0 = (long)blockIdx.x * kSolveRows;
     double ip = 0.0;
@@ -382,9 +407,7 @@ __global__ __launch_bounds__(256) void k_ls_ip(const float *__restrict__ partial
         const long i = row0 + e / Rp;
         const int b = e % Rp;
         if (i >= In || b >= R) continue;
-        double m = 0.0;
-        for (int sp = 0; sp < nsplit; ++sp) m += (double)partial[((long)sp * rows_pad + i) * Rp + b];
-        ip += m * (double)Ua0[i + (long)b * In];
+        ip += msum[i * Rp + b] * (double)Ua0[i + (long)b * In];
     }
     ip = block_sum(ip, sh);
     if (threadIdx.x == 0) ip_part[blockIdx.x] = ip;
@@ -468,6 +491,8 @@ struct CpPlan {
     double *Vinv[4] = {nullptr, nullptr, nullptr, nullptr};
     float *partial = nullptr;
     size_t partial_floats = 0;
+    double *msum = nullptr;  // FP64 sum of the partials, rows_pad x Rp
+    size_t msum_doubles = 0;
     double *ip_part = nullptr;
     double *gram_part = nullptr;
     double *fit_hist = nullptr;
@@ -517,10 +542,10 @@ struct CpPlan {
             if (Vinv[m]) (void)hipFree(Vinv[m]);
             panel[m] = nullptr; G[m] = nullptr; Vinv[m] = nullptr;
         }
-        void *ps[] = {partial, ip_part, gram_part, fit_hist, counter, status, normx2, scratch};
+        void *ps[] = {partial, msum, ip_part, gram_part, fit_hist, counter, status, normx2, scratch};
         for (void *p : ps)
             if (p) (void)hipFree(p);
-        partial = nullptr; ip_part = nullptr; gram_part = nullptr; fit_hist = nullptr;
+        partial = nullptr; msum = nullptr; ip_part = nullptr; gram_part = nullptr; fit_hist = nullptr;
         counter = nullptr; status = nullptr; normx2 = nullptr; scratch = nullptr;
         if (events) {
             for (int m = 0; m < 4; ++m) {
@@ -604,6 +629,15 @@ static int ensure_partial(CpPlan &P, const Storage &S) {
         TDX_HIP(hipMalloc(&P.partial, sizeof(float) * need));
         P.partial_floats = need;
     }
+    size_t mneed = 0;
+    for (int m = 0; m < P.order; ++m)
+        if ((size_t)S.mp[m].rows_pad * P.Rp > mneed) mneed = (size_t)S.mp[m].rows_pad * P.Rp;
+    if (mneed > P.msum_doubles) {
+        if (P.msum) TDX_HIP(hipFree(P.msum));
+        P.msum = nullptr;
+        TDX_HIP(hipMalloc(&P.msum, sizeof(double) * mneed));
+        P.msum_doubles = mneed;
+    }
     return TDX_OK;
 }
 
@@ -626,8 +660,8 @@ static int refresh_factors(CpPlan &P, const Storage &S, hipStream_t s) {
         const float *sc = (S.dtype == TDX_FP8_E4M3 && m == 0) ? S.scales : nullptr;
         int st = pack_panel(P.U[m], P.dims[m], P.R, P.Rp, sc, P.panel[m], s);
         if (st) return st;
-        k_gram_direct<<<1, 256, 0, s>>>(P.U[m], P.dims[m], (int)P.R, P.G[m]);
-        TDX_LAUNCH_CHECK();
+        st = gram_of(P.U[m], P.dims[m], (int)P.R, P.gram_part, P.G[m], s);
+        if (st) return st;
     }
     return TDX_OK;
 }
@@ -655,30 +689,30 @@ static int enqueue_sweep(CpPlan &P, const Storage &S, hipStream_t s, bool fit) {
         TDX_LAUNCH_CHECK();
     }
     for (int n = 0; n < P.order; ++n) {
-        // fork: V^{-1} on the side stream while the MTTKRP streams X
-        TDX_HIP(hipEventRecord(P.ev_fork[n], s));
-        TDX_HIP(hipStreamWaitEvent(P.side, P.ev_fork[n], 0));
-        k_vinv<<<1, 256, 0, P.side>>>(P.G[0], P.G[1], P.order > 2 ? P.G[2] : P.G[0],
-                                      P.order > 3 ? P.G[3] : P.G[0], P.order, n, R, P.Vinv[n], P.status);
+        // V^{-1} ahead of the MTTKRP on the same stream: a concurrent launch would hold one CU
+        // (and its LDS) while the MTTKRP grid is sized for every CU
+        k_vinv<<<1, 256, 0, s>>>(P.G[0], P.G[1], P.order > 2 ? P.G[2] : P.G[0], P.order > 3 ? P.G[3] : P.G[0],
+                                 P.order, n, R, P.Vinv[n], P.status);
         TDX_LAUNCH_CHECK();
-        TDX_HIP(hipEventRecord(P.ev_join[n], P.side));
         const float *panels[4] = {P.panel[0], P.panel[1], P.panel[2], P.panel[3]};
         const float *row_scale = (S.dtype == TDX_FP8_E4M3 && n == 0) ? S.scales : nullptr;
         const int *skip = (P.ls_mode && n == 0) ? &P.ls->skip : nullptr;
         int st = mttkrp_launch(S.mp[n], S.X, panels, row_scale, P.partial, s, skip);
         if (st) return st;
-        TDX_HIP(hipStreamWaitEvent(s, P.ev_join[n], 0));
         const long In = P.dims[n];
         const int nblk = (int)ceil_div(In, kSolveRows);
         const float *pscale = (S.dtype == TDX_FP8_E4M3 && n == 0) ? S.scales : nullptr;
-        k_solve<<<nblk, 256, 0, s>>>(P.partial, S.mp[n].nsplit, S.mp[n].rows_pad, P.Rp, In, R, P.Vinv[n], P.U[n],
-                                     P.panel[n], pscale, P.ip_part, P.gram_part, skip ? &P.ls->skip : nullptr);
+        const long mel = (long)S.mp[n].rows_pad * P.Rp;
+        k_split_sum<<<(int)ceil_div(mel, 256L), 256, 0, s>>>(P.partial, S.mp[n].nsplit, mel, P.msum);
+        k_solve<<<nblk, 256, 0, s>>>(P.msum, P.Rp, In, R, P.Vinv[n], P.U[n], P.panel[n], pscale, P.ip_part,
+                                     P.gram_part, skip ? &P.ls->skip : nullptr);
+        k_gram_psum<<<(int)ceil_div((long)R * R, 256L), 256, 0, s>>>(P.gram_part, nblk, R * R, P.G[n]);
         TDX_LAUNCH_CHECK();
-        const bool do_fit = fit && (n == P.order - 1);
-        k_gram_reduce<<<1, 256, 0, s>>>(P.gram_part, nblk, R, P.G[n], P.ip_part, P.G[0], P.G[1],
-                                        P.order > 2 ? P.G[2] : P.G[0], P.order > 3 ? P.G[3] : P.G[0], P.order,
-                                        do_fit ? 1 : 0, P.normx2, P.fit_hist, P.counter);
-        TDX_LAUNCH_CHECK();
+        if (fit && n == P.order - 1) {
+            k_fit<<<1, 256, 0, s>>>(P.ip_part, nblk, R, P.G[0], P.G[1], P.order > 2 ? P.G[2] : P.G[0],
+                                    P.order > 3 ? P.G[3] : P.G[0], P.order, P.normx2, P.fit_hist, P.counter);
+            TDX_LAUNCH_CHECK();
+        }
     }
     if (P.ls_mode) return enqueue_ls(P, S, s);
     return TDX_OK;
@@ -698,14 +732,17 @@ static int enqueue_ls(CpPlan &P, const Storage &S, hipStream_t s) {
                                         P.mlp, P.mlp_width, P.feat_hist, P.alpha_hist, P.plain_hist);
         k_extrapolate<<<256, 256, 0, s>>>(P.set_of(P.U), P.set_of(P.U1), P.set_of(P.Ua), P.panel_a[0], P.panel_a[1],
                                           P.panel_a[2], P.panel_a[3], P.Rp, P.ls, sc0);
-        k_gram_set<<<P.order, 256, 0, s>>>(P.set_of(P.Ua), P.Ga[0], P.Ga[1], P.Ga[2], P.Ga[3]);
-        TDX_LAUNCH_CHECK();
+        for (int m = 0; m < P.order; ++m) {
+            int st = gram_of(P.Ua[m], P.dims[m], R, P.gram_part, P.Ga[m], s);
+            if (st) return st;
+        }
         const float *pa[4] = {P.panel_a[0], P.panel_a[1], P.panel_a[2], P.panel_a[3]};
         const float *row_scale = (S.dtype == TDX_FP8_E4M3) ? S.scales : nullptr;
         int st = mttkrp_launch(S.mp[0], S.X, pa, row_scale, P.partial, s, nullptr);
         if (st) return st;
-        k_ls_ip<<<nblk0, 256, 0, s>>>(P.partial, S.mp[0].nsplit, S.mp[0].rows_pad, P.Rp, I0, R, P.Ua[0],
-                                      P.ip_part_a);
+        const long mel0 = (long)S.mp[0].rows_pad * P.Rp;
+        k_split_sum<<<(int)ceil_div(mel0, 256L), 256, 0, s>>>(P.partial, S.mp[0].nsplit, mel0, P.msum);
+        k_ls_ip<<<nblk0, 256, 0, s>>>(P.msum, P.Rp, I0, R, P.Ua[0], P.ip_part_a);
         const double *G3 = P.order > 3 ? P.Ga[3] : P.Ga[0];
         k_ls_decide<<<1, 256, 0, s>>>(P.ip_part_a, nblk0, P.Ga[0], P.Ga[1], P.Ga[2], G3, P.order, R, P.normx2,
                                       P.fit_hist, P.counter, P.ls_mode, g, P.ngrid, P.grid_hist, P.accepted_hist,

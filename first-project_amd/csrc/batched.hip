@@ -140,7 +140,7 @@ __global__ __launch_bounds__(BS) void k_cp_batched_sweep(const float *__restrict
                 V[e] = v;
             }
             __syncthreads();
-            if (block_spd_inverse(V, L, W, Vi, R, cs, sh, &fail) && tid == 0) atomicOr(status, 1);
+            if (block_spd_inverse<(kBMaxR * kBMaxR + BS - 1) / BS>(V, L, W, Vi, R, cs, sh, &fail) && tid == 0) atomicOr(status, 1);
             // U_n = M V^{-1}; <M, U_n> for the fit after the last mode
             double ip = 0.0;
             for (int e = tid; e < In * R; e += BS) {

@@ -94,50 +94,74 @@ __device__ inline void jacobi_eig_block(double *A, double *Q, int n, double *cs,
 }
 
 // Vinv = V^{-1} for a symmetric R x R V (all arrays column-major, R*R doubles, any memory
-// space visible to the whole workgroup; V is preserved).  Right-looking Cholesky on L,
-// W = L^{-1}, Vinv = W^T W.  If V is not numerically SPD, a pseudo-inverse from the Jacobi
+// space visible to the whole workgroup; V is preserved).  If V is not numerically SPD (a
+// relative pivot of the sweep operator below 1e-13), a pseudo-inverse from the Jacobi
 // eigendecomposition (eigenvalues below 1e-12 * max dropped) is returned instead and the
 // function returns 1 (in every thread).  cs needs 4*(R/2+1) doubles, sh blockDim doubles.
+// MAXEPT bounds the elements of V each thread holds: ceil(R^2 / blockDim) <= MAXEPT.
+template <int MAXEPT = 16>
 __device__ inline int block_spd_inverse(const double *V, double *L, double *W, double *Vinv, int R, double *cs,
                                         double *sh, int *fail) {
     const int tid = threadIdx.x, nt = blockDim.x, RR = R * R;
-    for (int e = tid; e < RR; e += nt) L[e] = V[e];
+    // Symmetric sweep operator (Goodnight 1979) on a copy of V, pivots k = 0..R-1 in order:
+    //   d = A[k][k];  A[i][j] -= A[i][k] A[k][j] / d  (i, j != k);
+    //   A[i][k] = A[k][i] = A[i][k] / d;  A[k][k] = -1 / d.
+    // After all R sweeps A = -V^{-1}.  The pivots are the Cholesky pivots L_kk^2; V is treated
+    // as numerically SPD when every pivot exceeds 1e-13 of its original diagonal entry (a
+    // smaller one means column k is dependent on the earlier ones to rounding).
+    // Each thread keeps its elements e = tid + nt u of A in registers; only the pivot column
+    // goes through LDS (W), one barrier pair per pivot.
+    constexpr int kMaxEpt = MAXEPT;
+    const int ept = (RR + nt - 1) / nt;
+    double a[kMaxEpt];
+    int ii[kMaxEpt], jj[kMaxEpt];
+    bool ok[kMaxEpt];
+#pragma unroll
+    for (int u = 0; u < kMaxEpt; ++u) {
+        const int e = tid + nt * u;
+        ok[u] = u < ept && e < RR;
+        ii[u] = ok[u] ? e % R : 0;
+        jj[u] = ok[u] ? e / R : 0;
+        a[u] = ok[u] ? V[e] : 0.0;
+    }
     if (tid == 0) *fail = 0;
     __syncthreads();
-    for (int j = 0; j < R; ++j) {
-        if (tid == 0) {
-            const double d = L[j + j * R];
-            if (!(d > 0.0) || !isfinite(d)) *fail = 1;
-            L[j + j * R] = sqrt(fmax(d, 0.0));
+    for (int k = 0; k < R; ++k) {
+#pragma unroll
+        for (int u = 0; u < kMaxEpt; ++u)
+            if (u < ept && ok[u] && jj[u] == k) W[ii[u]] = a[u];
+        __syncthreads();
+        const double d = W[k];
+        if (!(d > 1e-13 * V[k + k * R]) || !isfinite(d)) {
+            if (tid == 0) *fail = 1;
+            break;  // uniform: every thread read the same pivot
         }
-        __syncthreads();
-        if (*fail) break;
-        const double d = L[j + j * R];
-        for (int i = j + 1 + tid; i < R; i += nt) L[i + j * R] /= d;
-        __syncthreads();
-        const int m = R - j - 1;
-        for (int e = tid; e < m * m; e += nt) {
-            const int i = j + 1 + e % m, k = j + 1 + e / m;
-            if (k <= i) L[i + k * R] -= L[i + j * R] * L[k + j * R];
-        }
-        __syncthreads();
-    }
-    if (!*fail) {
-        for (int c = tid; c < R; c += nt) {
-            for (int i = 0; i < c; ++i) W[i + c * R] = 0.0;
-            for (int i = c; i < R; ++i) {
-                double s = (i == c) ? 1.0 : 0.0;
-                for (int k = c; k < i; ++k) s -= L[i + k * R] * W[k + c * R];
-                W[i + c * R] = s / L[i + i * R];
+        const double dinv = 1.0 / d;
+        // all reads of the pivot column first, then a branch-free update
+        double wi[kMaxEpt], wj[kMaxEpt];
+#pragma unroll
+        for (int u = 0; u < kMaxEpt; ++u)
+            if (u < ept) {
+                wi[u] = W[ii[u]];
+                wj[u] = W[jj[u]];
             }
-        }
-        __syncthreads();
-        for (int e = tid; e < RR; e += nt) {
-            const int a = e % R, b = e / R, k0 = a > b ? a : b;
-            double s = 0.0;
-            for (int k = k0; k < R; ++k) s += W[k + a * R] * W[k + b * R];
-            Vinv[e] = s;
-        }
+#pragma unroll
+        for (int u = 0; u < kMaxEpt; ++u)
+            if (u < ept) {
+                const bool ki = ii[u] == k, kj = jj[u] == k;
+                double v = fma(-wi[u] * dinv, wj[u], a[u]);
+                v = kj ? wi[u] * dinv : v;
+                v = ki ? wj[u] * dinv : v;
+                v = (ki && kj) ? -dinv : v;
+                a[u] = v;
+            }
+        __syncthreads();  // W is rewritten by the next pivot
+    }
+    __syncthreads();
+    if (!*fail) {
+#pragma unroll
+        for (int u = 0; u < kMaxEpt; ++u)
+            if (ok[u]) Vinv[tid + nt * u] = -a[u];
         __syncthreads();
         return 0;
     }

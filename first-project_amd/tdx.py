@@ -121,6 +121,13 @@ def lib():
     if _LIB is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"libtdx.so not built at {LIB_PATH}; run __graft_entry__.build()")
+        # torch ships its own HIP runtime under the same soname as ROCm's; whichever is loaded
+        # first serves the whole process, so load torch first and let libtdx bind to it (the
+        # tensors this module passes are torch allocations)
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, args in _SIGS.items():
             fn = getattr(L, name, None)
