@@ -25,6 +25,7 @@
 // split, are summed in LDS in a fixed order, and the workgroup writes one FP32 partial.
 // Partials are summed over splits in FP64 by the consumer (mttkrp_reduce or the ALS solve).
 #include "mttkrp.hpp"
+#include <stdlib.h>
 
 namespace tdx {
 
@@ -448,6 +449,16 @@ __global__ void k_mttkrp_reduce(const float *__restrict__ partial, float *__rest
 }
 
 // ---------------------------------------------------------------------------
+// TDX_MTTKRP_LDS=0 selects the register-tile kernel for every mode (A/B comparisons)
+static bool lds_enabled() {
+    static int on = -1;
+    if (on < 0) {
+        const char *e = getenv("TDX_MTTKRP_LDS");
+        on = (e && e[0] == '0') ? 0 : 1;
+    }
+    return on == 1;
+}
+
 static int storage_type(int dtype) {
     return dtype == TDX_F32 ? 0 : dtype == TDX_BF16 ? 1 : dtype == TDX_FP8_E4M3 ? 2 : -1;
 }
@@ -473,6 +484,28 @@ int mttkrp_plan(MttkrpPlan &p, int order, const long *dims, int mode, int dtype,
     const size_t align = (st == 2) ? 8 : 16;
     p.fast = rank <= 64 && (dims[0] % vec == 0) && (((uintptr_t)X) % align == 0);
     if (target_wgs <= 0) target_wgs = 512;
+    // output modes > 0: the LDS-staged kernel when mode 0 holds whole 16-byte vectors
+    const long lvec = (st == 0) ? 4 : (st == 1) ? 8 : 16;
+    if (lds_enabled() && p.fast && mttkrp_lds_covers(order, mode, rank) && dims[0] % lvec == 0 &&
+        ((uintptr_t)X) % 16 == 0) {
+        p.lds = true;
+        p.rows_per_wg = 128;
+        p.rowtiles = (int)ceil_div(In, 128);
+        p.rows_pad = (long)p.rowtiles * 128;
+        long S = 1;
+        for (int m = 1; m < order; ++m)
+            if (m != mode) S *= dims[m];
+        p.ksteps = S;
+        // grid (measured on the 1024^3 rank-64 tensor): 512 workgroups (two per CU) for FP32
+        // and BF16; FP8 tiles carry half the bytes per step and prefer 2048
+        const long lds_wgs = (st == 2) ? 2048 : 512;
+        long ns = ceil_div(lds_wgs, (long)p.rowtiles);
+        if (ns > S) ns = S;
+        if (ns < 1) ns = 1;
+        p.ksteps_per_split = ceil_div(S, ns);
+        p.nsplit = (int)ceil_div(S, p.ksteps_per_split);
+        return TDX_OK;
+    }
     if (p.fast) {
         const int KS = (st == 0) ? 16 : 32;
         p.rows_per_wg = (st == 0) ? 64 : 128;
@@ -562,6 +595,7 @@ int mttkrp_launch(const MttkrpPlan &p, const void *X, const float *const *panels
         if (m != p.mode && m != a.fmode) a.slow[a.nslow++] = m;
     const int stt = storage_type(p.dtype);
     if (p.dtype == TDX_FP8_E4M3 && p.mode == 0 && !row_scale) return record(TDX_ERR_ARG);
+    if (p.lds) return mttkrp_lds_launch(p, X, panels, partial, s, skip);
     if (p.fast) {
         const int KS = (stt == 0) ? 16 : 32;
         a.FB = ceil_div(a.If, KS);

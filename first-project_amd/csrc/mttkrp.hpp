@@ -20,6 +20,7 @@ struct MttkrpPlan {
     long rank = 0;
     int Rp = 0;          // rank padded to a multiple of 16
     bool fast = false;   // MFMA register-tile kernel usable
+    bool lds = false;    // LDS-staged kernel (output mode > 0); ksteps = slow indices
     int rows_per_wg = 0; // output rows per workgroup tile
     int rowtiles = 0;
     long rows_pad = 0;   // rowtiles * rows_per_wg
@@ -38,6 +39,11 @@ int mttkrp_plan(MttkrpPlan &p, int order, const long *dims, int mode, int dtype,
 // pre-scaled by the slice scales when mode != 0, and row_scale = scales when mode == 0.
 int mttkrp_launch(const MttkrpPlan &p, const void *X, const float *const *panels,
                   const float *row_scale, float *partial, hipStream_t s, const int *skip = nullptr);
+
+// the LDS-staged kernel for output modes > 0 (mttkrp_lds.hip)
+bool mttkrp_lds_covers(int order, int mode, long rank);
+int mttkrp_lds_launch(const MttkrpPlan &p, const void *X, const float *const *panels, float *partial,
+                      hipStream_t s, const int *skip);
 
 // panel packing (optionally scaling row i by scale[i])
 int pack_panel(const float *U, long I, long R, int Rp, const float *scale, float *panel, hipStream_t s);
