@@ -6,7 +6,7 @@
 // a one-thread kernel at the start of every sweep so the sweep can be replayed as a graph)
 // and performs the whole Gauss-Seidel sweep of its tensor locally:
 //   for n in 0..2:  M = MTTKRP_n streamed slab by slab (X(:,:,k0:k0+kc) in LDS, coalesced),
-//                   V = hadamard of the other Grams, FP64 Cholesky inverse (Jacobi
+//                   V = hadamard of the other Grams, FP64 sweep-operator inverse (Jacobi
 //                   pseudo-inverse fallback), U_n = M V^{-1}, G_n = U_n^T U_n (FP64)
 //   relerr from the fast-fit identity, appended to the tensor's history.
 // Every tensor element is read from HBM once per mode, exactly like the single-tensor path.
