@@ -10,11 +10,14 @@
 // The Gram never forms an unfolded copy (the reference materialises X2 with t2m and, for
 // order 4, X2 and X3 by identity-matrix GEMMs, src/tucker4_hosvd.cu:23-37): a workgroup
 // stages 32 fibres of the (P, In, Q) view in LDS and accumulates an In x In FP32 tile,
-// partial tiles are summed in FP64 in a fixed order.  The eigensolve is FP64 cyclic
-// Jacobi in one workgroup; eigenvector signs are fixed so that the entry of largest
-// magnitude is positive (syevdx leaves the sign unspecified).
+// partial tiles are summed in FP64 in a fixed order.  The eigensolve is the FP64
+// tridiagonal solver in one workgroup (subspace iteration first where the top eigenvalues
+// stand clear of the rest, warm-started from the previous factor in HOOI); eigenvector signs
+// are fixed so that the entry of largest magnitude is positive (syevdx leaves the sign
+// unspecified).
 #include "common.hpp"
 #include "eig.hpp"
+#include "eig_sub.hpp"
 #include "linalg.hpp"
 #include <vector>
 
@@ -95,12 +98,23 @@ __global__ void k_gram_sum(const float *__restrict__ part, int nblk, long In, do
 }
 
 // Top-r eigenvectors of the symmetric n x n G (FP64), one workgroup: U (n x r, col-major
-// FP32) receives the eigenvectors of the r largest eigenvalues in ascending order.
+// FP32) receives the eigenvectors of the r largest eigenvalues in ascending order.  With
+// try_sub the subspace-iteration path runs first (start block U0, or a built-in one when U0
+// is null; U0 may alias U) and the tridiagonal solver only when it declines.  method (optional,
+// device) receives 1 for the subspace path, 0 for the tridiagonal one.
 __global__ __launch_bounds__(kEigThreads) void k_top_eigvecs(const double *__restrict__ G, int n, int r,
-                                                             float *__restrict__ U, double *__restrict__ w,
-                                                             double *__restrict__ scratch) {
+                                                             const float *U0, int try_sub, float *U, double *w,
+                                                             double *__restrict__ scratch, int *method) {
     extern __shared__ double smem[];
+    if (try_sub && r <= kSubMaxR) {
+        if (sym_top_eigvecs_subspace(G, n, r, U0, U, w, smem)) {
+            if (method && threadIdx.x == 0) *method = 1;
+            return;
+        }
+        __syncthreads();  // the LDS is reused below
+    }
     sym_top_eigvecs_block(G, n, r, U, w, scratch, scratch + (long)n * n, smem);
+    if (method && threadIdx.x == 0) *method = 0;
 }
 
 __global__ void k_sumsq(const float *__restrict__ X, long n, double *__restrict__ out) {
@@ -139,10 +153,10 @@ struct TuckerWs {
     int gram_blocks = 0;
     double *G = nullptr;
     double *Q = nullptr;
-    float *y0 = nullptr, *y1 = nullptr;
+    float *y0 = nullptr, *y1 = nullptr, *y2 = nullptr;
     double *scal = nullptr;
     ~TuckerWs() {
-        void *ps[] = {gram_part, G, Q, y0, y1, scal};
+        void *ps[] = {gram_part, G, Q, y0, y1, y2, scal};
         for (void *p : ps)
             if (p) (void)hipFree(p);
     }
@@ -181,17 +195,35 @@ static int unfold_gram(const void *X, int xtype, int order, const long *dims, in
     return TDX_OK;
 }
 
-// scratch: 2 n^2 doubles
-static int top_eigvecs(const double *G, long n, long r, double *scratch, float *U, double *w, hipStream_t s) {
+// TDX_EIG_SUBSPACE=0 keeps every eigensolve on the tridiagonal path (A/B comparisons)
+static bool subspace_enabled() {
+    static int on = -1;
+    if (on < 0) {
+        const char *e = getenv("TDX_EIG_SUBSPACE");
+        on = (e && e[0] == '0') ? 0 : 1;
+    }
+    return on != 0;
+}
+
+static long eig_kernel_lds_doubles(int n) {
+    const long a = eig_lds_doubles(n), b = eig_sub_doubles(n);
+    return a > b ? a : b;
+}
+
+// scratch: 2 n^2 doubles.  try_sub: try the subspace path first, starting from U0 (or the
+// built-in block when U0 is null).
+static int top_eigvecs(const double *G, long n, long r, double *scratch, const float *U0, int try_sub, float *U,
+                       double *w, int *method, hipStream_t s) {
     if (n < 1 || n > kEigMaxN || r < 1 || r > n) return record(TDX_ERR_ARG);
     static bool attr_set = false;
     if (!attr_set) {
         TDX_HIP(hipFuncSetAttribute((const void *)k_top_eigvecs, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)(sizeof(double) * eig_lds_doubles(kEigMaxN))));
+                                    (int)(sizeof(double) * eig_kernel_lds_doubles(kEigMaxN))));
         attr_set = true;
     }
-    k_top_eigvecs<<<1, kEigThreads, sizeof(double) * eig_lds_doubles((int)n), s>>>(G, (int)n, (int)r, U, w,
-                                                                                   scratch);
+    const int sub = (try_sub && subspace_enabled()) ? 1 : 0;
+    k_top_eigvecs<<<1, kEigThreads, sizeof(double) * eig_kernel_lds_doubles((int)n), s>>>(
+        G, (int)n, (int)r, U0, sub, U, w, scratch, method);
     TDX_LAUNCH_CHECK();
     return TDX_OK;
 }
@@ -212,18 +244,18 @@ static int ws_setup(TuckerWs &W, int order, const long *dims, const long *ranks)
     }
     TDX_HIP(hipMalloc(&W.y0, sizeof(float) * maxsz));
     TDX_HIP(hipMalloc(&W.y1, sizeof(float) * maxsz));
+    TDX_HIP(hipMalloc(&W.y2, sizeof(float) * maxsz));
     return TDX_OK;
 }
 
-// Y = X x_{m in modes} U_m^T applied in the listed order; returns the result pointer
-// (one of the workspace buffers) and the resulting dims in cur
-static int ttm_chain(const void *X, int xtype, int order, const long *dims, const long *ranks, float *const *U,
-                     const int *modes, int nmodes, TuckerWs &W, long *cur, const float **out, hipStream_t s) {
-    for (int m = 0; m < order; ++m) cur[m] = dims[m];
+// Y = X x_{m in modes} U_m^T applied in the listed order, results alternating between the
+// buffers a and b (X itself when nmodes == 0); cur: the dims of X in, of the result out
+static int ttm_seq(const void *X, int xtype, int order, long *cur, const long *ranks, float *const *U,
+                   const int *modes, int nmodes, float *a, float *b, const float **out, hipStream_t s) {
     const void *src = X;
     for (int t = 0; t < nmodes; ++t) {
         const int m = modes[t];
-        float *dst = (t & 1) ? W.y1 : W.y0;
+        float *dst = (t & 1) ? b : a;
         int st = ttm_generic_t(src, t == 0 ? xtype : TDX_F32, order, cur, m, U[m], cur[m], 1, ranks[m], dst, s);
         if (st) return st;
         cur[m] = ranks[m];
@@ -231,6 +263,19 @@ static int ttm_chain(const void *X, int xtype, int order, const long *dims, cons
     }
     *out = (const float *)src;
     return TDX_OK;
+}
+
+// core = X x_1 U_1^T ... x_d U_d^T from the FP32 tensor, slowest mode first
+static int core_from(const float *X, int order, const long *dims, const long *ranks, float *const *U, TuckerWs &W,
+                     float *core, hipStream_t s) {
+    int modes[4], nm = 0;
+    for (int m = order - 1; m >= 1; --m) modes[nm++] = m;
+    long cur[4];
+    for (int m = 0; m < order; ++m) cur[m] = dims[m];
+    const float *Y = nullptr;
+    int st = ttm_seq(X, TDX_F32, order, cur, ranks, U, modes, nm, W.y0, W.y1, &Y, s);
+    if (st) return st;
+    return ttm_generic(Y, order, cur, 0, U[0], dims[0], 1, ranks[0], core, s);
 }
 
 static int tucker_run(const float *X, int order, const long *dims, const long *ranks, int iters, float *core,
@@ -266,37 +311,52 @@ static int tucker_run(const float *X, int order, const long *dims, const long *r
     // HOSVD init
     for (int m = 0; m < order; ++m) {
         st = unfold_gram(Xs, xtype, order, dims, m, W, W.G, s);
-        if (!st) st = top_eigvecs(W.G, dims[m], ranks[m], W.Q, U[m], nullptr, s);
+        if (!st) st = top_eigvecs(W.G, dims[m], ranks[m], W.Q, nullptr, 1, U[m], nullptr, nullptr, s);
         if (st) return st;
     }
     k_norm2_blocks<float><<<1024, 256, 0, s>>>(X, N, W.scal);
     k_sum_to<<<1, 256, 0, s>>>(W.scal, 1024, W.scal + 1024);
     TDX_LAUNCH_CHECK();
     double nx2 = 0.0;
-    long cur[4];
-    const float *Y = nullptr;
+    long nc = 1;
+    for (int m = 0; m < order; ++m) nc *= ranks[m];
+    // HOOI with a dimension tree: the modes split into L = {0..h-1} and R = {h..d-1};
+    // T = X x_{R} U^T serves every n in L, then T = X x_{L} U^T (with the new U_L) every n in
+    // R, so X is streamed twice per iteration rather than d times.  Each T contracts its
+    // slowest mode first.
+    const int h = order / 2;
     for (int it = 0; it < iters; ++it) {
-        for (int n = 0; n < order; ++n) {
-            int modes[4], nm = 0;
-            for (int m = 0; m < order; ++m)
-                if (m != n) modes[nm++] = m;
-            st = ttm_chain(Xs, xtype, order, dims, ranks, U, modes, nm, W, cur, &Y, s);
-            if (!st) st = unfold_gram(Y, TDX_F32, order, cur, n, W, W.G, s);
-            if (!st) st = top_eigvecs(W.G, dims[n], ranks[n], W.Q, U[n], nullptr, s);
+        const float *Y = nullptr;
+        long ydims[4];
+        for (int half = 0; half < 2; ++half) {
+            const int n0 = half ? h : 0, n1 = half ? order : h;
+            int tm[4], nt = 0;
+            for (int m = order - 1; m >= 0; --m)
+                if (m < n0 || m >= n1) tm[nt++] = m;
+            long tdims[4];
+            for (int m = 0; m < order; ++m) tdims[m] = dims[m];
+            const float *T = nullptr;
+            st = ttm_seq(Xs, xtype, order, tdims, ranks, U, tm, nt, W.y0, W.y1, &T, s);
             if (st) return st;
-        }
-        if (relerr_hist) {
-            // core from the last chain: Y x_{d-1} U_{d-1}^T; relerr^2 = 1 - ||core||^2/||X||^2.
-            // With BF16 storage the error is measured against the FP32 tensor: the core is
-            // recomputed from a chain that starts on X itself.
-            long nc = 1;
-            for (int m = 0; m < order; ++m) nc *= ranks[m];
-            if (bf16) {
-                int fm[4] = {0, 1, 2, 3};
-                st = ttm_chain(X, TDX_F32, order, dims, ranks, U, fm, order - 1, W, cur, &Y, s);
+            float *fa = (T == W.y0) ? W.y1 : W.y0;
+            for (int n = n0; n < n1; ++n) {
+                int sm[4], ns = 0;
+                for (int m = n1 - 1; m >= n0; --m)
+                    if (m != n) sm[ns++] = m;
+                for (int m = 0; m < order; ++m) ydims[m] = tdims[m];
+                st = ttm_seq(T, TDX_F32, order, ydims, ranks, U, sm, ns, W.y2, fa, &Y, s);
+                if (!st) st = unfold_gram(Y, TDX_F32, order, ydims, n, W, W.G, s);
+                // warm start: the factor of the previous iteration spans nearly the same subspace
+                if (!st) st = top_eigvecs(W.G, dims[n], ranks[n], W.Q, U[n], 1, U[n], nullptr, nullptr, s);
                 if (st) return st;
             }
-            st = ttm_generic(Y, order, cur, order - 1, U[order - 1], dims[order - 1], 1, ranks[order - 1], core, s);
+        }
+        if (relerr_hist) {
+            // relerr^2 = 1 - ||core||^2/||X||^2.  The core comes from the last Y (all modes
+            // but d-1 contracted); with BF16 storage it is recomputed from the FP32 tensor.
+            if (bf16) st = core_from(X, order, dims, ranks, U, W, core, s);
+            else st = ttm_generic(Y, order, ydims, order - 1, U[order - 1], dims[order - 1], 1, ranks[order - 1],
+                                  core, s);
             if (st) return st;
             {
                 // ||core||^2: block partials (the first 1024 slots are free once ||X||^2 is summed)
@@ -304,18 +364,15 @@ static int tucker_run(const float *X, int order, const long *dims, const long *r
                 k_norm2_blocks<float><<<nb, 256, 0, s>>>(core, nc, W.scal);
                 k_sum_to<<<1, 256, 0, s>>>(W.scal, nb, W.scal + 1025);
             }
-            double h[2];
-            TDX_HIP(hipMemcpyAsync(h, W.scal + 1024, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+            double hv[2];
+            TDX_HIP(hipMemcpyAsync(hv, W.scal + 1024, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
             TDX_HIP(hipStreamSynchronize(s));
-            nx2 = h[0];
-            relerr_hist[it] = sqrt(fmax(nx2 - h[1], 0.0) / nx2);
+            nx2 = hv[0];
+            relerr_hist[it] = sqrt(fmax(nx2 - hv[1], 0.0) / nx2);
         }
     }
     // final core, always from the FP32 tensor
-    int modes[4] = {0, 1, 2, 3};
-    st = ttm_chain(X, TDX_F32, order, dims, ranks, U, modes, order - 1, W, cur, &Y, s);
-    if (st) return st;
-    st = ttm_generic(Y, order, cur, order - 1, U[order - 1], dims[order - 1], 1, ranks[order - 1], core, s);
+    st = core_from(X, order, dims, ranks, U, W, core, s);
     if (st) return st;
     TDX_HIP(hipStreamSynchronize(s));
     return TDX_OK;
@@ -343,14 +400,23 @@ int tdx_unfold_gram(const float *X, int order, const long *dims, int mode, doubl
     return tdx_unfold_gram_ex(X, TDX_F32, order, dims, mode, G);
 }
 
-int tdx_sym_eig_topr(const double *G, long n, long r, float *U, double *w) {
+int tdx_sym_eig_topr_ex(const double *G, long n, long r, const float *U0, int flags, float *U, double *w,
+                        int *method) {
     TDX_CHECK_ARG(G && U && n >= 1 && n <= kEigMaxN && r >= 1 && r <= n);
     void *scr = nullptr;
-    int st = workspace(WS_MISC2, sizeof(double) * 2 * n * n, &scr);
+    int st = workspace(WS_MISC2, sizeof(double) * 2 * n * n + 256, &scr);
     if (st) return st;
-    st = top_eigvecs(G, n, r, (double *)scr, U, w, stream());
+    int *dmethod = (int *)((char *)scr + sizeof(double) * 2 * n * n);
+    st = top_eigvecs(G, n, r, (double *)scr, U0, (flags & TDX_EIG_TRY_SUBSPACE) ? 1 : 0, U, w, dmethod, stream());
+    int hm = 0;
+    if (!st) TDX_HIP(hipMemcpyAsync(&hm, dmethod, sizeof(int), hipMemcpyDeviceToHost, stream()));
     if (!st) TDX_HIP(hipStreamSynchronize(stream()));
+    if (!st && method) *method = hm;
     return record(st);
+}
+
+int tdx_sym_eig_topr(const double *G, long n, long r, float *U, double *w) {
+    return tdx_sym_eig_topr_ex(G, n, r, nullptr, 0, U, w, nullptr);
 }
 
 int tdx_tucker_hooi(const float *X, int order, const long *dims, const long *ranks, int iters, int bf16_ttm,

@@ -76,6 +76,7 @@ _SIGS = {
     "tdx_ttm_t": [_P, ctypes.c_int, ctypes.POINTER(_L), ctypes.c_int, _P, _L, _P],
     "tdx_unfold_gram": [_P, ctypes.c_int, ctypes.POINTER(_L), ctypes.c_int, _P],
     "tdx_sym_eig_topr": [_P, _L, _L, _P, _P],
+    "tdx_sym_eig_topr_ex": [_P, _L, _L, _P, ctypes.c_int, _P, _P, _P],
     "tdx_ttm_ex": [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_L), ctypes.c_int, _P, _L, _P],
     "tdx_unfold_gram_ex": [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_L), ctypes.c_int, _P],
     "tdx_rse": [_P, _P, _L, ctypes.POINTER(ctypes.c_float)],
@@ -351,6 +352,18 @@ def sym_eig_topr(G, n, r, U, w=None):
     _call("tdx_sym_eig_topr", ptr(G), n, r, ptr(U), ptr(w))
 
 
+TDX_EIG_TRY_SUBSPACE = 1
+
+
+def sym_eig_topr_ex(G, n, r, U, w=None, U0=None, subspace=True):
+    """sym_eig_topr with the subspace path tried first (start block U0, FP32 n x r, or the
+    built-in one); returns 1 if the subspace path produced U, 0 if the tridiagonal one did."""
+    m = ctypes.c_int(-1)
+    _call("tdx_sym_eig_topr_ex", ptr(G), n, r, ptr(U0), TDX_EIG_TRY_SUBSPACE if subspace else 0, ptr(U), ptr(w),
+          ctypes.byref(m))
+    return m.value
+
+
 def ttm_ex(X, dims, mode, U, r, Y, dtype=TDX_F32):
     """Y = X x_mode U^T with U (I_mode x r) column-major; X FP32 or BF16 bits."""
     _call("tdx_ttm_ex", ptr(X), dtype, len(dims), dims_arr(dims), mode, ptr(U), r, ptr(Y))
@@ -360,12 +373,14 @@ def unfold_gram(X, dims, mode, G, dtype=TDX_F32):
     _call("tdx_unfold_gram_ex", ptr(X), dtype, len(dims), dims_arr(dims), mode, ptr(G))
 
 
-def tucker_hooi(X, dims, ranks, iters, core, U, bf16_ttm=False):
+def tucker_hooi(X, dims, ranks, iters, core, U, bf16_ttm=False, history=True):
+    """HOSVD + `iters` HOOI iterations; returns the relative error after each iteration
+    (None with history=False, which skips computing them)."""
     import numpy as np
-    hist = np.zeros(max(iters, 1))
+    hist = np.zeros(max(iters, 1)) if history else None
     _call("tdx_tucker_hooi", ptr(X), len(dims), dims_arr(dims), dims_arr(ranks), iters, 1 if bf16_ttm else 0,
-          ptr(core), ptr_arr(U), hist.ctypes.data)
-    return hist[:iters]
+          ptr(core), ptr_arr(U), hist.ctypes.data if history else None)
+    return hist[:iters] if history else None
 
 
 class Batch:

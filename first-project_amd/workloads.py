@@ -107,19 +107,24 @@ def c4(ntensors=4096, seed=0, lo=16, hi=96):
     return X, b
 
 
-def run_c3(iters=10):
-    """HOSVD + `iters` HOOI iterations of the C3 tensor; returns (seconds, rel-err per iteration)."""
+def run_c3(iters=10, bf16=False, history=False, data=None):
+    """HOSVD + `iters` HOOI iterations of the C3 tensor (BF16 Grams and first TTMs with
+    bf16=True); returns (seconds, final relative error from the core, per-iteration errors or
+    None).  The final error is sqrt(1 - ||core||^2/||X||^2) in FP64, outside the timed call."""
     import time
 
     import torch
-    X, dims, ranks = c3()
+    X, dims, ranks = data if data is not None else c3()
     U = [torch.empty(d * r, device="cuda") for d, r in zip(dims, ranks)]
     core = torch.empty(int(np.prod(ranks)), device="cuda")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    hist = tdx.tucker_hooi(X, dims, ranks, iters, core, U)
+    hist = tdx.tucker_hooi(X, dims, ranks, iters, core, U, bf16_ttm=bf16, history=history)
     torch.cuda.synchronize()
-    return time.perf_counter() - t0, hist
+    sec = time.perf_counter() - t0
+    nx2 = float(X.double().square().sum())
+    nc2 = float(core.double().square().sum())
+    return sec, float(np.sqrt(max(nx2 - nc2, 0.0) / nx2)), hist
 
 
 if __name__ == "__main__":
@@ -127,7 +132,10 @@ if __name__ == "__main__":
     ap = argparse.ArgumentParser(description="run one benchmark recipe on cuda:0")
     ap.add_argument("recipe", choices=["c3"])
     ap.add_argument("--iters", type=int, default=10)
+    ap.add_argument("--bf16", action="store_true", help="BF16 Grams and first TTMs")
+    ap.add_argument("--history", action="store_true", help="relative error after every iteration")
     a = ap.parse_args()
     tdx.lib()
-    sec, hist = run_c3(a.iters)
-    print(f"c3: {sec:.4f} s, rel-err {list(map(float, hist))}")
+    sec, rel, hist = run_c3(a.iters, a.bf16, a.history)
+    print(f"c3 ({'bf16' if a.bf16 else 'fp32'}): {sec:.4f} s, final rel-err {rel:.6e}"
+          + (f", per iteration {list(map(float, hist))}" if hist is not None else ""))

@@ -108,6 +108,15 @@ int tdx_unfold_gram_ex(const void *X, int dtype, int order, const long *dims, in
  * symmetric FP64 n x n G, ascending, sign fixed so the largest-magnitude entry is positive;
  * w (n x 1, optional, NULL to skip) receives those r eigenvalues.  n <= 128. */
 int tdx_sym_eig_topr(const double *G, long n, long r, float *U, double *w);
+/* new: tdx_sym_eig_topr with a choice of method.  flags & TDX_EIG_TRY_SUBSPACE: first try
+ * subspace iteration + Rayleigh-Ritz (r <= 32) from the start block U0 (n x r FP32, may be
+ * NULL for a built-in one, may alias U), accepted only when certified to give the r largest
+ * eigenvalues to 1e-12 relative residual; otherwise the tridiagonal solver runs.  *method
+ * (host, optional) = 1 if the subspace path produced U, 0 otherwise.  Same output
+ * conventions as tdx_sym_eig_topr. */
+#define TDX_EIG_TRY_SUBSPACE 1
+int tdx_sym_eig_topr_ex(const double *G, long n, long r, const float *U0, int flags, float *U, double *w,
+                        int *method);
 
 /* ---- reconstruction and error ---- */
 /* rse: src/rse.cu:4-15 -- *error (host) = ||X - X1|| / ||X||.  Unlike the reference, X1
@@ -248,9 +257,12 @@ int tdx_tucker_hosvd4(const float *X, float *core, float *U1, float *U2, float *
 /* tucker_tensorcore: src/tucker_tensorcore.cu:3-139 -- BF16 TTM chain on MFMA, FP32 accumulation */
 int tdx_tucker_tensorcore(const float *X, float *core, float *U1, float *U2, float *U3,
                           long a, long b, long c, long r1, long r2, long r3);
-/* new: HOSVD init + `iters` HOOI iterations for order 3/4
+/* new: HOSVD init + `iters` HOOI iterations for orders 2-4
  * (decomposition/GPU/normal/tucker/tucker_hooi.cu:3-218, generalised).  bf16_ttm = 1 runs
- * the first TTM of every chain from a BF16 copy of X.  relerr_hist (host, may be NULL). */
+ * the Grams of X and the first TTM of every chain from a BF16 copy of X.  relerr_hist (host,
+ * may be NULL): the relative error after each iteration, from a core of the FP32 tensor
+ * (an extra FP32 TTM chain per iteration when bf16_ttm = 1); the final core is always
+ * computed from the FP32 tensor. */
 int tdx_tucker_hooi(const float *X, int order, const long *dims, const long *ranks, int iters,
                     int bf16_ttm, float *core, float *const *U, double *relerr_hist);
 

@@ -42,22 +42,33 @@ def _sym_cases():
     return out
 
 
+@pytest.mark.parametrize("sub", [False, True], ids=["tridiag", "try_subspace"])
 @pytest.mark.parametrize("name,G,r,unique", _sym_cases(), ids=[c[0] for c in _sym_cases()])
-def test_sym_eig_topr_against_lapack(name, G, r, unique):
+def test_sym_eig_topr_against_lapack(name, G, r, unique, sub):
     """Tridiagonalisation + multisection + inverse iteration against LAPACK's eigh (the
     reference calls cusolverDnDsyevdx): eigenvalues, residuals, orthonormality, and the
-    vectors themselves (same sign convention) where the top-r eigenvalues are distinct."""
+    vectors themselves (same sign convention) where the top-r eigenvalues are distinct.
+    With the subspace path tried first, the same checks hold whichever path answers
+    (eigenvalues to 1e-11 where the subspace path does): it must take the low-rank case and
+    must decline the zero matrix and the tied top-r set."""
     import torch
     n = G.shape[0]
     U = empty(n * r)
     w = torch.empty(r, dtype=torch.float64, device="cuda")
-    tdx.sym_eig_topr(dev(np.ascontiguousarray(G.ravel(order="F"), np.float64)), n, r, U, w)
+    Gd = dev(np.ascontiguousarray(G.ravel(order="F"), np.float64))
+    m = 0
+    if sub:
+        m = tdx.sym_eig_topr_ex(Gd, n, r, U, w)
+        if name in ("lowrank", "zero", "repeated"):
+            assert m == (1 if name == "lowrank" else 0), name
+    else:
+        tdx.sym_eig_topr(Gd, n, r, U, w)
     Ug = host(U).reshape((n, r), order="F").astype(np.float64)
     wg = w.cpu().numpy()
     wl, Vl = np.linalg.eigh(G)
     wl, Vl = wl[n - r:], Vl[:, n - r:]
     scale = max(1.0, np.max(np.abs(wl)))
-    assert np.max(np.abs(wg - wl)) <= 1e-13 * scale
+    assert np.max(np.abs(wg - wl)) <= (1e-11 if m else 1e-13) * scale
     assert np.max(np.abs(Ug.T @ Ug - np.eye(r))) < 1e-6
     assert np.max(np.linalg.norm(G @ Ug - Ug * wg, axis=0)) <= 1e-6 * scale
     if unique:
@@ -66,6 +77,74 @@ def test_sym_eig_topr_against_lapack(name, G, r, unique):
             if Vl[k, c] < 0:
                 Vl[:, c] = -Vl[:, c]
         assert np.max(np.abs(Ug - Vl)) < 1e-5
+
+
+def _signed_top(G, r):
+    n = G.shape[0]
+    wl, Vl = np.linalg.eigh(G)
+    wl, Vl = wl[n - r:], Vl[:, n - r:].copy()
+    for c in range(r):
+        k = int(np.argmax(np.abs(Vl[:, c])))
+        if Vl[k, c] < 0:
+            Vl[:, c] = -Vl[:, c]
+    return wl, Vl
+
+
+def _gapped_cases():
+    """Grams of rank-r-plus-0.5%-noise blocks: the top r eigenvalues stand clear of the rest."""
+    rng = np.random.default_rng(11)
+    out = []
+    for n, r, k in ((128, 32, 4096), (128, 32, 256), (64, 16, 300), (24, 5, 200), (12, 3, 50), (5, 5, 20),
+                    (7, 1, 30)):
+        A = rng.standard_normal((n, r)) @ rng.standard_normal((r, k))
+        A += 5e-3 * np.linalg.norm(A) / np.sqrt(n * k) * rng.standard_normal((n, k))
+        out.append((f"gram{n}_{r}_{k}", A @ A.T, r))
+    return out
+
+
+@pytest.mark.parametrize("name,G,r", _gapped_cases(), ids=[c[0] for c in _gapped_cases()])
+def test_sym_eig_subspace_path(name, G, r):
+    """Subspace iteration + Rayleigh-Ritz where the top-r eigenvalues stand clear of the rest:
+    taken (method 1) from the built-in start, from a perturbed warm start and from a warm start
+    in the output buffer itself, each matching LAPACK's eigenpairs.  Ritz values carry the
+    1e-12 relative residual the path stops at, hence 1e-11 on eigenvalues (the tridiagonal
+    solver reaches 1e-13)."""
+    import torch
+    n = G.shape[0]
+    Gd = dev(np.ascontiguousarray(G.ravel(order="F"), np.float64))
+    wl, Vl = _signed_top(G, r)
+    scale = np.max(np.abs(wl))
+    rng = np.random.default_rng(5)
+    P = (Vl + 1e-3 * rng.standard_normal(Vl.shape)).ravel(order="F").astype(np.float32)
+    for start in ("cold", "warm", "inplace"):
+        U = dev(P) if start == "inplace" else empty(n * r)
+        w = torch.empty(r, dtype=torch.float64, device="cuda")
+        U0 = {"cold": None, "warm": dev(P), "inplace": U}[start]
+        m = tdx.sym_eig_topr_ex(Gd, n, r, U, w, U0=U0)
+        assert m == 1, (name, start)
+        Ug = host(U).reshape((n, r), order="F").astype(np.float64)
+        assert np.max(np.abs(w.cpu().numpy() - wl)) <= 1e-11 * scale, (name, start)
+        assert np.max(np.abs(Ug - Vl)) < 1e-5, (name, start)
+
+
+def test_sym_eig_subspace_declines_wrong_invariant_start():
+    """A start block spanning an invariant subspace that misses the top eigenvector converges
+    at once to the wrong eigenvalues; the certificate rejects them and the tridiagonal solver
+    answers (method 0)."""
+    import torch
+    n, r = 16, 4
+    G = np.diag(np.arange(n, 0, -1).astype(np.float64) ** 3)  # eigenvalues 16^3 > 15^3 > ...
+    U0 = np.zeros((n, r), np.float32)
+    for c in range(r):
+        U0[c + 1, c] = 1.0  # e_2 .. e_5: misses e_1
+    U = empty(n * r)
+    w = torch.empty(r, dtype=torch.float64, device="cuda")
+    m = tdx.sym_eig_topr_ex(dev(np.ascontiguousarray(G.ravel(order="F"))), n, r, U, w,
+                            U0=dev(U0.ravel(order="F")))
+    assert m == 0
+    wl, Vl = _signed_top(G, r)
+    assert np.max(np.abs(w.cpu().numpy() - wl)) <= 1e-13 * wl[-1]
+    assert np.max(np.abs(host(U).reshape((n, r), order="F") - Vl)) < 1e-6
 
 
 GRAM_SHAPES = [
@@ -179,11 +258,17 @@ def test_hosvd_matches_oracle(dims, ranks):
     assert relnorm(host(core), co) < 1e-4
 
 
-def test_hooi_relerr_matches_oracle():
-    dims, ranks = (24, 20, 16, 12), (5, 4, 4, 3)
+HOOI_SHAPES = [((40, 30), (4, 4)), ((20, 16, 12), (4, 3, 5)), ((24, 20, 16, 12), (5, 4, 4, 3))]
+
+
+@pytest.mark.parametrize("dims,ranks", HOOI_SHAPES, ids=["x".join(map(str, d)) for d, _ in HOOI_SHAPES])
+def test_hooi_relerr_matches_oracle(dims, ranks):
+    """HOOI (dimension-tree TTM chains, warm-started eigensolves) against the oracle's
+    per-mode full chains and full eigensolves, orders 2-4: error history, core norm, factors;
+    the run without history gives the same final core bit for bit."""
     X = tucker_tensor(dims, ranks, 5e-3, seed=1)
     U0, _ = O.tucker_hosvd(X, dims, ranks)
-    _, co, hist_o = O.tucker_hooi(X, dims, ranks, 3, U0)
+    Uo, co, hist_o = O.tucker_hooi(X, dims, ranks, 3, U0)
     Ud = [empty(d * r) for d, r in zip(dims, ranks)]
     core = empty(int(np.prod(ranks)))
     hist = tdx.tucker_hooi(dev(X), dims, ranks, 3, core, Ud)
@@ -191,6 +276,12 @@ def test_hooi_relerr_matches_oracle():
     # ||core||^2/||X||^2 moves relerr (5e-3 here) by ~1e-5
     assert np.max(np.abs(hist - hist_o)) < 2e-5
     assert abs(np.linalg.norm(host(core)) - np.linalg.norm(co)) / np.linalg.norm(co) < 1e-5
+    for u, uo, d, r in zip(Ud, Uo, dims, ranks):
+        assert relnorm(host(u).reshape((d, r), order="F"), uo) < 1e-4
+    U2 = [empty(d * r) for d, r in zip(dims, ranks)]
+    core2 = empty(int(np.prod(ranks)))
+    assert tdx.tucker_hooi(dev(X), dims, ranks, 3, core2, U2, history=False) is None
+    assert np.array_equal(host(core2), host(core))
 
 
 def test_ttm_golden_and_random():
