@@ -43,6 +43,25 @@ void release_workspace();
 
 __host__ __device__ inline long ceil_div(long a, long b) { return (a + b - 1) / b; }
 
+// Streaming (nontemporal) loads for data read exactly once per kernel, such as the tensor in
+// MTTKRP: on MI355X a read stream of 16-byte nontemporal loads runs at ~7.1 TB/s against
+// ~6.3 TB/s with the default cache policy.
+__device__ __forceinline__ uint4 ld_stream(const uint4 *p) {
+    typedef unsigned v4 __attribute__((ext_vector_type(4)));
+    const v4 t = __builtin_nontemporal_load(reinterpret_cast<const v4 *>(p));
+    return uint4{t.x, t.y, t.z, t.w};
+}
+__device__ __forceinline__ float4 ld_stream(const float4 *p) {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    const v4 t = __builtin_nontemporal_load(reinterpret_cast<const v4 *>(p));
+    return float4{t.x, t.y, t.z, t.w};
+}
+__device__ __forceinline__ uint2 ld_stream(const uint2 *p) {
+    typedef unsigned v2 __attribute__((ext_vector_type(2)));
+    const v2 t = __builtin_nontemporal_load(reinterpret_cast<const v2 *>(p));
+    return uint2{t.x, t.y};
+}
+
 // ---- device helpers ------------------------------------------------------------
 __device__ __forceinline__ uint32_t f2u(float x) { return __float_as_uint(x); }
 __device__ __forceinline__ float u2f(uint32_t u) { return __uint_as_float(u); }

@@ -112,14 +112,14 @@ struct StepLoads {
             for (int q = 0; q < NL; ++q) {
                 const long j = fj + q;
                 const bool in = rvalid[0] && (j < If);
-                v[q] = *reinterpret_cast<const VT *>(X + slow_off + (in ? j : 0) * s1 + roff[0]);
+                v[q] = ld_stream(reinterpret_cast<const VT *>(X + slow_off + (in ? j : 0) * s1 + roff[0]));
                 ok |= (uint32_t)in << q;
             }
         } else {
             const bool fok = fj < If;
 #pragma unroll
             for (int t = 0; t < NL; ++t) {
-                v[t] = *reinterpret_cast<const VT *>(X + slow_off + (fok ? fj : 0) + roff[t]);
+                v[t] = ld_stream(reinterpret_cast<const VT *>(X + slow_off + (fok ? fj : 0) + roff[t]));
                 ok |= (uint32_t)(rvalid[t] && fok) << t;
             }
         }
@@ -484,26 +484,41 @@ int mttkrp_plan(MttkrpPlan &p, int order, const long *dims, int mode, int dtype,
     const size_t align = (st == 2) ? 8 : 16;
     p.fast = rank <= 64 && (dims[0] % vec == 0) && (((uintptr_t)X) % align == 0);
     if (target_wgs <= 0) target_wgs = 512;
-    // output modes > 0: the LDS-staged kernel when mode 0 holds whole 16-byte vectors
+    // the LDS-staged kernel: for output modes > 0 when mode 0 holds whole 16-byte vectors,
+    // for mode 0 when it holds whole row groups
     const long lvec = (st == 0) ? 4 : (st == 1) ? 8 : 16;
-    if (lds_enabled() && p.fast && mttkrp_lds_covers(order, mode, rank) && dims[0] % lvec == 0 &&
-        ((uintptr_t)X) % 16 == 0) {
+    if (lds_enabled() && p.fast && mttkrp_lds_covers(order, mode, rank, In, dtype) &&
+        (mode == 0 || dims[0] % lvec == 0) && ((uintptr_t)X) % 16 == 0) {
         p.lds = true;
         p.rows_per_wg = 128;
         p.rowtiles = (int)ceil_div(In, 128);
         p.rows_pad = (long)p.rowtiles * 128;
         long S = 1;
         for (int m = 1; m < order; ++m)
-            if (m != mode) S *= dims[m];
+            if (m != mode && m != fmode) S *= dims[m];
         p.ksteps = S;
         // grid (measured on the 1024^3 rank-64 tensor): 512 workgroups (two per CU) for FP32
-        // and BF16; FP8 tiles carry half the bytes per step and prefer 2048
+        // and BF16; FP8 tiles carry half the bytes per step and prefer 2048.  A split is one
+        // i_0 chunk (or a group of chunks when there are more chunks than splits) times a
+        // range of slow indices, so a workgroup loads its factor-panel chunk once and runs
+        // one pipelined loop over its slow range.
         const long lds_wgs = (st == 2) ? 2048 : 512;
+        const long kb = (st == 0) ? 64 : 128;
+        const long nfc = ceil_div(dims[fmode], kb);
         long ns = ceil_div(lds_wgs, (long)p.rowtiles);
-        if (ns > S) ns = S;
         if (ns < 1) ns = 1;
-        p.ksteps_per_split = ceil_div(S, ns);
-        p.nsplit = (int)ceil_div(S, p.ksteps_per_split);
+        long cps = 1, nss = 1;
+        if (ns >= nfc) {
+            nss = ns / nfc;
+            if (nss > S) nss = S;
+        } else {
+            cps = ceil_div(nfc, ns);
+        }
+        p.ksteps_per_split = ceil_div(S, nss);
+        nss = ceil_div(S, p.ksteps_per_split);
+        p.lds_cps = (int)cps;
+        p.lds_nss = (int)nss;
+        p.nsplit = (int)(ceil_div(nfc, cps) * nss);
         return TDX_OK;
     }
     if (p.fast) {
@@ -595,7 +610,7 @@ int mttkrp_launch(const MttkrpPlan &p, const void *X, const float *const *panels
         if (m != p.mode && m != a.fmode) a.slow[a.nslow++] = m;
     const int stt = storage_type(p.dtype);
     if (p.dtype == TDX_FP8_E4M3 && p.mode == 0 && !row_scale) return record(TDX_ERR_ARG);
-    if (p.lds) return mttkrp_lds_launch(p, X, panels, partial, s, skip);
+    if (p.lds) return mttkrp_lds_launch(p, X, panels, row_scale, partial, s, skip);
     if (p.fast) {
         const int KS = (stt == 0) ? 16 : 32;
         a.FB = ceil_div(a.If, KS);

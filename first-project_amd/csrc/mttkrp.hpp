@@ -27,6 +27,8 @@ struct MttkrpPlan {
     int nsplit = 0;      // K splits (workgroups per row tile)
     long ksteps = 0;
     long ksteps_per_split = 0;
+    int lds_cps = 1;     // LDS kernel: i_0 chunks per split
+    int lds_nss = 1;     // LDS kernel: slow-index ranges per chunk group
     size_t partial_floats() const { return (size_t)nsplit * rows_pad * Rp; }
 };
 
@@ -40,10 +42,10 @@ int mttkrp_plan(MttkrpPlan &p, int order, const long *dims, int mode, int dtype,
 int mttkrp_launch(const MttkrpPlan &p, const void *X, const float *const *panels,
                   const float *row_scale, float *partial, hipStream_t s, const int *skip = nullptr);
 
-// the LDS-staged kernel for output modes > 0 (mttkrp_lds.hip)
-bool mttkrp_lds_covers(int order, int mode, long rank);
-int mttkrp_lds_launch(const MttkrpPlan &p, const void *X, const float *const *panels, float *partial,
-                      hipStream_t s, const int *skip);
+// the LDS-staged kernel (mttkrp_lds.hip)
+bool mttkrp_lds_covers(int order, int mode, long rank, long In, int dtype);
+int mttkrp_lds_launch(const MttkrpPlan &p, const void *X, const float *const *panels, const float *row_scale,
+                      float *partial, hipStream_t s, const int *skip);
 
 // panel packing (optionally scaling row i by scale[i])
 int pack_panel(const float *U, long I, long R, int Rp, const float *scale, float *panel, hipStream_t s);
