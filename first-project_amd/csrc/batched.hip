@@ -242,9 +242,7 @@ static int launch_variant(int v, int grid, hipStream_t s, const float *X, float 
 }
 
 static int grid_for_variant(int v) {
-    int dev = 0, cus = 256;
-    (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const int cus = device_cus();
     int per_cu = 2;
     (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)kVariants[v].fn, kVariants[v].bs, 0);
     if (per_cu < 1) per_cu = 1;

@@ -11,6 +11,8 @@ namespace tdx {
 int record(int status);
 int record_hip(hipError_t e, const char *what, const char *file, int line);
 hipStream_t stream();
+// compute units of the current device (cached per device)
+int device_cus();
 
 #define TDX_HIP(call)                                                              \
     do {                                                                           \

@@ -497,12 +497,12 @@ int mttkrp_plan(MttkrpPlan &p, int order, const long *dims, int mode, int dtype,
         for (int m = 1; m < order; ++m)
             if (m != mode && m != fmode) S *= dims[m];
         p.ksteps = S;
-        // grid (measured on the 1024^3 rank-64 tensor): 512 workgroups (two per CU) for FP32
-        // and BF16; FP8 tiles carry half the bytes per step and prefer 2048.  A split is one
-        // i_0 chunk (or a group of chunks when there are more chunks than splits) times a
-        // range of slow indices, so a workgroup loads its factor-panel chunk once and runs
-        // one pipelined loop over its slow range.
-        const long lds_wgs = (st == 2) ? 2048 : 512;
+        // grid (measured on the 1024^3 rank-64 tensor, 256-4096 workgroups): one workgroup
+        // per CU for BF16, two for FP32, four for FP8 (its tiles carry half the bytes per
+        // step).  A split is one fast-index chunk (or a group of chunks when there are more
+        // chunks than splits) times a range of slow indices, so a workgroup loads its
+        // factor-panel chunk once and runs one pipelined loop over its slow range.
+        const long lds_wgs = (long)device_cus() * ((st == 1) ? 1 : (st == 0) ? 2 : 4);
         const long kb = (st == 0) ? 64 : 128;
         const long nfc = ceil_div(dims[fmode], kb);
         long ns = ceil_div(lds_wgs, (long)p.rowtiles);

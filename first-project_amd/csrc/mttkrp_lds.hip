@@ -48,7 +48,7 @@ struct LArgs {
     long dims[4];
     long In, If, S, sper, rows_pad;  // If: fast extent; S slow indices in total, sper per split
     int order, mode, nslow, Rp, rowtiles;
-    int cps, nss;                     // chunks per split, slow ranges per chunk group
+    int cps, nss, ncg;                // chunks per split, slow ranges per chunk group, chunk groups
     int slow[2];
 };
 
@@ -116,11 +116,26 @@ __global__ __launch_bounds__(256, 2) void k_mttkrp_lds(LArgs a) {
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lm = lane & 15, lg = lane >> 4;
-    const int rowtile = blockIdx.x % a.rowtiles;
-    const long split = blockIdx.x / a.rowtiles;
+    // Workgroups that run side by side should read whole rows of the tensor between them
+    // (DRAM page locality): for TR adjacent workgroups take adjacent row tiles (neighbouring
+    // 256-byte pieces of the same i_1 rows), otherwise adjacent chunk groups (the pieces of
+    // the same output rows).
+    int rowtile;
+    long cgroup, srange;
+    if constexpr (TR) {
+        rowtile = blockIdx.x % a.rowtiles;
+        const long split = blockIdx.x / a.rowtiles;
+        cgroup = split / a.nss;
+        srange = split % a.nss;
+    } else {
+        cgroup = blockIdx.x % a.ncg;
+        const long rest = blockIdx.x / a.ncg;
+        rowtile = (int)(rest % a.rowtiles);
+        srange = rest / a.rowtiles;
+    }
+    const long split = cgroup * a.nss + srange;
     const long row0 = (long)rowtile * 128;
     const long nfc = ceil_div(a.If, (long)KB);
-    const long cgroup = split / a.nss, srange = split % a.nss;
     const long fc0 = cgroup * a.cps, fc1 = min(fc0 + a.cps, nfc);
     const long sl0 = srange * a.sper;
     const long sl1 = min(sl0 + a.sper, a.S);
@@ -414,6 +429,7 @@ int mttkrp_lds_launch(const MttkrpPlan &p, const void *X, const float *const *pa
     a.sper = p.ksteps_per_split;
     a.cps = p.lds_cps;
     a.nss = p.lds_nss;
+    a.ncg = p.nsplit / p.lds_nss;
     a.nslow = 0;
     for (int m = 1; m < p.order; ++m)
         if (m != p.mode && m != fm) a.slow[a.nslow++] = m;

@@ -59,6 +59,23 @@ void release_workspace() {
     }
 }
 
+int device_cus() {
+    static int cache[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) dev = 0;
+    if (dev >= 64) {
+        int c = 0;
+        return (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && c > 0) ? c
+                                                                                                              : 256;
+    }
+    if (cache[dev] <= 0) {
+        int c = 0;
+        if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0) c = 256;
+        cache[dev] = c;
+    }
+    return cache[dev];
+}
+
 }  // namespace tdx
 
 extern "C" {
