@@ -35,8 +35,10 @@ __global__ __launch_bounds__(256) void k_vinv(const double *__restrict__ G0, con
                                               const double *__restrict__ G2, const double *__restrict__ G3,
                                               int order, int mode, int R, double *__restrict__ Vinv,
                                               int *__restrict__ status) {
+    // 64 KB of LDS, so that the workgroup fits beside an MTTKRP workgroup on one CU: the
+    // fallback's working copy of V is V itself (only its diagonal is read during the sweep,
+    // and the fallback needs V no more once it starts)
     __shared__ double V[kMaxR * kMaxR];
-    __shared__ double L[kMaxR * kMaxR];
     __shared__ double W[kMaxR * kMaxR];
     __shared__ double sh[256];
     __shared__ double cs[4 * (kMaxR / 2 + 1)];
@@ -49,7 +51,7 @@ __global__ __launch_bounds__(256) void k_vinv(const double *__restrict__ G0, con
         V[e] = v;
     }
     __syncthreads();
-    const int fb = block_spd_inverse<kMaxR * kMaxR / 256>(V, L, W, Vinv, R, cs, sh, &fail);
+    const int fb = block_spd_inverse<kMaxR * kMaxR / 256>(V, V, W, Vinv, R, cs, sh, &fail);
     if (fb && threadIdx.x == 0) atomicOr(status, 1);
 }
 
@@ -689,16 +691,26 @@ static int enqueue_sweep(CpPlan &P, const Storage &S, hipStream_t s, bool fit) {
         TDX_LAUNCH_CHECK();
     }
     for (int n = 0; n < P.order; ++n) {
-        // V^{-1} ahead of the MTTKRP on the same stream: a concurrent launch would hold one CU
-        // (and its LDS) while the MTTKRP grid is sized for every CU
-        k_vinv<<<1, 256, 0, s>>>(P.G[0], P.G[1], P.order > 2 ? P.G[2] : P.G[0], P.order > 3 ? P.G[3] : P.G[0],
-                                 P.order, n, R, P.Vinv[n], P.status);
+        // V^{-1} (one workgroup) and the MTTKRP depend only on the previous mode's update.
+        // When the MTTKRP grid leaves a workgroup slot free on every CU (one workgroup per CU)
+        // the inverse runs beside it on the side stream; otherwise ahead of it on the same
+        // stream, since it would hold a CU the MTTKRP grid needs.
+        const bool beside = S.mp[n].rowtiles * (long)S.mp[n].nsplit <= device_cus();
+        if (beside) {
+            TDX_HIP(hipEventRecord(P.ev_fork[n], s));
+            TDX_HIP(hipStreamWaitEvent(P.side, P.ev_fork[n], 0));
+        }
+        k_vinv<<<1, 256, 0, beside ? P.side : s>>>(P.G[0], P.G[1], P.order > 2 ? P.G[2] : P.G[0],
+                                                  P.order > 3 ? P.G[3] : P.G[0], P.order, n, R, P.Vinv[n],
+                                                  P.status);
         TDX_LAUNCH_CHECK();
+        if (beside) TDX_HIP(hipEventRecord(P.ev_join[n], P.side));
         const float *panels[4] = {P.panel[0], P.panel[1], P.panel[2], P.panel[3]};
         const float *row_scale = (S.dtype == TDX_FP8_E4M3 && n == 0) ? S.scales : nullptr;
         const int *skip = (P.ls_mode && n == 0) ? &P.ls->skip : nullptr;
         int st = mttkrp_launch(S.mp[n], S.X, panels, row_scale, P.partial, s, skip);
         if (st) return st;
+        if (beside) TDX_HIP(hipStreamWaitEvent(s, P.ev_join[n], 0));
         const long In = P.dims[n];
         const int nblk = (int)ceil_div(In, kSolveRows);
         const float *pscale = (S.dtype == TDX_FP8_E4M3 && n == 0) ? S.scales : nullptr;

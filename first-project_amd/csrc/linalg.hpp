@@ -94,7 +94,8 @@ __device__ inline void jacobi_eig_block(double *A, double *Q, int n, double *cs,
 }
 
 // Vinv = V^{-1} for a symmetric R x R V (all arrays column-major, R*R doubles, any memory
-// space visible to the whole workgroup; V is preserved).  If V is not numerically SPD (a
+// space visible to the whole workgroup; V is preserved unless L is V, in which case the
+// fallback below works in V).  If V is not numerically SPD (a
 // relative pivot of the sweep operator below 1e-13), a pseudo-inverse from the Jacobi
 // eigendecomposition (eigenvalues below 1e-12 * max dropped) is returned instead and the
 // function returns 1 (in every thread).  cs needs 4*(R/2+1) doubles, sh blockDim doubles.
@@ -110,28 +111,35 @@ __device__ inline int block_spd_inverse(const double *V, double *L, double *W, d
     // as numerically SPD when every pivot exceeds 1e-13 of its original diagonal entry (a
     // smaller one means column k is dependent on the earlier ones to rounding).
     // Each thread keeps its elements e = tid + nt u of A in registers; only the pivot column
-    // goes through LDS (W), one barrier pair per pivot.
+    // goes through LDS, alternating between two halves of W so that one barrier per pivot
+    // suffices (a thread writes pivot k+1's column only after every thread has passed pivot
+    // k's barrier, i.e. finished reading pivot k-1's).  W needs 2R doubles.
+    // Every one of the MAXEPT register slots is processed (no runtime bound on the loops:
+    // guards on a runtime count became a branch per slot); slots past R^2 hold zeros at
+    // index (0, 0) and only the pivot-column stores and the final write look at ok.
     constexpr int kMaxEpt = MAXEPT;
-    const int ept = (RR + nt - 1) / nt;
     double a[kMaxEpt];
     int ii[kMaxEpt], jj[kMaxEpt];
     bool ok[kMaxEpt];
 #pragma unroll
     for (int u = 0; u < kMaxEpt; ++u) {
         const int e = tid + nt * u;
-        ok[u] = u < ept && e < RR;
+        ok[u] = e < RR;
         ii[u] = ok[u] ? e % R : 0;
         jj[u] = ok[u] ? e / R : 0;
         a[u] = ok[u] ? V[e] : 0.0;
     }
     if (tid == 0) *fail = 0;
     __syncthreads();
+    // when R divides the block size a thread's elements share one row index
+    const bool one_row = (nt % R) == 0;
     for (int k = 0; k < R; ++k) {
+        double *Wk = W + (k & 1) * R;
 #pragma unroll
         for (int u = 0; u < kMaxEpt; ++u)
-            if (u < ept && ok[u] && jj[u] == k) W[ii[u]] = a[u];
+            if (ok[u] && jj[u] == k) Wk[ii[u]] = a[u];
         __syncthreads();
-        const double d = W[k];
+        const double d = Wk[k];
         if (!(d > 1e-13 * V[k + k * R]) || !isfinite(d)) {
             if (tid == 0) *fail = 1;
             break;  // uniform: every thread read the same pivot
@@ -139,23 +147,29 @@ __device__ inline int block_spd_inverse(const double *V, double *L, double *W, d
         const double dinv = 1.0 / d;
         // all reads of the pivot column first, then a branch-free update
         double wi[kMaxEpt], wj[kMaxEpt];
+        if (one_row) {
+            const double w0 = Wk[ii[0]];
 #pragma unroll
-        for (int u = 0; u < kMaxEpt; ++u)
-            if (u < ept) {
-                wi[u] = W[ii[u]];
-                wj[u] = W[jj[u]];
+            for (int u = 0; u < kMaxEpt; ++u) {
+                wi[u] = w0;
+                wj[u] = Wk[jj[u]];
             }
+        } else {
 #pragma unroll
-        for (int u = 0; u < kMaxEpt; ++u)
-            if (u < ept) {
-                const bool ki = ii[u] == k, kj = jj[u] == k;
-                double v = fma(-wi[u] * dinv, wj[u], a[u]);
-                v = kj ? wi[u] * dinv : v;
-                v = ki ? wj[u] * dinv : v;
-                v = (ki && kj) ? -dinv : v;
-                a[u] = v;
+            for (int u = 0; u < kMaxEpt; ++u) {
+                wi[u] = Wk[ii[u]];
+                wj[u] = Wk[jj[u]];
             }
-        __syncthreads();  // W is rewritten by the next pivot
+        }
+#pragma unroll
+        for (int u = 0; u < kMaxEpt; ++u) {
+            const bool ki = ii[u] == k, kj = jj[u] == k;
+            double v = fma(-wi[u] * dinv, wj[u], a[u]);
+            v = kj ? wi[u] * dinv : v;
+            v = ki ? wj[u] * dinv : v;
+            v = (ki && kj) ? -dinv : v;
+            a[u] = v;
+        }
     }
     __syncthreads();
     if (!*fail) {
@@ -165,7 +179,8 @@ __device__ inline int block_spd_inverse(const double *V, double *L, double *W, d
         __syncthreads();
         return 0;
     }
-    for (int e = tid; e < RR; e += nt) L[e] = V[e];
+    if (L != V)
+        for (int e = tid; e < RR; e += nt) L[e] = V[e];
     __syncthreads();
     jacobi_eig_block(L, W, R, cs, sh);
     double wmax = 0.0;
