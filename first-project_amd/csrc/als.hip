@@ -11,9 +11,9 @@
 // FP32 LU (getrf/getrs) with a cudaMalloc per call, and a transpose (Sgeam).
 //
 // Kernels per mode: k_vinv (one workgroup, symmetric sweep-operator inverse, Jacobi
-// pseudo-inverse fallback), the fused MTTKRP, then
-// k_split_sum (FP64 sum of the split-K partials), k_solve (one workgroup per 16 rows),
-// k_gram_psum (Gram of the new factor) and, once per sweep, k_fit.
+// pseudo-inverse fallback), the fused MTTKRP, then k_solve (one workgroup per 16 rows: FP64
+// sum of the split-K partials, U = M V^{-1}, panel, fit and Gram partials), k_gram_psum
+// (Gram of the new factor) and, once per sweep, k_fit.
 // A sweep is captured once into a hipGraph and replayed.
 #include "mttkrp.hpp"
 #include "linalg.hpp"
@@ -74,77 +74,107 @@ __global__ __launch_bounds__(256) void k_split_sum(const float *__restrict__ par
     msum[e] = s;
 }
 
-// U_n rows = M V^{-1} for kSolveRows rows per workgroup (256 threads, 4 (row, column)
-// elements each).  Also writes the panel layout (optionally row-scaled for FP8 storage), the
-// per-block inner product <M, U> (fit) and the per-block Gram partial U_blk^T U_blk.
-__global__ __launch_bounds__(256) void k_solve(const double *__restrict__ msum, int Rp, long In, int R,
-                                               const double *__restrict__ Vinv, float *__restrict__ U,
-                                               float *__restrict__ panel, const float *__restrict__ panel_scale,
-                                               double *__restrict__ ip_part, double *__restrict__ gram_part,
-                                               int *__restrict__ skip_reset) {
-    constexpr int EPT = kSolveRows * kMaxR / 256;  // elements per thread
+// U_n rows = M V^{-1} for kSolveRows rows per workgroup (256 threads), M summed here from
+// the MTTKRP's split partials in FP64 in split order (the order of k_split_sum).  Also
+// writes the panel layout (optionally row-scaled for FP8 storage), the per-block inner
+// product <M, U> (fit) and the per-block Gram partial U_blk^T U_blk.  Register blocks:
+// 1 x 4 outputs of U per thread (V^{-1} is symmetric, so its rows are read contiguously),
+// 4 x 4 of the Gram partial.
+__global__ __launch_bounds__(256) void k_solve(const float *__restrict__ partial, int nsplit, long rows_pad, int Rp,
+                                               long In, int R, const double *__restrict__ Vinv,
+                                               float *__restrict__ U, float *__restrict__ panel,
+                                               const float *__restrict__ panel_scale, double *__restrict__ ip_part,
+                                               double *__restrict__ gram_part, int *__restrict__ skip_reset) {
     __shared__ double Vs[kMaxR * kMaxR];
-    __shared__ double Ms[kSolveRows * kMaxR];
+    __shared__ double Ms[kSolveRows * kMaxR];  // M rows, then the U rows (as rounded FP32 values)
     __shared__ double sh[256];
     const int tid = threadIdx.x;
     const long row0 = (long)blockIdx.x * kSolveRows;
     const int RR = R * R;
     for (int e = tid; e < RR; e += 256) Vs[e] = Vinv[e];
-    const int nel = kSolveRows * Rp;
-    for (int e = tid; e < nel; e += 256) Ms[(e / Rp) * kMaxR + (e % Rp)] = msum[row0 * Rp + e];
+    const long pstride = rows_pad * Rp;
+    for (int e = tid; e < kSolveRows * Rp; e += 256) {
+        const int r = e / Rp, b = e % Rp;
+        const long i = row0 + r;
+        double acc = 0.0;
+        if (i < In) {
+            const float *pp = partial + i * Rp + b;
+            int sp = 0;
+            for (; sp + 8 <= nsplit; sp += 8) {
+                float v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) v[u] = pp[(long)(sp + u) * pstride];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) acc += (double)v[u];
+            }
+            for (; sp < nsplit; ++sp) acc += (double)pp[(long)sp * pstride];
+        }
+        Ms[r * kMaxR + b] = acc;
+    }
     __syncthreads();
-    // U = M V^{-1}, inner product with M
-    double uval[EPT];
+    // U = M V^{-1}: thread (row r, columns b0 .. b0 + 3)
+    const int r = tid >> 4, b0 = 4 * (tid & 15);
+    const long i = row0 + r;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    if (b0 < R) {
+        for (int a = 0; a < R; ++a) {
+            const double m = Ms[r * kMaxR + a];
+            const double *vr = Vs + a * R + b0;  // (V^{-1})[a][b0..b0+3] (past R: unused)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[q] = fma(m, vr[q], acc[q]);
+        }
+    }
+    float uf[4];
     double ip = 0.0;
 #pragma unroll
-    for (int u = 0; u < EPT; ++u) {
-        const int e = tid + 256 * u;
-        uval[u] = 0.0;
-        if (e < nel) {
-            const int r = e / Rp, b = e % Rp;
-            const long i = row0 + r;
-            float uf = 0.0f;
-            if (b < R && i < In) {
-                double sacc[4] = {0.0, 0.0, 0.0, 0.0};
-                int a = 0;
-                for (; a + 4 <= R; a += 4) {
+    for (int q = 0; q < 4; ++q) {
+        const bool in = i < In && b0 + q < R;
+        uf[q] = in ? (float)acc[q] : 0.0f;
+        if (in) ip += Ms[r * kMaxR + b0 + q] * (double)uf[q];
+    }
+    __syncthreads();  // every thread is done reading M
+    if (b0 < Rp) {
+        const float sc = (panel_scale && i < In) ? panel_scale[i] : 1.0f;
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) sacc[q] += Ms[r * kMaxR + a + q] * Vs[a + q + b * R];
-                }
-                for (; a < R; ++a) sacc[0] += Ms[r * kMaxR + a] * Vs[a + b * R];
-                uf = (float)((sacc[0] + sacc[1]) + (sacc[2] + sacc[3]));
-                ip += Ms[r * kMaxR + b] * (double)uf;
-            }
-            uval[u] = (double)uf;
+        for (int q = 0; q < 4; ++q) {
+            const int b = b0 + q;
+            if (b >= Rp) break;
+            Ms[r * kMaxR + b] = (double)uf[q];
+            if (i < In && b < R) U[i + (long)b * In] = uf[q];
+            panel[((i >> 3) * Rp + b) * 8 + (i & 7)] = uf[q] * sc;
         }
     }
-    __syncthreads();
-    // write outputs; stage U (as rounded FP32 values) for the Gram partial
-#pragma unroll
-    for (int u = 0; u < EPT; ++u) {
-        const int e = tid + 256 * u;
-        if (e < nel) {
-            const int r = e / Rp, b = e % Rp;
-            const long i = row0 + r;
-            Ms[r * kMaxR + b] = uval[u];
-            if (b < R && i < In) U[i + (long)b * In] = (float)uval[u];
-            float pv = (float)uval[u];
-            if (panel_scale && i < In) pv *= panel_scale[i];
-            panel[((i >> 3) * Rp + b) * 8 + (i & 7)] = pv;
-        }
-    }
-    ip = block_sum(ip, sh);
+    ip = block_sum(ip, sh);  // (its barriers also publish the U rows)
     if (tid == 0) ip_part[blockIdx.x] = ip;
     // the mode-0 MTTKRP that may have been skipped has already run (stream order)
     if (skip_reset && blockIdx.x == 0 && tid == 0) *skip_reset = 0;
-    // Ms now holds the U block (sync inside block_sum)
-    double *gp = gram_part + (long)blockIdx.x * RR;
-    for (int e = tid; e < RR; e += 256) {
-        const int a = e % R, b = e / R;
-        double s = 0.0;
+    // Gram partial: thread (rows a0 .. a0 + 3, columns c0 .. c0 + 3)
+    const int a0 = 4 * (tid >> 4), c0 = 4 * (tid & 15);
+    if (a0 < R && c0 < R) {
+        double g[4][4];
 #pragma unroll
-        for (int r = 0; r < kSolveRows; ++r) s += Ms[r * kMaxR + a] * Ms[r * kMaxR + b];
-        gp[e] = s;
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) g[p][q] = 0.0;
+#pragma unroll 4
+        for (int rr = 0; rr < kSolveRows; ++rr) {
+            double ua[4], ub[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                ua[q] = Ms[rr * kMaxR + a0 + q];
+                ub[q] = Ms[rr * kMaxR + c0 + q];
+            }
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) g[p][q] = fma(ua[p], ub[q], g[p][q]);
+        }
+        double *gp = gram_part + (long)blockIdx.x * RR;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+                if (a0 + p < R && c0 + q < R) gp[(a0 + p) + (c0 + q) * R] = g[p][q];
     }
 }
 
@@ -714,10 +744,8 @@ static int enqueue_sweep(CpPlan &P, const Storage &S, hipStream_t s, bool fit) {
         const long In = P.dims[n];
         const int nblk = (int)ceil_div(In, kSolveRows);
         const float *pscale = (S.dtype == TDX_FP8_E4M3 && n == 0) ? S.scales : nullptr;
-        const long mel = (long)S.mp[n].rows_pad * P.Rp;
-        k_split_sum<<<(int)ceil_div(mel, 256L), 256, 0, s>>>(P.partial, S.mp[n].nsplit, mel, P.msum);
-        k_solve<<<nblk, 256, 0, s>>>(P.msum, P.Rp, In, R, P.Vinv[n], P.U[n], P.panel[n], pscale, P.ip_part,
-                                     P.gram_part, skip ? &P.ls->skip : nullptr);
+        k_solve<<<nblk, 256, 0, s>>>(P.partial, S.mp[n].nsplit, S.mp[n].rows_pad, P.Rp, In, R, P.Vinv[n], P.U[n],
+                                     P.panel[n], pscale, P.ip_part, P.gram_part, skip ? &P.ls->skip : nullptr);
         k_gram_psum<<<(int)ceil_div((long)R * R, 256L), 256, 0, s>>>(P.gram_part, nblk, R * R, P.G[n]);
         TDX_LAUNCH_CHECK();
         if (fit && n == P.order - 1) {
