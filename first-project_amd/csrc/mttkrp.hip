@@ -55,16 +55,14 @@ __device__ __forceinline__ uint32_t pack_bf16(float a, float b) {
 }
 
 // 8 FP8 E4M3 codes (little-endian in two dwords) -> 8 BF16 packed in 4 dwords; exact
+// (every E4M3 value is a BF16 value; v_cvt_scalef32_pk_bf16_fp8 with scale 1 converts two
+// codes per instruction, bit-identical to widening through FP32 for all 256 codes)
 __device__ __forceinline__ uint4 fp8x8_to_bf16x8(uint2 q) {
-    f32x2 a = __builtin_amdgcn_cvt_pk_f32_fp8((int)q.x, false);
-    f32x2 b = __builtin_amdgcn_cvt_pk_f32_fp8((int)q.x, true);
-    f32x2 c = __builtin_amdgcn_cvt_pk_f32_fp8((int)q.y, false);
-    f32x2 d = __builtin_amdgcn_cvt_pk_f32_fp8((int)q.y, true);
     uint4 o;
-    o.x = pack_bf16(a.x, a.y);
-    o.y = pack_bf16(b.x, b.y);
-    o.z = pack_bf16(c.x, c.y);
-    o.w = pack_bf16(d.x, d.y);
+    o.x = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_scalef32_pk_bf16_fp8(q.x, 1.0f, false));
+    o.y = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_scalef32_pk_bf16_fp8(q.x, 1.0f, true));
+    o.z = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_scalef32_pk_bf16_fp8(q.y, 1.0f, false));
+    o.w = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_scalef32_pk_bf16_fp8(q.y, 1.0f, true));
     return o;
 }
 

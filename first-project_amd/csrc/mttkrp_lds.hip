@@ -61,12 +61,11 @@ __device__ __forceinline__ uint32_t lpack_bf16(float a, float b) {
     return __builtin_bit_cast(uint32_t, v);
 }
 
-// 4 FP8 E4M3 codes -> 4 BF16 (exact)
+// 4 FP8 E4M3 codes -> 4 BF16 (exact: v_cvt_scalef32_pk_bf16_fp8 with scale 1, two codes
+// per instruction, bit-identical to widening through FP32 for all 256 codes)
 __device__ __forceinline__ uint2 lfp8x4_to_bf16(uint32_t q) {
-    const lf32x2 a = __builtin_amdgcn_cvt_pk_f32_fp8((int)q, false);
-    const lf32x2 b = __builtin_amdgcn_cvt_pk_f32_fp8((int)q, true);
-    return uint2{(__float_as_uint(a.x) >> 16) | (__float_as_uint(a.y) & 0xffff0000u),
-                 (__float_as_uint(b.x) >> 16) | (__float_as_uint(b.y) & 0xffff0000u)};
+    return uint2{__builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_scalef32_pk_bf16_fp8(q, 1.0f, false)),
+                 __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_scalef32_pk_bf16_fp8(q, 1.0f, true))};
 }
 
 template <int ST> struct LTile;
