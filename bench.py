@@ -3,17 +3,18 @@
 
 Workload (dense 1024 x 1024 x 1024 tensor, rank 64): X = [[A,B,C]] + 0.1% Gaussian noise,
 factor entries N(0,1) with columns normalised to unit 2-norm (Philox seed 0), noise from
-seed 2, initial factors N(0,1) from seed 1.  The tensor is generated on the device and is
-resident in HBM before any timed region.
+seed 2, initial factors N(0,1) from each of the --seeds.  The tensor is generated on the
+device and is resident in HBM before any timed region.
 
  * plain FP32 path: FP32 tensor, FP32-accumulating MFMA MTTKRP, FP64 solve, 27 sweeps.
    Its final direct (compensated FP64) relative error is the accuracy reference.
  * mixed path (the value): BF16 tensor on MFMA with FP32 accumulation for --steps sweeps,
-   then 2 FP32 refinement sweeps.  value = device seconds until its relative error is
-   within 1e-4 of the reference.
+   then 2 FP32 refinement sweeps.  value = median over the init seeds of the device seconds
+   until its relative error is within 1e-4 of the reference.
  * MTTKRP per mode for FP32 / BF16 / FP8-E4M3 storage (hipEvents on the library stream).
  * cpu_baseline: the oracle's FP64 ALS (OpenMP) on the same tensor, one sweep timed on the
-   host cores and extrapolated to the plain path's sweep count to target.
+   host cores and extrapolated to the plain path's sweep count to target; plus the 128^3
+   rank-16 configuration in full on the oracle beside the GPU (times and per-sweep errors).
 
 Prints ONE JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
 """
@@ -286,9 +287,34 @@ def main():
         O.cp_als(Xh, dims, U0, 1)
         cpu_sweep = time.perf_counter() - t0
         sweeps_needed = int(round(s32_med))
+        model = None
+        try:
+            with open("/proc/cpuinfo") as f:
+                model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), None)
+        except OSError:
+            pass
         cpu = {"value": round(cpu_sweep * sweeps_needed, 3), "unit": "s", "cores": threads, "kind": "port",
                "sample": f"1 FP64 ALS sweep (3 MTTKRPs + Cholesky solves) of the {n}^3 rank-{R} tensor on "
-                         f"{threads} OpenMP threads = {cpu_sweep:.2f} s, x {sweeps_needed} sweeps to target"}
+                         f"{threads} OpenMP threads = {cpu_sweep:.2f} s, x {sweeps_needed} sweeps to target",
+               "cpu_model": model, "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "nproc": os.cpu_count()}
+        # C1 in full on the oracle beside the GPU (same tensor, same U(0,1) init from seed 1):
+        # 128^3 rank 16, 50 plain sweeps, per-sweep relative errors compared
+        import workloads as W
+        X1, dims1, R1 = W.c1()
+        U1 = [O.uniform(d * R1, 1, m).reshape((d, R1), order="F") for m, d in enumerate(dims1)]
+        t0 = time.perf_counter()
+        _, fit_c, _, _ = O.cp_als(X1.cpu().numpy(), dims1, [u.astype(np.float64) for u in U1], 50)
+        c1_cpu = time.perf_counter() - t0
+        o1 = tdx.default_cp_options()
+        o1.max_sweeps, o1.init, o1.record_fit = 50, 1, 1
+        Ud1 = [torch.from_numpy(np.ascontiguousarray(u.ravel(order="F"))).to(dev) for u in U1]
+        _, _, fit_g, ms_g = tdx.cp_als_ex(X1, dims1, R1, Ud1, o1, want_ms=True)
+        fit_g = np.asarray(fit_g, np.float64)
+        cpu["c1"] = {"cpu_s": round(c1_cpu, 3), "gpu_s": round(float(np.sum(ms_g)) / 1e3, 5),
+                     "relerr_final_cpu_fp64": float(fit_c[-1]), "relerr_final_gpu_fp32": float(fit_g[-1]),
+                     "max_abs_relerr_diff_over_sweeps": float(np.max(np.abs(fit_g - fit_c))),
+                     "workload": "128^3 rank 16, U(0,1) factors, 1% noise, U(0,1) init (seed 1), 50 sweeps"}
+        del X1
 
     extra = None if args.no_extra else extras(args, log)
 
