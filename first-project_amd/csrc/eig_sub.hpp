@@ -10,7 +10,10 @@
 //                          stop when rho <= 1e-12 ||H||_F; V = orth(Z) }
 //   (H + H^T)/2 = W diag(theta) W^T by cyclic Jacobi in round-robin order; eigenvectors V W.
 // orth() is Cholesky QR of the column-normalised block, done twice when the first pass finds
-// nearly dependent columns.
+// nearly dependent columns.  Every product (A V, V^T Z, V S, X^T X, X M, V W) runs on FP64
+// MFMA (v_mfma_f64_16x16x4f64: exact products, FP64 accumulation), with A held in registers
+// for the whole call; the Cholesky factorisation and triangular inverse run in one wave's
+// registers; the Jacobi rounds take two barriers each.
 //
 // The result is accepted only under a certificate that the theta are the r LARGEST
 // eigenvalues, valid for any symmetric A: with orthonormal V and residual bound e = rho +
@@ -26,15 +29,37 @@
 namespace tdx {
 
 static constexpr int kSubMaxR = 32;
+static constexpr int kSubRows = kEigMaxN;     // V and Z always hold 128 rows (zero past n)
 static constexpr int kSubLd = kSubMaxR;       // row stride of the n x r blocks V, Z
-static constexpr int kSubHld = kSubMaxR + 1;  // row stride of the r x r blocks H, W, S
+static constexpr int kSubHld = kSubMaxR + 1;  // row stride of the r x r blocks H, W, S, M
 static constexpr int kSubMaxIt = 12;
 
-// dynamic LDS doubles of the subspace path for order n: V, Z (n x kSubLd), H, W, S
-// (kSubMaxR x kSubHld), cinv, dj (kSubMaxR), part (8 x kSubMaxR), the Jacobi parameters
-// (8 x 16) and the reduction slots (64)
-__host__ __device__ constexpr long eig_sub_doubles(int n) {
-    return 2L * n * kSubLd + 3L * kSubMaxR * kSubHld + 2L * kSubMaxR + 8L * kSubMaxR + 8L * 16 + 64;
+// dynamic LDS doubles of the subspace path (any n <= 128): V, Z (128 x kSubLd), H, W, S, M
+// (kSubMaxR x kSubHld), 64 reduction slots, signs (32), positions (32 ints) and a flag
+__host__ __device__ constexpr long eig_sub_doubles(int) {
+    return 2L * kSubRows * kSubLd + 4L * kSubMaxR * kSubHld + 64 + 32 + 16 + 8;
+}
+
+typedef double sub_d4 __attribute__((ext_vector_type(4)));
+// D += A B on a 16 x 16 tile: lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16] of the
+// 4-deep slice; register v of lane l holds D[l / 16 + 4 v][l % 16]
+__device__ __forceinline__ sub_d4 sub_mfma(double a, double b, sub_d4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+// x of lane j (j wave-uniform)
+__device__ __forceinline__ double sub_rl(double x, int j) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
+    const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, j), hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), j);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// 1/sqrt(a) to full double precision: the hardware estimate refined by two Newton steps
+// (y <- y (3 - a y^2) / 2); a is finite and positive wherever it is used here
+__device__ __forceinline__ double rsq_f64(double a) {
+    double y = __builtin_amdgcn_rsq(a);
+    y = y * fma(-0.5 * a * y, y, 1.5);
+    y = y * fma(-0.5 * a * y, y, 1.5);
+    return y;
 }
 
 // sum over the workgroup; every thread gets the total (red: 4 doubles, reused after a barrier)
@@ -46,124 +71,129 @@ __device__ inline double sub_blk_sum(double v, double *red) {
     return (red[0] + red[1]) + (red[2] + red[3]);
 }
 
-// X (n x r, stride kSubLd, columns >= r zero) <- orthonormal basis of its column span by
-// Cholesky QR of the column-normalised block.  Returns the smallest Cholesky pivot (1 for
-// orthonormal input, 0 for a zero or numerically dependent column).
-__device__ inline double sub_orth_pass(double *X, int n, int r, double *S, double *cinv, double *dj,
-                                       double *part) {
-    const int tid = threadIdx.x;
-    {
-        const int c = tid & 31, p = tid >> 5;
-        double s = 0.0;
-        if (c < r)
-            for (int i = p; i < n; i += 8) s = fma(X[i * kSubLd + c], X[i * kSubLd + c], s);
-        part[p * 32 + c] = s;
+// Out (32 x 32, stride kSubHld) = X^T Y over the 128 rows of X, Y (stride kSubLd); wave w
+// computes tile (w / 2, w % 2)
+__device__ inline void sub_gram(const double *X, const double *Y, double *Out) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, lr = lane & 15, lk = lane >> 4;
+    const int tr = wv >> 1, tc = wv & 1;
+    sub_d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 8
+    for (int ks = 0; ks < kSubRows / 4; ++ks) {
+        const int k = 4 * ks + lk;
+        acc = sub_mfma(X[k * kSubLd + 16 * tr + lr], Y[k * kSubLd + 16 * tc + lr], acc);
     }
-    __syncthreads();
-    if (tid < r) {
-        double s = 0.0;
-        for (int p = 0; p < 8; ++p) s += part[p * 32 + tid];
-        cinv[tid] = (s > 0.0) ? 1.0 / sqrt(s) : 0.0;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Out[(16 * tr + lk + 4 * v) * kSubHld + 16 * tc + lr] = acc[v];
+}
+
+// acc = rows 32 w .. 32 w + 31 of X (128 x 32, stride kSubLd) times M (32 x 32, stride kSubHld)
+__device__ inline void sub_xm(const double *X, const double *M, sub_d4 (&acc)[2][2]) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) acc[rt][ct] = sub_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ks = 0; ks < kSubMaxR / 4; ++ks) {
+        const int k = 4 * ks + lk;
+        const double a0 = X[(32 * wv + lr) * kSubLd + k], a1 = X[(32 * wv + 16 + lr) * kSubLd + k];
+        const double b0 = M[k * kSubHld + lr], b1 = M[k * kSubHld + 16 + lr];
+        acc[0][0] = sub_mfma(a0, b0, acc[0][0]);
+        acc[0][1] = sub_mfma(a0, b1, acc[0][1]);
+        acc[1][0] = sub_mfma(a1, b0, acc[1][0]);
+        acc[1][1] = sub_mfma(a1, b1, acc[1][1]);
     }
-    __syncthreads();
-    for (int c = 0; c < r; ++c)
-        if (cinv[c] == 0.0) return 0.0;  // uniform
-    // S = Xn^T Xn (lower triangle used), 2 x 2 blocks per thread
-    {
-        const int a = 2 * (tid >> 4), b = 2 * (tid & 15);
-        if (a < r && b <= a) {
-            double s00 = 0.0, s01 = 0.0, s10 = 0.0, s11 = 0.0;
-            for (int i = 0; i < n; ++i) {
-                const double *xr = X + i * kSubLd;
-                const double xa0 = xr[a], xa1 = xr[a + 1], xb0 = xr[b], xb1 = xr[b + 1];
-                s00 = fma(xa0, xb0, s00);
-                s01 = fma(xa0, xb1, s01);
-                s10 = fma(xa1, xb0, s10);
-                s11 = fma(xa1, xb1, s11);
-            }
-            const double na0 = cinv[a], nb0 = cinv[b];
-            const double na1 = (a + 1 < r) ? cinv[a + 1] : 0.0, nb1 = (b + 1 < r) ? cinv[b + 1] : 0.0;
-            S[a * kSubHld + b] = s00 * na0 * nb0;
-            if (b + 1 < r) S[a * kSubHld + b + 1] = s01 * na0 * nb1;
-            if (a + 1 < r) {
-                S[(a + 1) * kSubHld + b] = s10 * na1 * nb0;
-                if (b + 1 < r) S[(a + 1) * kSubHld + b + 1] = s11 * na1 * nb1;
+}
+
+// stores the rows < n of a sub_xm / product result into Y (stride kSubLd)
+__device__ inline void sub_store_rows(const sub_d4 (&acc)[2][2], double *Y, int n) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = 32 * wv + 16 * rt + lk + 4 * v;
+            if (i < n) {
+                Y[i * kSubLd + lr] = acc[rt][0][v];
+                Y[i * kSubLd + 16 + lr] = acc[rt][1][v];
             }
         }
-    }
+}
+
+// X (rows < n, columns < r; zero elsewhere) <- orthonormal basis of its column span by
+// Cholesky QR of the column-normalised block: S = X^T X, its normalised Cholesky factor L and
+// L^{-1} in wave 0's registers (lane l holds row l), X <- X diag(1/sqrt(S_jj)) L^{-T}.
+// Returns the smallest Cholesky pivot (1 for orthonormal input; 0 when a column is zero or
+// numerically dependent: the call then leaves X unchanged).
+__device__ inline double sub_orth_pass(double *X, int n, int r, double *S, double *M, double *flag) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    sub_gram(X, X, S);
     __syncthreads();
-    // right-looking Cholesky on the unscaled columns: step j subtracts S_ij S_lj / d_j
-    double pmin = 1.0;
-    for (int j = 0; j < r; ++j) {
-        const double d = S[j * kSubHld + j];
-        pmin = fmin(pmin, d);
-        if (!(d > 1e-10)) return 0.0;  // uniform: every thread read the same value
-        if (tid == 0) dj[j] = d;
-        const int m = r - j - 1;
-        const double dinv = 1.0 / d;
-        for (int e = tid; e < m * m; e += kEigThreads) {
-            const int i = j + 1 + e / m, l = j + 1 + e % m;
-            if (l <= i) S[i * kSubHld + l] -= S[i * kSubHld + j] * S[l * kSubHld + j] * dinv;
-        }
-        __syncthreads();
-    }
-    // L_ij = S_ij / sqrt(d_j), L_jj = sqrt(d_j); keep 1 / L_jj in dj
-    for (int e = tid; e < r * r; e += kEigThreads) {
-        const int i = e / r, j = e % r;
-        if (j < i) S[i * kSubHld + j] *= 1.0 / sqrt(dj[j]);
-    }
-    __syncthreads();
-    if (tid < r) dj[tid] = 1.0 / sqrt(dj[tid]);
-    __syncthreads();
-    // rows of Q = Xn L^{-T}: forward substitution L q = x per row
-    if (tid < n) {
-        double q[kSubMaxR];
-        double *xr = X + tid * kSubLd;
+    if (wv == 0) {
+        const int l = lane;
+        double a[kSubMaxR];
+#pragma unroll
+        for (int j = 0; j < kSubMaxR; ++j) a[j] = (l < r && j < r) ? S[l * kSubHld + j] : 0.0;
+        const double dg = (l < r) ? S[l * kSubHld + l] : 1.0;
+        const double cl = (dg > 0.0) ? 1.0 / sqrt(dg) : 0.0;
+        bool bad = false;
 #pragma unroll
         for (int j = 0; j < kSubMaxR; ++j) {
-            double s = 0.0;
-            if (j < r) {
-                s = xr[j] * cinv[j];
-                const double *lr = S + j * kSubHld;
-#pragma unroll
-                for (int k = 0; k < j; ++k) s = fma(-lr[k], q[k], s);
-                s *= dj[j];
-            }
-            q[j] = s;
+            const double cj = sub_rl(cl, j);
+            bad |= (j < r) && !(cj > 0.0);
+            a[j] *= cl * cj;
         }
+        double pmin = 1.0;
+        // right-looking Cholesky of the normalised Gram, whole symmetric matrix kept
 #pragma unroll
-        for (int j = 0; j < kSubMaxR; ++j)
-            if (j < r) xr[j] = q[j];
+        for (int j = 0; j < kSubMaxR; ++j) {
+            if (j < r && !bad) {
+                const double d = sub_rl(a[j], j);
+                pmin = fmin(pmin, d);
+                if (!(d > 1e-10)) {
+                    bad = true;
+                } else {
+                    const double inv = 1.0 / sqrt(d);
+                    const double lj = a[j] * inv;  // L[l][j] (for l == j: sqrt(d))
+                    a[j] = lj;
+#pragma unroll
+                    for (int m = j + 1; m < kSubMaxR; ++m) a[m] = fma(-lj, sub_rl(lj, m), a[m]);
+                }
+            }
+        }
+        // lane c < r: column c of L^{-1} by forward substitution, then row c of
+        // M = diag(cl) L^{-T}: M[c][i] = cl_c * Linv[i][c]
+        if (!bad) {
+            double y[kSubMaxR];
+#pragma unroll
+            for (int i = 0; i < kSubMaxR; ++i) {
+                double sacc = (i == l) ? 1.0 : 0.0;
+#pragma unroll
+                for (int j = 0; j < i; ++j) sacc = fma(-sub_rl(a[j], i), y[j], sacc);
+                y[i] = (i < r) ? sacc / sub_rl(a[i], i) : 0.0;
+            }
+            if (l < kSubMaxR)
+#pragma unroll
+                for (int i = 0; i < kSubMaxR; ++i) M[l * kSubHld + i] = (l < r) ? cl * y[i] : 0.0;
+        }
+        if (l == 0) flag[0] = bad ? 0.0 : pmin;
     }
     __syncthreads();
-    return pmin;
+    const double pm = flag[0];
+    if (!(pm > 0.0)) return 0.0;  // uniform
+    sub_d4 acc[2][2];
+    sub_xm(X, M, acc);
+    __syncthreads();  // every wave has read X
+    sub_store_rows(acc, X, n);
+    __syncthreads();
+    return pm;
 }
 
-__device__ inline bool sub_orth(double *X, int n, int r, double *S, double *cinv, double *dj, double *part) {
-    double p = sub_orth_pass(X, n, r, S, cinv, dj, part);
+__device__ inline bool sub_orth(double *X, int n, int r, double *S, double *M, double *flag) {
+    double p = sub_orth_pass(X, n, r, S, M, flag);
     if (p <= 0.0) return false;
-    if (p < 1e-2) p = sub_orth_pass(X, n, r, S, cinv, dj, part);  // CholQR2
+    if (p < 1e-2) p = sub_orth_pass(X, n, r, S, M, flag);  // CholQR2
     return p > 0.0;
-}
-
-// Y (n x r) = X (n x r) * M (r x r, stride kSubHld); thread (row, 16-column half)
-__device__ inline void sub_rmul(const double *X, const double *M, double *Y, int n, int r) {
-    const int i = threadIdx.x & 127, c0 = (threadIdx.x >> 7) * 16;
-    if (i < n && c0 < r) {
-        double acc[16];
-#pragma unroll
-        for (int u = 0; u < 16; ++u) acc[u] = 0.0;
-        const double *xr = X + i * kSubLd;
-        for (int k = 0; k < r; ++k) {
-            const double x = xr[k];
-            const double *mr = M + k * kSubHld + c0;
-#pragma unroll
-            for (int u = 0; u < 16; ++u) acc[u] = fma(x, mr[u], acc[u]);
-        }
-#pragma unroll
-        for (int u = 0; u < 16; ++u)
-            if (c0 + u < r) Y[i * kSubLd + c0 + u] = acc[u];
-    }
 }
 
 // Ag: n x n symmetric (column-major) in global memory.  U0: n x r FP32 start block
@@ -172,28 +202,23 @@ __device__ inline void sub_rmul(const double *X, const double *M, double *Y, int
 // nothing.  Launch with kEigThreads threads and eig_sub_doubles(n) * 8 bytes of dynamic LDS.
 __device__ inline bool sym_top_eigvecs_subspace(const double *Ag, int n, int r, const float *U0, float *U,
                                                 double *w, double *smem) {
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, lr = lane & 15, lk = lane >> 4;
     double *V = smem;
-    double *Z = V + (long)n * kSubLd;
-    double *H = Z + (long)n * kSubLd;
+    double *Z = V + kSubRows * kSubLd;
+    double *H = Z + kSubRows * kSubLd;
     double *Wm = H + kSubMaxR * kSubHld;
     double *S = Wm + kSubMaxR * kSubHld;
-    double *cinv = S + kSubMaxR * kSubHld;
-    double *dj = cinv + kSubMaxR;
-    double *part = dj + kSubMaxR;                // 8 x 32
-    double *jpar = part + 8 * kSubMaxR;           // Jacobi: 8 arrays of 16
-    double *red = jpar + 8 * 16;                  // reductions
-    double *jc = jpar, *js = jpar + 16, *jt = jpar + 32, *japp = jpar + 48, *jaqq = jpar + 64,
-           *japq = jpar + 80;
-    int *jp = reinterpret_cast<int *>(jpar + 96), *jq = jp + 16;
-    int *pos = reinterpret_cast<int *>(jpar + 112);  // 32 ints
-    double *sg = red + 8;                            // 32 signs
+    double *M = S + kSubMaxR * kSubHld;
+    double *red = M + kSubMaxR * kSubHld;  // 64 slots
+    double *sg = red + 64;                 // 32 signs
+    int *pos = reinterpret_cast<int *>(sg + 32);  // 32 ints
+    double *flag = sg + 48;
 
-    // start block; columns r.. stay zero throughout
-    for (int e = tid; e < n * kSubLd; e += kEigThreads) {
+    // start block; rows n.. and columns r.. stay zero throughout
+    for (int e = tid; e < kSubRows * kSubLd; e += kEigThreads) {
         const int i = e / kSubLd, c = e % kSubLd;
         double v = 0.0;
-        if (c < r) {
+        if (c < r && i < n) {
             if (U0) {
                 v = (double)U0[(long)c * n + i];
             } else {
@@ -211,75 +236,64 @@ __device__ inline bool sym_top_eigvecs_subspace(const double *Ag, int n, int r, 
         V[e] = v;
         Z[e] = 0.0;
     }
+    // A in registers as MFMA A operands: rows 32 wv + 16 rt + lr, columns 4 ks + lk
+    double gA[2][kSubRows / 4];
     double f2 = 0.0;
-    for (int e = tid; e < n * n; e += kEigThreads) f2 = fma(Ag[e], Ag[e], f2);
-    f2 = sub_blk_sum(f2, red);
-    if (!sub_orth(V, n, r, S, cinv, dj, part)) return false;
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int ks = 0; ks < kSubRows / 4; ++ks) {
+            const int i = 32 * wv + 16 * rt + lr, k = 4 * ks + lk;
+            const double g = (i < n && k < n) ? Ag[i + (long)k * n] : 0.0;
+            gA[rt][ks] = g;
+            f2 = fma(g, g, f2);
+        }
+    f2 = sub_blk_sum(f2, red);  // (its barriers also publish the start block)
+    if (!sub_orth(V, n, r, S, M, flag)) return false;
 
     double rho = 0.0, hf = 0.0, rho_prev = 0.0;
     bool conv = false;
     for (int it = 0; it < kSubMaxIt; ++it) {
-        // Z = A V: thread (row i, 16 columns); A from global (L2-resident), V rows broadcast
+        // Z = A V
         {
-            const int i = tid & 127, c0 = (tid >> 7) * 16;
-            if (i < n && c0 < r) {
-                double acc[16];
+            sub_d4 acc[2][2];
 #pragma unroll
-                for (int u = 0; u < 16; ++u) acc[u] = 0.0;
-                const double *acol = Ag + i;
-#pragma unroll 4
-                for (int k = 0; k < n; ++k) {
-                    const double a = acol[(long)k * n];
-                    const double *vr = V + k * kSubLd + c0;
+            for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
-                    for (int u = 0; u < 16; ++u) acc[u] = fma(a, vr[u], acc[u]);
+                for (int ct = 0; ct < 2; ++ct) acc[rt][ct] = sub_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int ks = 0; ks < kSubRows / 4; ++ks) {
+                const double *vr = V + (4 * ks + lk) * kSubLd + lr;
+                const double b0 = vr[0], b1 = vr[16];
+#pragma unroll
+                for (int rt = 0; rt < 2; ++rt) {
+                    acc[rt][0] = sub_mfma(gA[rt][ks], b0, acc[rt][0]);
+                    acc[rt][1] = sub_mfma(gA[rt][ks], b1, acc[rt][1]);
                 }
-#pragma unroll
-                for (int u = 0; u < 16; ++u) Z[i * kSubLd + c0 + u] = acc[u];
             }
+            sub_store_rows(acc, Z, n);
         }
         __syncthreads();
-        // S = V^T Z (all r x r entries), 2 x 2 blocks per thread
-        {
-            const int a = 2 * (tid >> 4), b = 2 * (tid & 15);
-            if (a < r && b < r) {
-                double s00 = 0.0, s01 = 0.0, s10 = 0.0, s11 = 0.0;
-                for (int i = 0; i < n; ++i) {
-                    const double va0 = V[i * kSubLd + a], va1 = V[i * kSubLd + a + 1];
-                    const double zb0 = Z[i * kSubLd + b], zb1 = Z[i * kSubLd + b + 1];
-                    s00 = fma(va0, zb0, s00);
-                    s01 = fma(va0, zb1, s01);
-                    s10 = fma(va1, zb0, s10);
-                    s11 = fma(va1, zb1, s11);
-                }
-                S[a * kSubHld + b] = s00;
-                if (b + 1 < r) S[a * kSubHld + b + 1] = s01;
-                if (a + 1 < r) {
-                    S[(a + 1) * kSubHld + b] = s10;
-                    if (b + 1 < r) S[(a + 1) * kSubHld + b + 1] = s11;
-                }
-            }
-        }
+        // S = V^T Z
+        sub_gram(V, Z, S);
         __syncthreads();
         // rho = ||Z - V S||_F, H = (S + S^T) / 2, ||H||_F
         double rr = 0.0, hh = 0.0;
         {
-            const int i = tid & 127, c0 = (tid >> 7) * 16;
-            if (i < n && c0 < r) {
-                double acc[16];
+            sub_d4 acc[2][2];
+            sub_xm(V, S, acc);
 #pragma unroll
-                for (int u = 0; u < 16; ++u) acc[u] = Z[i * kSubLd + c0 + u];
-                const double *vr = V + i * kSubLd;
-                for (int k = 0; k < r; ++k) {
-                    const double v = vr[k];
-                    const double *sr = S + k * kSubHld + c0;
+            for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
-                    for (int u = 0; u < 16; ++u) acc[u] = fma(-v, sr[u], acc[u]);
-                }
+                for (int ct = 0; ct < 2; ++ct)
 #pragma unroll
-                for (int u = 0; u < 16; ++u)
-                    if (c0 + u < r) rr = fma(acc[u], acc[u], rr);
-            }
+                    for (int v = 0; v < 4; ++v) {
+                        const int i = 32 * wv + 16 * rt + lk + 4 * v, c = 16 * ct + lr;
+                        if (i < n && c < r) {
+                            const double d = Z[i * kSubLd + c] - acc[rt][ct][v];
+                            rr = fma(d, d, rr);
+                        }
+                    }
         }
         for (int e = tid; e < r * r; e += kEigThreads) {
             const int a = e / r, b = e % r;
@@ -295,7 +309,7 @@ __device__ inline bool sym_top_eigvecs_subspace(const double *Ag, int n, int r, 
         }
         if (it >= 2 && rho > 0.25 * rho_prev) return false;  // too slow for this path
         rho_prev = rho;
-        if (!sub_orth(Z, n, r, S, cinv, dj, part)) return false;
+        if (!sub_orth(Z, n, r, S, M, flag)) return false;
         double *t = V;
         V = Z;
         Z = t;
@@ -303,85 +317,99 @@ __device__ inline bool sym_top_eigvecs_subspace(const double *Ag, int n, int r, 
     if (!conv || !(hf > 0.0)) return false;
 
     // ---- Rayleigh-Ritz: H = W diag(theta) W^T by cyclic Jacobi, round-robin pairs ----
-    for (int e = tid; e < r * kSubHld; e += kEigThreads) Wm[e] = ((e / kSubHld) == (e % kSubHld)) ? 1.0 : 0.0;
+    // Thread (group g = tid / 32, index j = tid % 32) handles pairs g and g + 8: it computes
+    // their rotations itself (the 32 threads of a group share a wave, so all of them read the
+    // pair's entries before any of them writes), rotates rows p, q of H in column j and
+    // columns p, q of W in row j, then (after a barrier) columns p, q of H in row j.
+    for (int e = tid; e < kSubMaxR * kSubHld; e += kEigThreads) Wm[e] = ((e / kSubHld) == (e % kSubHld)) ? 1.0 : 0.0;
+    __syncthreads();
     const int re = r + (r & 1), npair = re >> 1, m = re - 1;
+    const int jg = tid >> 5, jj = tid & 31;
     double off = 0.0;
-    bool jconv = false;
-    for (int sw = 0; sw < 16; ++sw) {
+    bool jconv = (m <= 0);
+    for (int sw = 0; sw < 16 && !jconv; ++sw) {
         for (int k = 0; k < m; ++k) {
-            if (tid < npair) {
+            int pp[2], qq[2];
+            double c_[2], s_[2], t_[2], app_[2], aqq_[2], apq_[2];
+            bool on[2];
+#pragma unroll
+            for (int h2 = 0; h2 < 2; ++h2) {
+                const int pi = jg + 8 * h2;
                 int p, q;
-                if (tid == 0) {
+                if (pi == 0) {
                     p = re - 1;
                     q = k;
                 } else {
-                    p = (k + tid) % m;
-                    q = (k - tid + m) % m;
+                    p = k + pi;
+                    if (p >= m) p -= m;
+                    q = k - pi + m;
+                    if (q >= m) q -= m;
                 }
+                on[h2] = pi < npair && p < r && q < r;
+                pp[h2] = p;
+                qq[h2] = q;
                 double c = 1.0, s = 0.0, t = 0.0, app = 0.0, aqq = 0.0, apq = 0.0;
-                const bool on = p < r && q < r;
-                if (on) {
+                if (on[h2]) {
                     app = H[p * kSubHld + p];
                     aqq = H[q * kSubHld + q];
                     apq = H[p * kSubHld + q];
                     if (apq != 0.0) {
-                        const double tau = (aqq - app) / (2.0 * apq);
-                        t = (fabs(tau) > 1e150) ? 0.5 / tau
-                                                : copysign(1.0, tau) / (fabs(tau) + sqrt(fma(tau, tau, 1.0)));
-                        c = 1.0 / sqrt(fma(t, t, 1.0));
+                        // t = sign(tau) / (|tau| + sqrt(tau^2 + 1)), c = 1 / sqrt(t^2 + 1), with
+                        // the hardware reciprocal / reciprocal-root estimates refined by two
+                        // Newton steps (full FP64 precision; c^2 + s^2 = 1 to rounding)
+                        const double tau = (aqq - app) * rcp_f64(2.0 * apq);
+                        if (fabs(tau) > 1e150) {
+                            t = 0.5 * rcp_f64(tau);
+                        } else {
+                            const double a2 = fma(tau, tau, 1.0);
+                            const double y = rsq_f64(a2);
+                            t = copysign(1.0, tau) * rcp_f64(fma(a2, y, fabs(tau)));
+                        }
+                        c = rsq_f64(fma(t, t, 1.0));
                         s = t * c;
                     }
                 }
-                jp[tid] = on ? p : -1;
-                jq[tid] = q;
-                jc[tid] = c;
-                js[tid] = s;
-                jt[tid] = t;
-                japp[tid] = app;
-                jaqq[tid] = aqq;
-                japq[tid] = apq;
+                c_[h2] = c;
+                s_[h2] = s;
+                t_[h2] = t;
+                app_[h2] = app;
+                aqq_[h2] = aqq;
+                apq_[h2] = apq;
+            }
+            // rows p, q of H (column jj) and columns p, q of W (row jj)
+#pragma unroll
+            for (int h2 = 0; h2 < 2; ++h2) {
+                if (!on[h2] || jj >= r) continue;
+                const int p = pp[h2], q = qq[h2];
+                const double c = c_[h2], s = s_[h2];
+                const double hp = H[p * kSubHld + jj], hq = H[q * kSubHld + jj];
+                const double wp = Wm[jj * kSubHld + p], wq = Wm[jj * kSubHld + q];
+                H[p * kSubHld + jj] = c * hp - s * hq;
+                H[q * kSubHld + jj] = s * hp + c * hq;
+                Wm[jj * kSubHld + p] = c * wp - s * wq;
+                Wm[jj * kSubHld + q] = s * wp + c * wq;
             }
             __syncthreads();
-            // rows p, q of H and columns p, q of W
-            for (int e = tid; e < 2 * npair * r; e += kEigThreads) {
-                const int which = e / (npair * r), rem = e % (npair * r);
-                const int pi = rem / r, j = rem % r;
-                const int p = jp[pi];
-                if (p < 0) continue;
-                const int q = jq[pi];
-                const double c = jc[pi], s = js[pi];
-                if (which == 0) {
-                    const double hp = H[p * kSubHld + j], hq = H[q * kSubHld + j];
-                    H[p * kSubHld + j] = c * hp - s * hq;
-                    H[q * kSubHld + j] = s * hp + c * hq;
-                } else {
-                    const double wp = Wm[j * kSubHld + p], wq = Wm[j * kSubHld + q];
-                    Wm[j * kSubHld + p] = c * wp - s * wq;
-                    Wm[j * kSubHld + q] = s * wp + c * wq;
-                }
-            }
-            __syncthreads();
-            // columns p, q of H; the (p, q) block is set exactly
-            for (int e = tid; e < npair * r; e += kEigThreads) {
-                const int pi = e / r, j = e % r;
-                const int p = jp[pi];
-                if (p < 0) continue;
-                const int q = jq[pi];
+            // columns p, q of H (row jj); the (p, q) block is set exactly
+#pragma unroll
+            for (int h2 = 0; h2 < 2; ++h2) {
+                if (!on[h2] || jj >= r) continue;
+                const int p = pp[h2], q = qq[h2];
                 double np_, nq_;
-                if (j == p) {
-                    np_ = japp[pi] - jt[pi] * japq[pi];
+                if (jj == p) {
+                    np_ = app_[h2] - t_[h2] * apq_[h2];
                     nq_ = 0.0;
-                } else if (j == q) {
+                } else if (jj == q) {
                     np_ = 0.0;
-                    nq_ = jaqq[pi] + jt[pi] * japq[pi];
+                    nq_ = aqq_[h2] + t_[h2] * apq_[h2];
                 } else {
-                    const double c = jc[pi], s = js[pi];
-                    const double hp = H[j * kSubHld + p], hq = H[j * kSubHld + q];
+                    const double c = c_[h2], s = s_[h2];
+                    const double hp = H[jj * kSubHld + p], hq = H[jj * kSubHld + q];
                     np_ = c * hp - s * hq;
                     nq_ = s * hp + c * hq;
                 }
-                H[j * kSubHld + p] = np_;
-                H[j * kSubHld + q] = nq_;
+                H[jj * kSubHld + p] = np_;
+                H[jj * kSubHld + q] = nq_;
             }
             __syncthreads();
         }
@@ -391,10 +419,7 @@ __device__ inline bool sym_top_eigvecs_subspace(const double *Ag, int n, int r, 
             if (a != b) o = fma(H[a * kSubHld + b], H[a * kSubHld + b], o);
         }
         off = sqrt(sub_blk_sum(o, red));
-        if (off <= 1e-14 * hf) {
-            jconv = true;
-            break;
-        }
+        jconv = off <= 1e-14 * hf;
     }
     if (!jconv) return false;
 
@@ -411,7 +436,11 @@ __device__ inline bool sym_top_eigvecs_subspace(const double *Ag, int n, int r, 
     if (!(tmin - eb > sqrt(rest))) return false;
 
     // ---- eigenvectors Y = V W (into Z), ascending order, sign, FP32 ----
-    sub_rmul(V, Wm, Z, n, r);
+    {
+        sub_d4 acc[2][2];
+        sub_xm(V, Wm, acc);
+        sub_store_rows(acc, Z, n);
+    }
     if (tid < r) {
         const double th = H[tid * kSubHld + tid];
         int rank = 0;
