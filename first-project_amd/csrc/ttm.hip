@@ -12,7 +12,7 @@
 //    block t holds rows 8 m + t, BF16, or 64 h + 4 m + t, FP32, m = lane % 16).
 //  !TR (P == 1): output rows are q, the contracted i is contiguous: direct 16-byte loads.
 // The accumulator tile goes through LDS (32 columns at a time) so that Y is written with
-// contiguous 256-byte stores.  Each output is a sum over i in a fixed order.
+// contiguous 256-byte stores.  X is read with nontemporal loads (streamed once).  Each output is a sum over i in a fixed order.
 #include "common.hpp"
 #include <type_traits>
 
@@ -154,7 +154,7 @@ __global__ __launch_bounds__(256, 1) void k_ttm_mfma(const void *__restrict__ Xv
                         for (int j = 0; j < 8; ++j) {
                             const long i = (long)s * KS + 8 * lg + j;
                             const bool in = rok[0] && i < In;
-                            v[cs_][j] = *reinterpret_cast<const VT *>(X + xb[0] + (in ? i : 0) * P);
+                            v[cs_][j] = ld_stream(reinterpret_cast<const VT *>(X + xb[0] + (in ? i : 0) * P));
                             ok[cs_] |= (uint32_t)in << j;
                         }
                     } else if constexpr (TR) {
@@ -164,7 +164,7 @@ __global__ __launch_bounds__(256, 1) void k_ttm_mfma(const void *__restrict__ Xv
                             for (int c = 0; c < 4; ++c) {
                                 const long i = (long)s * KS + 4 * lg + c;
                                 const bool in = rok[h] && i < In;
-                                v[cs_][4 * h + c] = *reinterpret_cast<const VT *>(X + xb[h] + (in ? i : 0) * P);
+                                v[cs_][4 * h + c] = ld_stream(reinterpret_cast<const VT *>(X + xb[h] + (in ? i : 0) * P));
                                 ok[cs_] |= (uint32_t)in << (4 * h + c);
                             }
                     } else {
@@ -173,7 +173,7 @@ __global__ __launch_bounds__(256, 1) void k_ttm_mfma(const void *__restrict__ Xv
                         for (int b = 0; b < 8; ++b) {
                             const long q = r0 + 16 * b + lm;
                             const bool in = q < rows && i < In;
-                            v[cs_][b] = *reinterpret_cast<const VT *>(X + (in ? i + q * In : 0));
+                            v[cs_][b] = ld_stream(reinterpret_cast<const VT *>(X + (in ? i + q * In : 0)));
                             ok[cs_] |= (uint32_t)in << b;
                         }
                     }
