@@ -5,7 +5,9 @@
 // Each workgroup pulls tensors from a size-sorted work queue (one atomic counter, reset by
 // a one-thread kernel at the start of every sweep so the sweep can be replayed as a graph)
 // and performs the whole Gauss-Seidel sweep of its tensor locally:
-//   for n in 0..2:  M = MTTKRP_n streamed slab by slab (X(:,:,k0:k0+kc) in LDS, coalesced),
+//   for n in 0..2:  M = MTTKRP_n streamed slab by slab (X(:,:,k0:k0+kc) in LDS, coalesced;
+//                   FP32 MFMA: mode 0 M += slab * KR rows, modes 1, 2 T = slab^T * A then a
+//                   segmented sum over the slab's columns),
 //                   V = hadamard of the other Grams, FP64 sweep-operator inverse (Jacobi
 //                   pseudo-inverse fallback), U_n = M V^{-1}, G_n = U_n^T U_n (FP64)
 //   relerr from the fast-fit identity, appended to the tensor's history.
@@ -31,6 +33,8 @@ struct BDesc {
     int I, J, K, R;
 };
 
+typedef float bf32x4 __attribute__((ext_vector_type(4)));
+
 template <int BS, int SLAB>
 __global__ __launch_bounds__(BS) void k_cp_batched_sweep(const float *__restrict__ X, float *__restrict__ U,
                                                          const BDesc *__restrict__ desc, const int *__restrict__ sched,
@@ -40,6 +44,7 @@ __global__ __launch_bounds__(BS) void k_cp_batched_sweep(const float *__restrict
                                                          const int *__restrict__ sweep_ctr, int *__restrict__ status) {
     __shared__ float slab[SLAB];
     __shared__ float Us[3][kBMaxDim * kBMaxR];
+    __shared__ float Ut[kBMaxDim * kBMaxR];  // A transposed, [i][r] with r < 16 (zero padded)
     __shared__ float Ms[kBMaxDim * kBMaxR];
     __shared__ float Ys[kSlabCols * kBMaxR > kBMaxDim * kBMaxR ? kSlabCols * kBMaxR : kBMaxDim * kBMaxR];
     __shared__ double Gs[3][kBMaxR * kBMaxR];
@@ -72,62 +77,135 @@ __global__ __launch_bounds__(BS) void k_cp_batched_sweep(const float *__restrict
             }
         __syncthreads();
         // slabs are runs of consecutive columns c = j + k*J of the mode-1 unfolding
-        // (contiguous in memory); at most kSlabCols columns so the per-column scratch fits
+        // (contiguous in memory), stored in LDS with column stride I + 1 (the MFMA operand
+        // reads walk both indices); at most kSlabCols columns so the per-column scratch fits
         const int ncol = d.J * d.K;
-        const int CC = min(SLAB / d.I, kSlabCols);
+        const int Ip = d.I + 1;
+        const int CC = min(SLAB / Ip, kSlabCols);
+        constexpr int NW = BS / 64;
+        const int lane = tid & 63, wv = tid >> 6, lr = lane & 15, lk = lane >> 4;
         double ip_last = 0.0;
         for (int n = 0; n < 3; ++n) {
             const int In = dims[n];
             for (int e = tid; e < In * R; e += BS) Ms[e] = 0.0f;
-            // -------- MTTKRP, slab by slab --------
+            if (n == 1) {
+                // modes 1 and 2 contract i with the (just updated) A: A^T rows, zero padded
+                for (int e = tid; e < kBMaxDim * kBMaxR; e += BS) {
+                    const int i = e >> 4, r = e & 15;
+                    Ut[e] = (i < d.I && r < R) ? Us[0][i + r * d.I] : 0.0f;
+                }
+            }
+            // mode 0 accumulators: row tiles wv + NW u of M (16 rows x 16 columns each)
+            bf32x4 acc0[(8 + NW - 1) / NW];
+#pragma unroll
+            for (int u = 0; u < (8 + NW - 1) / NW; ++u) acc0[u] = bf32x4{0.f, 0.f, 0.f, 0.f};
+            // -------- MTTKRP, slab by slab; the next slab's elements are loaded into
+            // registers while the current one is multiplied --------
+            // element e = tid + BS u of a slab is (i, q) = (e mod I, e div I); (i0, q0) for
+            // u = 0 and the per-u step (di, dq) are computed once
+            const int q0 = tid / d.I, i0 = tid - q0 * d.I, dq = BS / d.I, di = BS - dq * d.I;
+            constexpr int EPT = (SLAB + BS - 1) / BS;
+            float pre[EPT];
+            auto fetch = [&](int cs) {  // loads of the slab at column cs (clamped into it)
+                const int ccs = min(CC, ncol - cs);
+                const long base = (long)cs * d.I, last = (long)ccs * d.I - 1;
+#pragma unroll
+                for (int u = 0; u < EPT; ++u) {
+                    const long e = tid + (long)BS * u;
+                    pre[u] = x[base + (e < last ? e : last)];
+                }
+            };
+            fetch(0);
             for (int c0 = 0; c0 < ncol; c0 += CC) {
                 const int cc = min(CC, ncol - c0);
-                for (int e = tid; e < cc * d.I; e += BS) slab[e] = x[(long)c0 * d.I + e];
+                {
+                    int i = i0, q = q0;
+#pragma unroll
+                    for (int u = 0; u < EPT; ++u) {
+                        if (q < cc) slab[i + q * Ip] = pre[u];
+                        i += di;
+                        q += dq;
+                        if (i >= d.I) {
+                            i -= d.I;
+                            ++q;
+                        }
+                    }
+                }
                 if (n == 0) {
                     // Khatri-Rao rows of the slab's columns: Ys[q*16 + r] = B[j_q,r] C[k_q,r]
-                    for (int e = tid; e < cc * R; e += BS) {
-                        const int q = e / R, r = e % R, c = c0 + q;
-                        Ys[q * kBMaxR + r] = Us[1][c % d.J + r * d.J] * Us[2][c / d.J + r * d.K];
+                    for (int e = tid; e < cc * kBMaxR; e += BS) {
+                        const int q = e >> 4, r = e & 15, c = c0 + q;
+                        Ys[e] = (r < R) ? Us[1][c % d.J + r * d.J] * Us[2][c / d.J + r * d.K] : 0.0f;
                     }
                 }
                 __syncthreads();
+                fetch((c0 + CC < ncol) ? c0 + CC : c0);  // the last slab re-reads itself (never stored)
                 if (n == 0) {
-                    for (int e = tid; e < d.I * R; e += BS) {
-                        const int i = e % d.I, r = e / d.I;
-                        float acc = 0.0f;
-                        for (int q = 0; q < cc; ++q) acc = fmaf(slab[i + q * d.I], Ys[q * kBMaxR + r], acc);
-                        Ms[e] += acc;
-                    }
-                } else if (n == 1) {
-                    // M[j,r] += sum over the slab's columns with this j of C[k,r] sum_i S[i,c] A[i,r]
-                    for (int e = tid; e < d.J * R; e += BS) {
-                        const int j = e % d.J, r = e / d.J;
-                        float acc = 0.0f;
-                        for (int q = (j - c0 % d.J + d.J) % d.J; q < cc; q += d.J) {
-                            const float *sc = slab + q * d.I;
-                            float a2 = 0.0f;
-                            for (int i = 0; i < d.I; ++i) a2 = fmaf(sc[i], Us[0][i + r * d.I], a2);
-                            acc = fmaf(a2, Us[2][(c0 + q) / d.J + r * d.K], acc);
+                    // M += slab (I x cc) * KR (cc x 16): A[i][q] = slab, B[q][r] = Ys
+                    const int nks = (cc + 3) >> 2;
+#pragma unroll
+                    for (int u = 0; u < (8 + NW - 1) / NW; ++u) {
+                        const int t = wv + NW * u;
+                        if (16 * t < d.I) {
+                            const int i = 16 * t + lr;
+                            for (int ks = 0; ks < nks; ++ks) {
+                                const int q = 4 * ks + lk;
+                                const float a = (i < d.I && q < cc) ? slab[i + q * Ip] : 0.0f;
+                                const float b = (q < cc) ? Ys[q * kBMaxR + lr] : 0.0f;
+                                acc0[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc0[u], 0, 0, 0);
+                            }
                         }
-                        Ms[e] += acc;
                     }
                 } else {
-                    // Ys[q,r] = B[j_q,r] sum_i S[i,q] A[i,r]; then M[k,r] += sum of Ys over k's columns
-                    for (int e = tid; e < cc * R; e += BS) {
-                        const int q = e % cc, r = e / cc, c = c0 + q;
-                        const float *sc = slab + q * d.I;
-                        float a2 = 0.0f;
-                        for (int i = 0; i < d.I; ++i) a2 = fmaf(sc[i], Us[0][i + r * d.I], a2);
-                        Ys[q * kBMaxR + r] = a2 * Us[1][c % d.J + r * d.J];
+                    // T (cc x 16) = slab^T (cc x I) * A (I x 16) -> Ys
+                    const int nks = (d.I + 3) >> 2;
+                    for (int t = wv; 16 * t < cc; t += NW) {
+                        bf32x4 tacc = {0.f, 0.f, 0.f, 0.f};
+                        const int q = 16 * t + lr;
+                        for (int ks = 0; ks < nks; ++ks) {
+                            const int i = 4 * ks + lk;
+                            const float a = (q < cc && i < d.I) ? slab[i + q * Ip] : 0.0f;
+                            tacc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Ut[i * kBMaxR + lr], tacc, 0, 0, 0);
+                        }
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            const int qq = 16 * t + 4 * lk + v;
+                            if (qq < cc) Ys[qq * kBMaxR + lr] = tacc[v];
+                        }
                     }
                     __syncthreads();
-                    const int kf = c0 / d.J, kl = (c0 + cc - 1) / d.J;
-                    for (int e = tid; e < (kl - kf + 1) * R; e += BS) {
-                        const int k = kf + e / R, r = e % R;
-                        const int qa = max(0, k * d.J - c0), qb = min(cc, (k + 1) * d.J - c0);
-                        float acc = 0.0f;
-                        for (int q = qa; q < qb; ++q) acc += Ys[q * kBMaxR + r];
-                        Ms[k + r * d.K] += acc;
+                    if (n == 1) {
+                        // M[j,r] += sum over the slab's columns with this j of T[q,r] C[k_q,r]
+                        for (int e = tid; e < d.J * R; e += BS) {
+                            const int j = e % d.J, r = e / d.J;
+                            float a2 = 0.0f;
+                            for (int q = (j - c0 % d.J + d.J) % d.J; q < cc; q += d.J)
+                                a2 = fmaf(Ys[q * kBMaxR + r], Us[2][(c0 + q) / d.J + r * d.K], a2);
+                            Ms[e] += a2;
+                        }
+                    } else {
+                        // M[k,r] += sum over k's columns of T[q,r] B[j_q,r]
+                        const int kf = c0 / d.J, kl = (c0 + cc - 1) / d.J;
+                        for (int e = tid; e < (kl - kf + 1) * R; e += BS) {
+                            const int k = kf + e / R, r = e % R;
+                            const int qa = max(0, k * d.J - c0), qb = min(cc, (k + 1) * d.J - c0);
+                            float a2 = 0.0f;
+                            for (int q = qa; q < qb; ++q) a2 = fmaf(Ys[q * kBMaxR + r], Us[1][(c0 + q) % d.J + r * d.J], a2);
+                            Ms[k + r * d.K] += a2;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+            if (n == 0) {
+                // mode-0 accumulators -> M (row 4 (lane / 16) + v of the tile, column lane % 16)
+#pragma unroll
+                for (int u = 0; u < (8 + NW - 1) / NW; ++u) {
+                    const int t = wv + NW * u;
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int i = 16 * t + 4 * lk + v;
+                        if (i < d.I && lr < R) Ms[i + lr * d.I] = acc0[u][v];
                     }
                 }
                 __syncthreads();
@@ -226,6 +304,8 @@ static const Variant kVariants[] = {
     {256, 8192, k_cp_batched_sweep<256, 8192>, "bs256_slab8k"},
     {512, 8192, k_cp_batched_sweep<512, 8192>, "bs512_slab8k"},
     {256, 12288, k_cp_batched_sweep<256, 12288>, "bs256_slab12k"},
+    {512, 12288, k_cp_batched_sweep<512, 12288>, "bs512_slab12k"},
+    {512, 16384, k_cp_batched_sweep<512, 16384>, "bs512_slab16k"},
 };
 static constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
