@@ -31,6 +31,8 @@ static constexpr int kGramRows = 64;   // rows per Gram-partial workgroup
 // V = hadamard_{m != mode} G_m; Vinv = V^{-1} by Cholesky (lower) and L^{-1}; on failure
 // a pseudo-inverse from the Jacobi eigendecomposition (eigenvalues below 1e-12 * max
 // are dropped) and status |= 1.
+// EPT: register slots per thread of the inverse, ceil(R^2 / 256) (1, 4 or 16 for R <= 16, 32, 64)
+template <int EPT>
 __global__ __launch_bounds__(256) void k_vinv(const double *__restrict__ G0, const double *__restrict__ G1,
                                               const double *__restrict__ G2, const double *__restrict__ G3,
                                               int order, int mode, int R, double *__restrict__ Vinv,
@@ -51,7 +53,7 @@ __global__ __launch_bounds__(256) void k_vinv(const double *__restrict__ G0, con
         V[e] = v;
     }
     __syncthreads();
-    const int fb = block_spd_inverse<kMaxR * kMaxR / 256>(V, V, W, Vinv, R, cs, sh, &fail);
+    const int fb = block_spd_inverse<EPT>(V, V, W, Vinv, R, cs, sh, &fail);
     if (fb && threadIdx.x == 0) atomicOr(status, 1);
 }
 
@@ -730,9 +732,13 @@ static int enqueue_sweep(CpPlan &P, const Storage &S, hipStream_t s, bool fit) {
             TDX_HIP(hipEventRecord(P.ev_fork[n], s));
             TDX_HIP(hipStreamWaitEvent(P.side, P.ev_fork[n], 0));
         }
-        k_vinv<<<1, 256, 0, beside ? P.side : s>>>(P.G[0], P.G[1], P.order > 2 ? P.G[2] : P.G[0],
-                                                  P.order > 3 ? P.G[3] : P.G[0], P.order, n, R, P.Vinv[n],
-                                                  P.status);
+        {
+            const hipStream_t vs = beside ? P.side : s;
+            const double *g2 = P.order > 2 ? P.G[2] : P.G[0], *g3 = P.order > 3 ? P.G[3] : P.G[0];
+            if (R <= 16) k_vinv<1><<<1, 256, 0, vs>>>(P.G[0], P.G[1], g2, g3, P.order, n, R, P.Vinv[n], P.status);
+            else if (R <= 32) k_vinv<4><<<1, 256, 0, vs>>>(P.G[0], P.G[1], g2, g3, P.order, n, R, P.Vinv[n], P.status);
+            else k_vinv<16><<<1, 256, 0, vs>>>(P.G[0], P.G[1], g2, g3, P.order, n, R, P.Vinv[n], P.status);
+        }
         TDX_LAUNCH_CHECK();
         if (beside) TDX_HIP(hipEventRecord(P.ev_join[n], P.side));
         const float *panels[4] = {P.panel[0], P.panel[1], P.panel[2], P.panel[3]};

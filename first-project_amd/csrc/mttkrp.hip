@@ -507,8 +507,12 @@ int mttkrp_plan(MttkrpPlan &p, int order, const long *dims, int mode, int dtype,
         if (ns < 1) ns = 1;
         long cps = 1, nss = 1;
         if (ns >= nfc) {
+            // at least 8 slow indices per split: small tensors otherwise produce many tiny
+            // splits whose partials cost more to sum than they save
             nss = ns / nfc;
-            if (nss > S) nss = S;
+            const long cap = ceil_div(S, 8L);
+            if (nss > cap) nss = cap;
+            if (nss < 1) nss = 1;
         } else {
             cps = ceil_div(nfc, ns);
         }
