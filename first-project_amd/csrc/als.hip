@@ -28,33 +28,39 @@ static constexpr int kMaxR = 64;     // rank limit of the LDS-resident solve
 static constexpr int kSolveRows = 16;  // rows per k_solve workgroup
 static constexpr int kGramRows = 64;   // rows per Gram-partial workgroup
 
-// V = hadamard_{m != mode} G_m; Vinv = V^{-1} by Cholesky (lower) and L^{-1}; on failure
-// a pseudo-inverse from the Jacobi eigendecomposition (eigenvalues below 1e-12 * max
-// are dropped) and status |= 1.
-// EPT: register slots per thread of the inverse, ceil(R^2 / 256) (1, 4 or 16 for R <= 16, 32, 64)
-template <int EPT>
+// V = hadamard_{m != mode} G_m; Vinv = V^{-1} by the symmetric sweep operator (FP64, the
+// register-row form of sweep_inverse_rows); if a pivot fails, a pseudo-inverse from the
+// Jacobi eigendecomposition (eigenvalues below 1e-12 * max dropped) and status |= 1.
+// NS: register slots per lane, R <= 4 NS (4, 8 or 16 for R <= 16, 32, 64).
+// About 4 KB of LDS, so that the workgroup fits on a CU beside an MTTKRP or TTM workgroup
+// (up to 136 KB): V is read from the Grams, and the rare fallback works in fb_ws (global,
+// 2 R^2 doubles: a copy of V and the eigenvectors).
+template <int NS>
 __global__ __launch_bounds__(256) void k_vinv(const double *__restrict__ G0, const double *__restrict__ G1,
                                               const double *__restrict__ G2, const double *__restrict__ G3,
                                               int order, int mode, int R, double *__restrict__ Vinv,
-                                              int *__restrict__ status) {
-    // 64 KB of LDS, so that the workgroup fits beside an MTTKRP workgroup on one CU: the
-    // fallback's working copy of V is V itself (only its diagonal is read during the sweep,
-    // and the fallback needs V no more once it starts)
-    __shared__ double V[kMaxR * kMaxR];
-    __shared__ double W[kMaxR * kMaxR];
+                                              int *__restrict__ status, double *__restrict__ fb_ws) {
+    __shared__ double Wb[128];
+    __shared__ double dg[kMaxR];
     __shared__ double sh[256];
     __shared__ double cs[4 * (kMaxR / 2 + 1)];
     __shared__ int fail;
     const double *G[4] = {G0, G1, G2, G3};
-    for (int e = threadIdx.x; e < R * R; e += blockDim.x) {
+    auto vat = [&](int i, int c) {
         double v = 1.0;
         for (int m = 0; m < order; ++m)
-            if (m != mode) v *= G[m][e];
-        V[e] = v;
+            if (m != mode) v *= G[m][i + c * R];
+        return v;
+    };
+    for (int k = threadIdx.x; k < R; k += blockDim.x) dg[k] = vat(k, k);
+    // (sweep_inverse_rows starts with a barrier)
+    if (sweep_inverse_rows<NS>(vat, dg, Wb, Vinv, R, &fail)) {
+        double *L = fb_ws, *Q = fb_ws + R * R;
+        for (int e = threadIdx.x; e < R * R; e += blockDim.x) L[e] = vat(e % R, e / R);
+        __syncthreads();
+        pinv_jacobi(L, Q, Vinv, R, cs, sh);
+        if (threadIdx.x == 0) atomicOr(status, 1);
     }
-    __syncthreads();
-    const int fb = block_spd_inverse<EPT>(V, V, W, Vinv, R, cs, sh, &fail);
-    if (fb && threadIdx.x == 0) atomicOr(status, 1);
 }
 
 // M = sum of the split-K partials of the MTTKRP, FP64, in split order (deterministic);
@@ -86,7 +92,8 @@ __global__ __launch_bounds__(256) void k_solve(const float *__restrict__ partial
                                                long In, int R, const double *__restrict__ Vinv,
                                                float *__restrict__ U, float *__restrict__ panel,
                                                const float *__restrict__ panel_scale, double *__restrict__ ip_part,
-                                               double *__restrict__ gram_part, int *__restrict__ skip_reset) {
+                                               double *__restrict__ gram_part, int *__restrict__ skip_reset,
+                                               int quant) {
     __shared__ double Vs[kMaxR * kMaxR];
     __shared__ double Ms[kSolveRows * kMaxR];  // M rows, then the U rows (as rounded FP32 values)
     __shared__ double sh[256];
@@ -126,24 +133,32 @@ __global__ __launch_bounds__(256) void k_solve(const float *__restrict__ partial
             for (int q = 0; q < 4; ++q) acc[q] = fma(m, vr[q], acc[q]);
         }
     }
-    float uf[4];
+    // quant: the factor becomes the BF16 value of its panel entry (u sc rounded to BF16,
+    // divided by sc), so that the factor, its Gram and the panel the shared pass reads as
+    // BF16 MFMA operands describe the same iterate
+    const float sc = (panel_scale && i < In) ? panel_scale[i] : 1.0f;
+    float uf[4], pf[4];
     double ip = 0.0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const bool in = i < In && b0 + q < R;
         uf[q] = in ? (float)acc[q] : 0.0f;
+        pf[q] = uf[q] * sc;
+        if (quant) {
+            pf[q] = (float)(__bf16)pf[q];
+            uf[q] = pf[q] / sc;
+        }
         if (in) ip += Ms[r * kMaxR + b0 + q] * (double)uf[q];
     }
     __syncthreads();  // every thread is done reading M
     if (b0 < Rp) {
-        const float sc = (panel_scale && i < In) ? panel_scale[i] : 1.0f;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int b = b0 + q;
             if (b >= Rp) break;
             Ms[r * kMaxR + b] = (double)uf[q];
             if (i < In && b < R) U[i + (long)b * In] = uf[q];
-            panel[((i >> 3) * Rp + b) * 8 + (i & 7)] = uf[q] * sc;
+            panel[((i >> 3) * Rp + b) * 8 + (i & 7)] = pf[q];
         }
     }
     ip = block_sum(ip, sh);  // (its barriers also publish the U rows)
@@ -534,6 +549,10 @@ struct CpPlan {
     int *status = nullptr;
     double *normx2 = nullptr;
     double *scratch = nullptr;
+    double *vinv_ws = nullptr;  // k_vinv fallback workspace, 2 R^2 doubles
+    bool shared_pass = true;    // modes 1 and 2 from one pass over X where eligible
+    float *T = nullptr;         // shared-pass intermediate X x_0 A^T (I2 x I1 x Rp)
+    size_t t_floats = 0;
     int max_sweeps_hist = 0;
     hipStream_t side = nullptr;
     hipEvent_t ev_fork[4], ev_join[4];
@@ -576,11 +595,12 @@ struct CpPlan {
             if (Vinv[m]) (void)hipFree(Vinv[m]);
             panel[m] = nullptr; G[m] = nullptr; Vinv[m] = nullptr;
         }
-        void *ps[] = {partial, msum, ip_part, gram_part, fit_hist, counter, status, normx2, scratch};
+        void *ps[] = {partial, msum, ip_part, gram_part, fit_hist, counter, status, normx2, scratch, vinv_ws, T};
         for (void *p : ps)
             if (p) (void)hipFree(p);
         partial = nullptr; msum = nullptr; ip_part = nullptr; gram_part = nullptr; fit_hist = nullptr;
-        counter = nullptr; status = nullptr; normx2 = nullptr; scratch = nullptr;
+        counter = nullptr; status = nullptr; normx2 = nullptr; scratch = nullptr; vinv_ws = nullptr; T = nullptr;
+        t_floats = 0;
         if (events) {
             for (int m = 0; m < 4; ++m) {
                 (void)hipEventDestroy(ev_fork[m]);
@@ -598,6 +618,7 @@ struct Storage {
     int dtype;
     const float *scales;  // FP8 slice scales
     MttkrpPlan mp[4];
+    DtreePlan dt;         // order 3, BF16/FP8: modes 1 and 2 from one shared pass over X
 };
 
 static int plan_setup(CpPlan &P, int order, const long *dims, long R, float *const *factors, int max_hist) {
@@ -622,6 +643,7 @@ static int plan_setup(CpPlan &P, int order, const long *dims, long R, float *con
     TDX_HIP(hipMalloc(&P.status, sizeof(int)));
     TDX_HIP(hipMalloc(&P.normx2, sizeof(double)));
     TDX_HIP(hipMalloc(&P.scratch, sizeof(double) * 4096));
+    TDX_HIP(hipMalloc(&P.vinv_ws, sizeof(double) * 2 * R * R));
     if (P.ls_mode) {
         auto alloc = [&](void **p, size_t bytes) -> int {
             TDX_HIP(hipMalloc(p, bytes));
@@ -657,6 +679,15 @@ static int ensure_partial(CpPlan &P, const Storage &S) {
     size_t need = 0;
     for (int m = 0; m < P.order; ++m)
         if (S.mp[m].partial_floats() > need) need = S.mp[m].partial_floats();
+    if (S.dt.on) {
+        if (S.dt.partial_floats() > need) need = S.dt.partial_floats();
+        if (S.dt.t_floats() > P.t_floats) {
+            if (P.T) TDX_HIP(hipFree(P.T));
+            P.T = nullptr;
+            TDX_HIP(hipMalloc(&P.T, sizeof(float) * S.dt.t_floats()));
+            P.t_floats = S.dt.t_floats();
+        }
+    }
     if (need > P.partial_floats) {
         if (P.partial) TDX_HIP(hipFree(P.partial));
         P.partial = nullptr;
@@ -683,6 +714,8 @@ static int storage_setup(CpPlan &P, Storage &S, const void *X, int dtype, const 
         int st = mttkrp_plan(S.mp[m], P.order, P.dims, m, dtype, P.R, X);
         if (st) return st;
     }
+    int st = dtree_plan(S.dt, P.order, P.dims, dtype, P.R, P.shared_pass);
+    if (st) return st;
     return ensure_partial(P, S);
 }
 
@@ -727,7 +760,10 @@ static int enqueue_sweep(CpPlan &P, const Storage &S, hipStream_t s, bool fit) {
         // When the MTTKRP grid leaves a workgroup slot free on every CU (one workgroup per CU)
         // the inverse runs beside it on the side stream; otherwise ahead of it on the same
         // stream, since it would hold a CU the MTTKRP grid needs.
-        const bool beside = S.mp[n].rowtiles * (long)S.mp[n].nsplit <= device_cus();
+        // Shared pass (order 3, BF16/FP8): mode 1 from one pass over X that also writes T,
+        // mode 2 from T.  The inverse (4 KB of LDS) fits beside either kernel's workgroups.
+        const bool dt = S.dt.on && n > 0;
+        const bool beside = dt || S.mp[n].rowtiles * (long)S.mp[n].nsplit <= device_cus();
         if (beside) {
             TDX_HIP(hipEventRecord(P.ev_fork[n], s));
             TDX_HIP(hipStreamWaitEvent(P.side, P.ev_fork[n], 0));
@@ -735,23 +771,41 @@ static int enqueue_sweep(CpPlan &P, const Storage &S, hipStream_t s, bool fit) {
         {
             const hipStream_t vs = beside ? P.side : s;
             const double *g2 = P.order > 2 ? P.G[2] : P.G[0], *g3 = P.order > 3 ? P.G[3] : P.G[0];
-            if (R <= 16) k_vinv<1><<<1, 256, 0, vs>>>(P.G[0], P.G[1], g2, g3, P.order, n, R, P.Vinv[n], P.status);
-            else if (R <= 32) k_vinv<4><<<1, 256, 0, vs>>>(P.G[0], P.G[1], g2, g3, P.order, n, R, P.Vinv[n], P.status);
-            else k_vinv<16><<<1, 256, 0, vs>>>(P.G[0], P.G[1], g2, g3, P.order, n, R, P.Vinv[n], P.status);
+            if (R <= 16) k_vinv<4><<<1, 256, 0, vs>>>(P.G[0], P.G[1], g2, g3, P.order, n, R, P.Vinv[n], P.status,
+                                                    P.vinv_ws);
+            else if (R <= 32) k_vinv<8><<<1, 256, 0, vs>>>(P.G[0], P.G[1], g2, g3, P.order, n, R, P.Vinv[n], P.status,
+                                                    P.vinv_ws);
+            else k_vinv<16><<<1, 256, 0, vs>>>(P.G[0], P.G[1], g2, g3, P.order, n, R, P.Vinv[n], P.status,
+                                                    P.vinv_ws);
         }
         TDX_LAUNCH_CHECK();
         if (beside) TDX_HIP(hipEventRecord(P.ev_join[n], P.side));
         const float *panels[4] = {P.panel[0], P.panel[1], P.panel[2], P.panel[3]};
         const float *row_scale = (S.dtype == TDX_FP8_E4M3 && n == 0) ? S.scales : nullptr;
         const int *skip = (P.ls_mode && n == 0) ? &P.ls->skip : nullptr;
-        int st = mttkrp_launch(S.mp[n], S.X, panels, row_scale, P.partial, s, skip);
+        int st;
+        int nsplit = S.mp[n].nsplit;
+        long rows_pad = S.mp[n].rows_pad;
+        if (dt && n == 1) {
+            st = dtree_m1_launch(S.dt, S.X, P.panel[0], P.U[2], P.T, P.partial, s);
+            nsplit = S.dt.nsplit1;
+            rows_pad = S.dt.I1;
+        } else if (dt && n == 2) {
+            st = dtree_m2_launch(S.dt, P.T, P.panel[1], P.partial, s);
+            nsplit = S.dt.nsplit2;
+            rows_pad = S.dt.I2;
+        } else {
+            st = mttkrp_launch(S.mp[n], S.X, panels, row_scale, P.partial, s, skip);
+        }
         if (st) return st;
         if (beside) TDX_HIP(hipStreamWaitEvent(s, P.ev_join[n], 0));
         const long In = P.dims[n];
         const int nblk = (int)ceil_div(In, kSolveRows);
         const float *pscale = (S.dtype == TDX_FP8_E4M3 && n == 0) ? S.scales : nullptr;
-        k_solve<<<nblk, 256, 0, s>>>(P.partial, S.mp[n].nsplit, S.mp[n].rows_pad, P.Rp, In, R, P.Vinv[n], P.U[n],
-                                     P.panel[n], pscale, P.ip_part, P.gram_part, skip ? &P.ls->skip : nullptr);
+        // with the shared pass, A is kept BF16-exact (the pass multiplies by it in BF16)
+        const int quant = (S.dt.on && n == 0) ? 1 : 0;
+        k_solve<<<nblk, 256, 0, s>>>(P.partial, nsplit, rows_pad, P.Rp, In, R, P.Vinv[n], P.U[n], P.panel[n], pscale,
+                                     P.ip_part, P.gram_part, skip ? &P.ls->skip : nullptr, quant);
         k_gram_psum<<<(int)ceil_div((long)R * R, 256L), 256, 0, s>>>(P.gram_part, nblk, R * R, P.G[n]);
         TDX_LAUNCH_CHECK();
         if (fit && n == P.order - 1) {
@@ -896,6 +950,7 @@ void tdx_cp_default_options(tdx_cp_options *o) {
     o->line_search = 0;
     o->use_graph = 1;
     o->mlp_width = 64;
+    o->shared_pass = 1;
 }
 
 int tdx_cp_als_ex(const float *X, int order, const long *dims, long rank, float *const *factors,
@@ -934,6 +989,7 @@ int tdx_cp_als_ex(const float *X, int order, const long *dims, long rank, float 
     const int total = opt.max_sweeps + (opt.storage == TDX_F32 ? 0 : opt.refine_sweeps);
     if (opt.line_search < 0 || opt.line_search > 3) return record(TDX_ERR_ARG);
     P.ls_mode = opt.line_search;
+    P.shared_pass = opt.shared_pass != 0;
     if (P.ls_mode == 1) {
         if (!opt.mlp_params || opt.mlp_width <= 0 || opt.mlp_width > 128) return record(TDX_ERR_ARG);
         P.mlp = opt.mlp_params;

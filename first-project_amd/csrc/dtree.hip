@@ -1,0 +1,488 @@
+// dtree.hip -- the shared pass of an order-3 CP-ALS sweep (dimension tree), gfx950.
+//
+// Within a sweep (modes 0, 1, 2 in order) the mode-1 MTTKRP uses (A_new, C) and the mode-2
+// MTTKRP uses (A_new, B_new).  Both contract X with A_new over i0 first:
+//   T[i2][i1][r] = sum_{i0} X[i0, i1, i2] A_new[i0][r]
+//   M1[i1][r]    = sum_{i2} T[i2][i1][r] C[i2][r]          (C is current during mode 1)
+//   M2[i2][r]    = sum_{i1} T[i2][i1][r] B_new[i1][r]
+// so one pass over X (k_ttm_m1: T and the mode-1 partials in the same pass) and one pass over
+// T (k_m2, I1 I2 Rp floats instead of I0 I1 I2 elements) replace two passes over X.  This is
+// the dimension-tree MTTKRP of Phan, Tichavsky and Cichocki ("Fast alternating LS
+// algorithms for high order CANDECOMP/PARAFAC tensor factorizations", IEEE TSP 61(19),
+// 2013), restricted to order 3.  The arithmetic is that of the fused MTTKRP reassociated:
+// the reference computes X(n) (KR product) per mode (src/cp_als.cu:46-52, 66-69, 85-88).
+//
+// k_ttm_m1 (BF16 or FP8 storage):
+//   * a workgroup (4 waves) owns 128 consecutive i1 and a range of i2; A_new (the mode-0
+//     panel, pre-scaled by the FP8 slice scales) is converted once into BF16 MFMA operands
+//     in LDS (I0 Rp 2 bytes, 128 KB at I0 = 1024, Rp = 64), in the exact per-lane order the
+//     MFMA reads (one conflict-free 16-byte read per lane per operand);
+//   * each wave computes T^T tiles (Rp x 32 columns i1) with v_mfma_f32_16x16x32_bf16: the
+//     A operand is A_new^T from LDS, the B operand X read straight from HBM into registers.
+//     The contraction order over i0 is free, so each lane owns 32 contiguous bytes of a
+//     column per "run" (two k-steps of BF16, four of FP8; FP8 widened exactly to BF16 with
+//     v_cvt_scalef32_pk_bf16_fp8) and a wave reads whole 128-byte lines of 16 columns, 64
+//     bytes of each per instruction (this operand-order load streams at ~4.3 TB/s against
+//     ~6.2-6.7 for 128-512 contiguous bytes per column per instruction);
+//   * chunks of 4 (BF16) / 8 (FP8) k-steps (8 KB of X per wave) rotate through four
+//     register buffers across the i2 loop, three chunks in flight while one is multiplied
+//     (nontemporal loads; every load is issued unconditionally, the chunks past the end
+//     re-read a valid one), and after the last chunk of a column i2 the tile is multiplied
+//     into the mode-1 partial by C[i2][:] (from LDS) and stored to T (16-byte stores, r
+//     fastest) after the next column's first chunk;
+//   * the mode-1 partial of the workgroup's i2 range goes to partial[split][i1][Rp], the
+//     layout k_solve sums (in split order, FP64).
+// k_m2: lane (i2, 4 consecutive r) walks an i1 range of T rows (16-byte loads, a T row is
+//   Rp*4 contiguous bytes) against B_new from its panel (8 rows per 32-byte load), FP32
+//   FMAs in i1 order; partial[split][i2][Rp].
+#include "mttkrp.hpp"
+
+namespace tdx {
+
+typedef float dt_f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 dt_bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 dt_bf16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t dt_pack(float a, float b) {
+    dt_bf16x2 v;
+    v.x = (__bf16)a;  // round to nearest even (v_cvt_pk_bf16_f32)
+    v.y = (__bf16)b;
+    return __builtin_bit_cast(uint32_t, v);
+}
+
+// 4 FP8 E4M3 codes -> 4 BF16, exact (scale 1)
+__device__ __forceinline__ uint2 dt_fp8x4(uint32_t q) {
+    return uint2{__builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_scalef32_pk_bf16_fp8(q, 1.0f, false)),
+                 __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_scalef32_pk_bf16_fp8(q, 1.0f, true))};
+}
+
+// ES: bytes per element; KPL: k-steps (32 i0 each) per 32-byte lane run; CH: k-steps per chunk
+// (two runs: 8 KB of X per wave)
+template <int ST> struct DtTile;
+template <> struct DtTile<1> { static constexpr int ES = 2, KPL = 2, CH = 4; };
+template <> struct DtTile<2> { static constexpr int ES = 1, KPL = 4, CH = 8; };
+
+struct DtArgs {
+    const void *X;
+    const float *panelA;  // mode-0 panel (FP8: scaled by the slice scales)
+    const float *C;       // mode-2 factor, column-major I2 x R
+    float *T;             // I2 x I1 x Rp, r fastest
+    float *partial;       // nsplit1 x I1 x Rp
+    long I0, I1, I2;
+    int Rp, R;
+    int i2per, blocks_i1;
+};
+
+// LDS: A operands ((I0/32) NRB 1 KB) and the C block (i2per Rp floats)
+inline size_t dtree_lds_bytes(long I0, int nrb, int i2per, int Rp) {
+    return (size_t)(I0 / 32) * nrb * 1024 + (size_t)i2per * Rp * 4;
+}
+
+template <int ST, int NRB, int NCH>
+__global__ __launch_bounds__(256, 1) void k_ttm_m1(DtArgs a) {
+    constexpr int ES = DtTile<ST>::ES, KPL = DtTile<ST>::KPL, CH = DtTile<ST>::CH, RUNS = CH / KPL;
+    constexpr int NB = NCH < 4 ? NCH : 4;  // chunk buffers; NCH % NB == 0
+    static_assert(NCH >= 2 && NCH % NB == 0, "NCH: chunks per column");
+    extern __shared__ __align__(16) unsigned char dsm[];
+    uint4 *aop = reinterpret_cast<uint4 *>(dsm);  // (I0/32) * NRB * 64 operands of 16 bytes
+    const long nks = a.I0 / 32;
+    float *cs = reinterpret_cast<float *>(dsm + nks * NRB * 1024);  // i2per x Rp
+    const int tid = threadIdx.x, lane = tid & 63, lm = lane & 15, g = lane >> 4;
+    const int wave = tid >> 6;
+    const int bi = blockIdx.x % a.blocks_i1, split = blockIdx.x / a.blocks_i1;
+    const long i2s = (long)split * a.i2per;
+    const long i2e = (i2s + a.i2per < a.I2) ? i2s + a.i2per : a.I2;
+    const int len = (int)(i2e - i2s);  // >= 1: the grid has ceil(I2 / i2per) splits
+    // A operand (k-step ks, row block rb, lane l): rows r = 16 rb + l % 16, the 8 i0 of lane
+    // group l / 16 within the run ks / KPL, step ks % KPL
+    // (nent is a multiple of 256; 4 entries per thread in flight, the ones past nent repeat
+    // the last entry's loads and are not stored)
+    const long nent = nks * NRB * 64;
+    for (long e0 = tid; e0 < nent; e0 += 4 * 256) {
+        float4 p[4][2];
+        long dst[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const long e = (e0 + 256L * u < nent) ? e0 + 256L * u : nent - 1;
+            const int l = (int)(e & 63);
+            const long t = e >> 6;
+            const int rb = (int)(t % NRB);
+            const long ks = t / NRB;
+            const int r = 16 * rb + (l & 15), gg = l >> 4;
+            const long i0 = 32L * KPL * (ks / KPL) + 8L * KPL * gg + 8L * (ks % KPL);
+            const float4 *pp = reinterpret_cast<const float4 *>(a.panelA + (((i0 >> 3) * a.Rp + r) << 3));
+            p[u][0] = pp[0];
+            p[u][1] = pp[1];
+            dst[u] = e;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (e0 + 256L * u < nent)
+                aop[dst[u]] = uint4{dt_pack(p[u][0].x, p[u][0].y), dt_pack(p[u][0].z, p[u][0].w),
+                                    dt_pack(p[u][1].x, p[u][1].y), dt_pack(p[u][1].z, p[u][1].w)};
+    }
+    // C block: consecutive threads read consecutive i2 of one column (coalesced)
+    for (int e = tid; e < len * a.Rp; e += 256) {
+        const int r = e / len, il = e % len;
+        cs[il * a.Rp + r] = r < a.R ? a.C[(i2s + il) + (long)r * a.I2] : 0.0f;
+    }
+    __syncthreads();  // the only barrier
+    const long i1w = (long)bi * 128 + 32 * wave;  // first column of this wave's 32
+    if (i1w >= a.I1) return;                      // I1 % 32 == 0: whole tiles only
+
+    const unsigned char *X = static_cast<const unsigned char *>(a.X);
+    const long colb = a.I0 * ES, slabb = a.I1 * colb;  // bytes per column i1, per slab i2
+    constexpr long chb = (long)RUNS * 128;             // bytes of one chunk of a column
+    const long total = (long)len * NCH;
+    const unsigned char *xl[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) xl[c] = X + (i2s * a.I1 + i1w + 16 * c + lm) * colb + 8L * KPL * g * ES;
+    const long last_off = (long)(len - 1) * slabb;  // re-read target past the end (valid)
+
+    dt_f32x4 acc[NRB][2], m1[NRB][2];
+#pragma unroll
+    for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            acc[rb][c] = dt_f32x4{0.f, 0.f, 0.f, 0.f};
+            m1[rb][c] = dt_f32x4{0.f, 0.f, 0.f, 0.f};
+        }
+
+    typedef uint4 Buf[2][RUNS][2];
+    auto issue = [&](Buf &b, long off) {
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int j = 0; j < RUNS; ++j) {
+                const uint4 *p = reinterpret_cast<const uint4 *>(xl[c] + off + j * 128);
+                b[c][j][0] = ld_stream(p);
+                b[c][j][1] = ld_stream(p + 1);
+            }
+    };
+    auto compute = [&](const Buf &b, int kc, int zoff) {
+        const uint4 *ab = aop + zoff + kc * (CH * NRB * 64) + lane;
+#pragma unroll
+        for (int j = 0; j < RUNS; ++j)
+#pragma unroll
+            for (int h = 0; h < KPL; ++h) {
+                const int ksl = j * KPL + h;  // k-step within the chunk
+                dt_bf16x8 av[NRB], bv[2];
+#pragma unroll
+                for (int rb = 0; rb < NRB; ++rb) av[rb] = __builtin_bit_cast(dt_bf16x8, ab[(ksl * NRB + rb) * 64]);
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    if constexpr (ST == 1) {
+                        bv[c] = __builtin_bit_cast(dt_bf16x8, b[c][j][h]);
+                    } else {
+                        const uint4 &w = b[c][j][h >> 1];
+                        const uint32_t q0 = (h & 1) ? w.z : w.x, q1 = (h & 1) ? w.w : w.y;
+                        const uint2 lo = dt_fp8x4(q0), hi = dt_fp8x4(q1);
+                        bv[c] = __builtin_bit_cast(dt_bf16x8, uint4{lo.x, lo.y, hi.x, hi.y});
+                    }
+                }
+#pragma unroll
+                for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+                    for (int c = 0; c < 2; ++c)
+                        acc[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[rb], bv[c], acc[rb][c], 0, 0, 0);
+            }
+    };
+    // finished tile (rows r = 16 rb + 4 g + v, columns i1 = i1w + 16 c + lm): into the mode-1
+    // partial now, saved for its T store
+    dt_f32x4 tsv[NRB][2];
+    auto take = [&](long i2l) {
+        const float *cp = cs + i2l * a.Rp + 4 * g;
+#pragma unroll
+        for (int rb = 0; rb < NRB; ++rb) {
+            const dt_f32x4 cv = *reinterpret_cast<const dt_f32x4 *>(cp + 16 * rb);
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                tsv[rb][c] = acc[rb][c];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) m1[rb][c][v] = fmaf(acc[rb][c][v], cv[v], m1[rb][c][v]);
+                acc[rb][c] = dt_f32x4{0.f, 0.f, 0.f, 0.f};
+            }
+        }
+    };
+    auto store_t = [&](long i2l) {
+        float *tp = a.T + ((i2s + i2l) * a.I1 + i1w + lm) * a.Rp + 4 * g;
+#pragma unroll
+        for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) *reinterpret_cast<dt_f32x4 *>(tp + 16L * c * a.Rp + 16 * rb) = tsv[rb][c];
+    };
+
+    // The chunk sequence runs column after column (NCH chunks each); chunk q lives in buffer
+    // q % NB, and the loads of chunks q+1 .. q+NB-1 are in flight while chunk q is multiplied.
+    // Columns are unrolled over their chunks (the loop body has no branches) and a column's
+    // T store is issued after the next column's first chunk, so on every path into the loop
+    // the stores are older than the loads in flight and the compiler's load-counter waits
+    // keep NB-1 chunks outstanding (stores issued after the prefetch made them drain it at
+    // every column).  Each wave walks a column's chunks from its own rotation (the order of
+    // the sum over i0 is free).  Chunks past the end re-read the last column's first chunk
+    // (a valid address, never multiplied).
+    const int rot = (int)((blockIdx.x * 4u + (unsigned)wave) & (NCH - 1));
+    long qi = 0, i2_i = 0;
+    int kc_i = 0;
+    auto next_off = [&]() {
+        const long o = qi < total ? i2_i * slabb + (long)((kc_i + rot) & (NCH - 1)) * chb : last_off;
+        ++qi;
+        const bool w = kc_i == NCH - 1;
+        i2_i += w ? 1 : 0;
+        kc_i = w ? 0 : kc_i + 1;
+        return o;
+    };
+    Buf b[NB];
+#pragma unroll
+    for (int u = 0; u < NB - 1; ++u) issue(b[u], next_off());
+#pragma unroll
+    for (int kc = 0; kc < NCH; ++kc) {  // column 0
+        issue(b[(kc + NB - 1) % NB], next_off());
+        __builtin_amdgcn_sched_barrier(0);
+        compute(b[kc % NB], (kc + rot) & (NCH - 1), 0);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    take(0);
+    for (int i2l = 1; i2l < len; ++i2l) {
+        // the A operands of a chunk are the same for every column: an opaque zero offset
+        // keeps their LDS reads inside the loop (hoisted, they held every chunk's operands
+        // in registers at once)
+        int zoff = 0;
+        asm volatile("" : "+s"(zoff));
+#pragma unroll
+        for (int kc = 0; kc < NCH; ++kc) {
+            issue(b[(kc + NB - 1) % NB], next_off());
+            __builtin_amdgcn_sched_barrier(0);  // the loads stay ahead of this chunk's MFMAs
+            compute(b[kc % NB], (kc + rot) & (NCH - 1), zoff);
+            __builtin_amdgcn_sched_barrier(0);
+            if (kc == 0) {
+                store_t(i2l - 1);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        take(i2l);
+    }
+    store_t(len - 1);
+    // mode-1 partial of this split
+#pragma unroll
+    for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+            *reinterpret_cast<dt_f32x4 *>(a.partial + ((long)split * a.I1 + i1w + 16 * c + lm) * a.Rp + 16 * rb +
+                                          4 * g) = m1[rb][c];
+}
+
+// M2 partials: lane (i2 = GPW i2 per wave, r = 4 q .. 4 q + 3), i1 in [split i1per, ...)
+template <int NQ>
+__global__ __launch_bounds__(256) void k_m2(const float *__restrict__ T, const float *__restrict__ panelB,
+                                            float *__restrict__ partial, long I1, long I2, int Rp, int i1per,
+                                            int nb2) {
+    constexpr int GPW = 64 / NQ;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int q = lane % NQ, gi = lane / NQ;
+    const int split = blockIdx.x / nb2, b = blockIdx.x % nb2;
+    const long i2 = ((long)b * 4 + wave) * GPW + gi;
+    if (i2 >= I2) return;  // no barriers in this kernel
+    const long i1s = (long)split * i1per;
+    const long i1e = (i1s + i1per < I1) ? i1s + i1per : I1;  // multiples of 8
+    const float *tr = T + (i2 * I1) * Rp + 4 * q;
+    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+    for (long i1 = i1s; i1 < i1e; i1 += 8) {
+        float4 t[8], bb[4][2];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) t[j] = *reinterpret_cast<const float4 *>(tr + (i1 + j) * Rp);
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+            const float4 *pb = reinterpret_cast<const float4 *>(panelB + (((i1 >> 3) * Rp + 4 * q + rr) << 3));
+            bb[rr][0] = pb[0];
+            bb[rr][1] = pb[1];
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const float4 &x = t[j];
+            const float4 &b0 = bb[0][j >> 2], &b1 = bb[1][j >> 2], &b2 = bb[2][j >> 2], &b3 = bb[3][j >> 2];
+            const int jj = j & 3;
+            const float y0 = jj == 0 ? b0.x : jj == 1 ? b0.y : jj == 2 ? b0.z : b0.w;
+            const float y1 = jj == 0 ? b1.x : jj == 1 ? b1.y : jj == 2 ? b1.z : b1.w;
+            const float y2 = jj == 0 ? b2.x : jj == 1 ? b2.y : jj == 2 ? b2.z : b2.w;
+            const float y3 = jj == 0 ? b3.x : jj == 1 ? b3.y : jj == 2 ? b3.z : b3.w;
+            acc[0] = fmaf(x.x, y0, acc[0]);
+            acc[1] = fmaf(x.y, y1, acc[1]);
+            acc[2] = fmaf(x.z, y2, acc[2]);
+            acc[3] = fmaf(x.w, y3, acc[3]);
+        }
+    }
+    *reinterpret_cast<float4 *>(partial + ((long)split * I2 + i2) * Rp + 4 * q) =
+        float4{acc[0], acc[1], acc[2], acc[3]};
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+// a == nullptr: set the kernel's dynamic LDS limit; otherwise launch it
+template <int ST, int NRB, int NCH>
+static int m1_kernel(const DtreePlan &d, const DtArgs *a, hipStream_t s) {
+    if (!a) {
+        TDX_HIP(hipFuncSetAttribute((const void *)k_ttm_m1<ST, NRB, NCH>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)d.lds1));
+        return TDX_OK;
+    }
+    k_ttm_m1<ST, NRB, NCH><<<d.blocks_i1 * d.nsplit1, 256, d.lds1, s>>>(*a);
+    return TDX_OK;
+}
+
+template <int ST, int NRB>
+static int m1_by_nch(const DtreePlan &d, const DtArgs *a, hipStream_t s) {
+    switch (d.nch) {
+        case 2: return m1_kernel<ST, NRB, 2>(d, a, s);
+        case 4: return m1_kernel<ST, NRB, 4>(d, a, s);
+        case 8: return m1_kernel<ST, NRB, 8>(d, a, s);
+        case 16: return m1_kernel<ST, NRB, 16>(d, a, s);
+        default: return record(TDX_ERR_UNSUPPORTED);
+    }
+}
+
+template <int ST>
+static int m1_by_nrb(const DtreePlan &d, const DtArgs *a, hipStream_t s) {
+    switch (d.nrb) {
+        case 1: return m1_by_nch<ST, 1>(d, a, s);
+        case 2: return m1_by_nch<ST, 2>(d, a, s);
+        case 4: return m1_by_nch<ST, 4>(d, a, s);
+        default: return record(TDX_ERR_UNSUPPORTED);
+    }
+}
+
+static int m1_call(const DtreePlan &d, const DtArgs *a, hipStream_t s) {
+    return d.st == 1 ? m1_by_nrb<1>(d, a, s) : m1_by_nrb<2>(d, a, s);
+}
+
+int dtree_plan(DtreePlan &d, int order, const long *dims, int dtype, long R, bool enabled) {
+    d = DtreePlan();
+    if (!enabled || order != 3 || R < 1 || R > 64) return TDX_OK;
+    if (dtype != TDX_BF16 && dtype != TDX_FP8_E4M3) return TDX_OK;
+    const int st = (dtype == TDX_BF16) ? 1 : 2;
+    const long CH = (st == 1) ? DtTile<1>::CH : DtTile<2>::CH;
+    d.st = st;
+    d.R = (int)R;
+    d.Rp = (int)(ceil_div(R, 16) * 16);
+    d.nrb = d.Rp / 16;
+    if (d.Rp == 48) return TDX_OK;  // k_m2 maps 4 columns per lane onto 64 lanes: Rp / 4 | 64
+    d.I0 = dims[0];
+    d.I1 = dims[1];
+    d.I2 = dims[2];
+    if (d.I0 % (32 * CH) != 0 || d.I1 % 32 != 0 || d.I2 < 1) return TDX_OK;
+    d.nch = (int)(d.I0 / (32 * CH));  // chunks per column: 2, 4, 8 or 16
+    if (d.nch != 2 && d.nch != 4 && d.nch != 8 && d.nch != 16) return TDX_OK;
+    const long cus = device_cus();
+    d.blocks_i1 = (int)ceil_div(d.I1, 128);
+    long ns = cus / d.blocks_i1;
+    if (ns < 1) ns = 1;
+    if (ns > d.I2) ns = d.I2;
+    d.i2per = (int)ceil_div(d.I2, ns);
+    d.nsplit1 = (int)ceil_div(d.I2, (long)d.i2per);
+    d.lds1 = dtree_lds_bytes(d.I0, d.nrb, d.i2per, d.Rp);
+    if (d.lds1 > 160 * 1024) return TDX_OK;  // LDS per workgroup on gfx950
+    // k_m2: about 16 waves per CU over (I2 / GPW) i2 groups x nsplit2 i1 ranges
+    const long gpw = 64 / (d.Rp / 4);
+    const long groups = ceil_div(d.I2, gpw);
+    long n2 = ceil_div(16 * cus, groups);
+    if (n2 > d.I1 / 8) n2 = d.I1 / 8;
+    if (n2 < 1) n2 = 1;
+    d.i1per = (int)(8 * ceil_div(d.I1 / 8, n2));
+    d.nsplit2 = (int)ceil_div(d.I1, (long)d.i1per);
+    d.nb2 = (int)ceil_div(d.I2, 4 * gpw);
+    const int rc = m1_call(d, nullptr, nullptr);
+    if (rc) return rc;
+    d.on = true;
+    return TDX_OK;
+}
+
+int dtree_m1_launch(const DtreePlan &d, const void *X, const float *panelA, const float *C, float *T, float *partial,
+                    hipStream_t s) {
+    if (!d.on) return record(TDX_ERR_UNSUPPORTED);
+    if ((reinterpret_cast<uintptr_t>(X) & 15) != 0) return record(TDX_ERR_ARG);
+    DtArgs a;
+    a.X = X;
+    a.panelA = panelA;
+    a.C = C;
+    a.T = T;
+    a.partial = partial;
+    a.I0 = d.I0;
+    a.I1 = d.I1;
+    a.I2 = d.I2;
+    a.Rp = d.Rp;
+    a.R = d.R;
+    a.i2per = d.i2per;
+    a.blocks_i1 = d.blocks_i1;
+    const int rc = m1_call(d, &a, s);
+    if (rc) return rc;
+    TDX_LAUNCH_CHECK();
+    return TDX_OK;
+}
+
+int dtree_m2_launch(const DtreePlan &d, const float *T, const float *panelB, float *partial, hipStream_t s) {
+    if (!d.on) return record(TDX_ERR_UNSUPPORTED);
+    const int grid = d.nb2 * d.nsplit2;
+    const int nq = d.Rp / 4;
+    if (nq == 16) k_m2<16><<<grid, 256, 0, s>>>(T, panelB, partial, d.I1, d.I2, d.Rp, d.i1per, d.nb2);
+    else if (nq == 8) k_m2<8><<<grid, 256, 0, s>>>(T, panelB, partial, d.I1, d.I2, d.Rp, d.i1per, d.nb2);
+    else k_m2<4><<<grid, 256, 0, s>>>(T, panelB, partial, d.I1, d.I2, d.Rp, d.i1per, d.nb2);
+    TDX_LAUNCH_CHECK();
+    return TDX_OK;
+}
+
+}  // namespace tdx
+
+using namespace tdx;
+
+extern "C" {
+
+int tdx_mttkrp_dtree(const void *X, int dtype, const float *scales, const long *dims, const float *const *factors,
+                     long rank, float *M1, float *M2) {
+    if (!X || !dims || !factors || !factors[0] || !factors[1] || !factors[2] || !M1 || !M2)
+        return record(TDX_ERR_ARG);
+    if (dtype == TDX_FP8_E4M3 && !scales) return record(TDX_ERR_ARG);
+    DtreePlan d;
+    int st = dtree_plan(d, 3, dims, dtype, rank, true);
+    if (st) return st;
+    if (!d.on) return record(TDX_ERR_UNSUPPORTED);
+    hipStream_t s = stream();
+    float *panel[2] = {nullptr, nullptr};
+    for (int m = 0; m < 2; ++m) {
+        void *ws = nullptr;
+        st = workspace(WS_PANEL0 + m, sizeof(float) * panel_floats(dims[m], d.Rp), &ws);
+        if (st) return st;
+        st = pack_panel(factors[m], dims[m], rank, d.Rp, (dtype == TDX_FP8_E4M3 && m == 0) ? scales : nullptr,
+                        (float *)ws, s);
+        if (st) return record(st);
+        panel[m] = (float *)ws;
+    }
+    void *T = nullptr, *part = nullptr;
+    st = workspace(WS_MISC0, sizeof(float) * d.t_floats(), &T);
+    if (st) return st;
+    st = workspace(WS_PARTIAL, sizeof(float) * d.partial_floats(), &part);
+    if (st) return st;
+    st = dtree_m1_launch(d, X, panel[0], factors[2], (float *)T, (float *)part, s);
+    if (st) return st;
+    MttkrpPlan p1;
+    p1.order = 3;
+    for (int m = 0; m < 3; ++m) p1.dims[m] = dims[m];
+    p1.rank = rank;
+    p1.Rp = d.Rp;
+    p1.mode = 1;
+    p1.rows_pad = d.I1;
+    p1.nsplit = d.nsplit1;
+    st = mttkrp_reduce(p1, (const float *)part, M1, s);
+    if (st) return record(st);
+    st = dtree_m2_launch(d, (const float *)T, panel[1], (float *)part, s);
+    if (st) return st;
+    MttkrpPlan p2 = p1;
+    p2.mode = 2;
+    p2.rows_pad = d.I2;
+    p2.nsplit = d.nsplit2;
+    st = mttkrp_reduce(p2, (const float *)part, M2, s);
+    return record(st);
+}
+
+}  // extern "C"

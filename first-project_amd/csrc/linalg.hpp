@@ -93,6 +93,26 @@ __device__ inline void jacobi_eig_block(double *A, double *Q, int n, double *cs,
     }
 }
 
+// Vinv = pseudo-inverse of the symmetric R x R matrix in L (destroyed) from its Jacobi
+// eigendecomposition; eigenvalues below 1e-12 of the largest are dropped.  W receives the
+// eigenvectors (R*R doubles), cs needs 4*(R/2+1) doubles, sh blockDim doubles.
+__device__ inline void pinv_jacobi(double *L, double *W, double *Vinv, int R, double *cs, double *sh) {
+    const int tid = threadIdx.x, nt = blockDim.x, RR = R * R;
+    jacobi_eig_block(L, W, R, cs, sh);
+    double wmax = 0.0;
+    for (int a = 0; a < R; ++a) wmax = fmax(wmax, fabs(L[a + a * R]));
+    for (int e = tid; e < RR; e += nt) {
+        const int a = e % R, b = e / R;
+        double s = 0.0;
+        for (int k = 0; k < R; ++k) {
+            const double lam = L[k + k * R];
+            if (fabs(lam) > 1e-12 * wmax) s += W[a + k * R] * W[b + k * R] / lam;
+        }
+        Vinv[e] = s;
+    }
+    __syncthreads();
+}
+
 // Vinv = V^{-1} for a symmetric R x R V (all arrays column-major, R*R doubles, any memory
 // space visible to the whole workgroup; V is preserved unless L is V, in which case the
 // fallback below works in V).  If V is not numerically SPD (a
@@ -182,20 +202,68 @@ __device__ inline int block_spd_inverse(const double *V, double *L, double *W, d
     if (L != V)
         for (int e = tid; e < RR; e += nt) L[e] = V[e];
     __syncthreads();
-    jacobi_eig_block(L, W, R, cs, sh);
-    double wmax = 0.0;
-    for (int a = 0; a < R; ++a) wmax = fmax(wmax, fabs(L[a + a * R]));
-    for (int e = tid; e < RR; e += nt) {
-        const int a = e % R, b = e / R;
-        double s = 0.0;
-        for (int k = 0; k < R; ++k) {
-            const double lam = L[k + k * R];
-            if (fabs(lam) > 1e-12 * wmax) s += W[a + k * R] * W[b + k * R] / lam;
+    pinv_jacobi(L, W, Vinv, R, cs, sh);
+    return 1;
+}
+
+// Vinv = V^{-1} for a symmetric R x R V (R <= 4 NS <= 64) by the symmetric sweep operator
+// of block_spd_inverse (same pivots, same operations per element, so the same result bit
+// for bit), for a workgroup of exactly 256 threads.  vat(i, c) returns V[i][c] (read once
+// per element, at the start); diag[k] = V[k][k] (any memory visible to the workgroup).
+// Layout: lane i of wave w holds row i (rows >= R are zero) of the columns c = w + 4 s in
+// register slot s, so the slot holding pivot column k is slot k/4 of wave k%4 (a uniform
+// register index, s_set_gpr_idx) and only that wave stores the column (one 8-byte LDS write
+// per lane, a uniform branch).  The column
+// goes to an LDS half Wb[(k & 1) * 64 ..], permuted so that the values a wave needs (rows
+// w, w+4, ...) are contiguous: the entry of row c at (c % 4) * 16 + c / 4 (16-byte broadcast
+// reads).  One barrier per pivot (the halves alternate).  Returns 1 (every thread) when a
+// relative pivot is below 1e-13 or not finite; Vinv is then not written.  Wb: 128 doubles.
+template <int NS, typename VAt>
+__device__ inline int sweep_inverse_rows(VAt vat, const double *diag, double *Wb, double *Vinv, int R, int *fail) {
+    static_assert(NS == 4 || NS == 8 || NS == 16, "NS: slots per lane");
+    const int tid = threadIdx.x, i = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    double a[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const int c = w + 4 * s;
+        a[s] = (i < R && c < R) ? vat(i, c) : 0.0;
+    }
+    const int pi = (i & 3) * 16 + (i >> 2);  // this lane's row in the permuted column
+    if (tid == 0) *fail = 0;
+    __syncthreads();
+    int bad = 0;
+    for (int k = 0; k < 4 * NS; ++k) {
+        if (k >= R) break;
+        double *Wk = Wb + (k & 1) * 64;
+        if (w == (k & 3)) Wk[pi] = a[k >> 2];
+        __syncthreads();
+        const double d = Wk[(k & 3) * 16 + (k >> 2)];
+        if (!(d > 1e-13 * diag[k]) || !isfinite(d)) {
+            bad = 1;
+            break;  // uniform: every thread read the same pivot
         }
-        Vinv[e] = s;
+        const double dinv = 1.0 / d;
+        const double t = Wk[pi] * dinv;
+        const bool rk = i == k;
+        const double alpha = rk ? 0.0 : 1.0, beta = rk ? dinv : -t;
+        const double *wr = Wk + w * 16;  // A[w + 4 s][k], s = 0 .. NS-1
+#pragma unroll
+        for (int s = 0; s < NS; ++s) a[s] = fma(beta, wr[s], alpha * a[s]);
+        if (w == (k & 3)) a[k >> 2] = rk ? -dinv : t;
+    }
+    if (bad) {
+        if (tid == 0) *fail = 1;
+        __syncthreads();
+        return 1;
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const int c = w + 4 * s;
+        if (i < R && c < R) Vinv[i + c * R] = -a[s];
     }
     __syncthreads();
-    return 1;
+    return 0;
 }
 
 }  // namespace tdx

@@ -47,6 +47,31 @@ bool mttkrp_lds_covers(int order, int mode, long rank, long In, int dtype);
 int mttkrp_lds_launch(const MttkrpPlan &p, const void *X, const float *const *panels, const float *row_scale,
                       float *partial, hipStream_t s, const int *skip);
 
+// the shared pass of an order-3 CP-ALS sweep (dtree.hip): T = X x_0 A^T with the mode-1
+// partials in the same pass over X, then the mode-2 partials from T
+struct DtreePlan {
+    bool on = false;  // eligible (order 3, BF16/FP8 storage, I0 = 128 nch (BF16) / 256 nch (FP8)
+                      // with nch in {2, 4, 8, 16}, I1 % 32 == 0, Rp in {16, 32, 64}, A operands
+                      // fit in LDS)
+    int st = 0, nrb = 0, nch = 0, Rp = 0, R = 0;
+    long I0 = 0, I1 = 0, I2 = 0;
+    int blocks_i1 = 0, nsplit1 = 0, i2per = 0;  // k_ttm_m1 grid: blocks_i1 x nsplit1
+    size_t lds1 = 0;
+    int nsplit2 = 0, i1per = 0, nb2 = 0;        // k_m2 grid: nb2 x nsplit2
+    size_t t_floats() const { return (size_t)I1 * I2 * Rp; }
+    size_t partial_floats() const {
+        const size_t a = (size_t)nsplit1 * I1 * Rp, b = (size_t)nsplit2 * I2 * Rp;
+        return a > b ? a : b;
+    }
+};
+// fills d (d.on = false when not enabled or when the shape or storage is not covered)
+int dtree_plan(DtreePlan &d, int order, const long *dims, int dtype, long R, bool enabled);
+// partial[split][i1][Rp] (nsplit1 splits, rows_pad I1) and T; C column-major I2 x R
+int dtree_m1_launch(const DtreePlan &d, const void *X, const float *panelA, const float *C, float *T, float *partial,
+                    hipStream_t s);
+// partial[split][i2][Rp] (nsplit2 splits, rows_pad I2) from T and the mode-1 panel
+int dtree_m2_launch(const DtreePlan &d, const float *T, const float *panelB, float *partial, hipStream_t s);
+
 // panel packing (optionally scaling row i by scale[i])
 int pack_panel(const float *U, long I, long R, int Rp, const float *scale, float *panel, hipStream_t s);
 

@@ -43,7 +43,8 @@ class CpOptions(ctypes.Structure):
                 ("init", ctypes.c_int), ("seed", ctypes.c_uint64), ("tol", ctypes.c_double),
                 ("record_fit", ctypes.c_int), ("line_search", ctypes.c_int), ("use_graph", ctypes.c_int),
                 ("mlp_params", _P), ("mlp_width", ctypes.c_int), ("ngrid", ctypes.c_int),
-                ("grid", ctypes.c_double * 8), ("trace", ctypes.POINTER(CpTrace)), ("target_relerr", ctypes.c_double)]
+                ("grid", ctypes.c_double * 8), ("trace", ctypes.POINTER(CpTrace)), ("target_relerr", ctypes.c_double),
+                ("shared_pass", ctypes.c_int)]
 
 
 class CpReport(ctypes.Structure):
@@ -72,6 +73,7 @@ _SIGS = {
                       ctypes.c_int, _P],
     "tdx_bench_mttkrp": [_P, ctypes.c_int, _P, ctypes.c_int, ctypes.POINTER(_L), ctypes.POINTER(_P), _L,
                          ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)],
+    "tdx_mttkrp_dtree": [_P, ctypes.c_int, _P, ctypes.POINTER(_L), ctypes.POINTER(_P), _L, _P, _P],
     "tdx_ttm": [_P, _P, _P, _L, _L, _L, _L, _L],
     "tdx_ttm_t": [_P, ctypes.c_int, ctypes.POINTER(_L), ctypes.c_int, _P, _L, _P],
     "tdx_unfold_gram": [_P, ctypes.c_int, ctypes.POINTER(_L), ctypes.c_int, _P],
@@ -230,6 +232,11 @@ def mttkrp(X, left, right, XU, flag, m, n, k, r):
 def mttkrp_ex(X, dtype, scales, dims, factors, rank, mode, out):
     _call("tdx_mttkrp_ex", ptr(X), dtype, ptr(scales), len(dims), dims_arr(dims), ptr_arr(factors), rank, mode,
           ptr(out))
+
+
+def mttkrp_dtree(X, dtype, scales, dims, factors, rank, M1, M2):
+    """Order 3: M1 (mode 1) and M2 (mode 2) from the shared pass (T = X x_0 A^T)."""
+    _call("tdx_mttkrp_dtree", ptr(X), dtype, ptr(scales), dims_arr(dims), ptr_arr(factors), rank, ptr(M1), ptr(M2))
 
 
 def bench_mttkrp(X, dtype, scales, dims, factors, rank, mode, iters):

@@ -90,6 +90,15 @@ int tdx_mttkrp_ex(const void *X, int dtype, const float *scales, int order, cons
  * library stream (panels and workspace prepared once); *avg_ms = mean launch time. */
 int tdx_bench_mttkrp(const void *X, int dtype, const float *scales, int order, const long *dims,
                      const float *const *factors, long rank, int mode, int iters, float *avg_ms);
+/* new, no counterpart in the reference (which runs one MTTKRP per mode, src/cp_als.cu:52,69,88):
+ * the shared pass of an order-3 sweep.  M1 = X(1) (C (.) A) (I1 x rank) and M2 = X(2) (B (.) A)
+ * (I2 x rank), both column-major like tdx_mttkrp_ex modes 1 and 2, from one pass over X that
+ * writes T = X x_0 A^T and a pass over T.  BF16 or FP8-E4M3 storage (scales as for
+ * tdx_mttkrp_ex); requires I0 = 128 k (BF16) or 256 k (FP8) with k in {2, 4, 8, 16}, I1 % 32 == 0,
+ * rank <= 64
+ * with rank rounded up to 16 in {16, 32, 64}; otherwise TDX_ERR_UNSUPPORTED. */
+int tdx_mttkrp_dtree(const void *X, int dtype, const float *scales, const long *dims,
+                     const float *const *factors, long rank, float *M1, float *M2);
 /* ttm: src/ttm.cu:4-47 -- XU = X x_flag U with U (d x mode size) col-major;
  * outputs d x b x c, a x d x c or a x b x d.  FP32 in/out. */
 int tdx_ttm(const float *X, const float *U, float *XU, long flag, long a, long b, long c, long d);
@@ -166,6 +175,10 @@ typedef struct {
     double grid[8];
     struct tdx_cp_trace *trace; /* optional per-sweep host outputs */
     double target_relerr;    /* stop once the relative error is <= this (0: off) */
+    int shared_pass;         /* order 3, BF16/FP8 storage: the mode-1 and mode-2 MTTKRPs from one
+                                pass over X that also writes T = X x_0 A^T (I2 I1 Rp floats), then
+                                a pass over T (default 1; no counterpart in the reference, which
+                                runs one MTTKRP per mode); 0: one fused MTTKRP per mode */
 } tdx_cp_options;
 
 /* features of the extrapolation MLP, computed on the device after every sweep k:

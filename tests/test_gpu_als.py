@@ -75,6 +75,18 @@ def test_c1_plain_fp32_tracks_fp64_oracle():
     assert np.all(np.diff(fit) <= 1e-6)
 
 
+@pytest.mark.parametrize("R", [24, 40, 64])
+def test_larger_ranks_track_fp64_oracle(R):
+    """Ranks past 16 and 32 take the 8- and 16-slot variants of the FP64 inverse."""
+    dims, sweeps = (96, 80, 72), 12
+    X, _ = make_cp_tensor(dims, R, eta=1e-2, kind="normal")
+    U0 = init_factors(dims, R, seed=7)
+    st, rep, fit, U = run_gpu_als(X, dims, R, U0, sweeps)
+    assert st == tdx.TDX_OK and rep.sweeps_done == sweeps
+    _, _, dr_o, _ = O.cp_als(host(X), dims, U0, sweeps, direct=True)
+    assert np.max(np.abs(fit - dr_o)) <= 1e-4, np.max(np.abs(fit - dr_o))
+
+
 def test_fit_identity_vs_direct_residual_on_device():
     dims, R = (64, 48, 40), 8
     X, _ = make_cp_tensor(dims, R, eta=1e-3)
@@ -107,6 +119,50 @@ def test_mixed_precision_reaches_oracle(storage):
     _, _, dr_o, _ = O.cp_als(Xh, dims, U0, 27, direct=True)
     rel = O.cp_relerr(Xh, dims, [u.astype(np.float64) for u in U])
     assert abs(rel - dr_o[-1]) <= 1e-4, (rel, dr_o[-1])
+
+
+@pytest.mark.parametrize("storage,dims", [(tdx.TDX_BF16, (256, 96, 64)), (tdx.TDX_FP8_E4M3, (512, 64, 48))])
+@pytest.mark.parametrize("shared", [1, 0])
+def test_mixed_sweeps_reach_oracle_with_and_without_shared_pass(storage, dims, shared):
+    """Order 3, BF16/FP8 storage, modes 1 and 2 from the shared pass (T = X x_0 A^T, A kept
+    BF16-exact) or from one fused MTTKRP per mode: 25 sweeps + 2 FP32 refinement sweeps end
+    at the FP64 oracle's relative error to 1e-4."""
+    R = 16
+    X, _ = make_cp_tensor(dims, R, eta=1e-3, kind="normal")
+    U0 = init_factors(dims, R, seed=5)
+    Ud = [dev(u.ravel(order="F")) for u in U0]
+    o = tdx.default_cp_options()
+    o.max_sweeps, o.refine_sweeps, o.storage, o.init, o.record_fit = 25, 2, storage, 1, 1
+    o.shared_pass = shared
+    st, rep, fit, ms = tdx.cp_als_ex(X, dims, R, Ud, o)
+    assert st == tdx.TDX_OK and rep.sweeps_done == 27
+    U = [host(u).reshape((d, R), order="F") for u, d in zip(Ud, dims)]
+    Xh = host(X)
+    _, _, dr_o, _ = O.cp_als(Xh, dims, U0, 27, direct=True)
+    rel = O.cp_relerr(Xh, dims, [u.astype(np.float64) for u in U])
+    assert abs(rel - dr_o[-1]) <= 1e-4, (rel, dr_o[-1])
+
+
+def test_shared_pass_fast_fit_is_the_iterates_error():
+    """BF16 storage with the shared pass: A is BF16-exact, so the normal equations, the Grams
+    and the pass over X all describe the same iterate and the per-sweep fast-fit identity
+    (FP64) matches the direct relative error of the returned factors against the FP32 tensor
+    (FP64 oracle) to 5e-5, at a transient and on the plateau."""
+    dims, R = (512, 64, 48), 16
+    X, _ = make_cp_tensor(dims, R, eta=1e-3, kind="normal")
+    U0 = init_factors(dims, R, seed=5)
+    Xh = host(X)
+    for k in (4, 25):
+        Ud = [dev(u.ravel(order="F")) for u in U0]
+        o = tdx.default_cp_options()
+        o.max_sweeps, o.refine_sweeps, o.storage, o.init, o.record_fit = k, 0, tdx.TDX_BF16, 1, 1
+        st, rep, fit, ms = tdx.cp_als_ex(X, dims, R, Ud, o)
+        U = [host(u).reshape((d, R), order="F").astype(np.float64) for u, d in zip(Ud, dims)]
+        true = O.cp_relerr(Xh, dims, U)
+        assert abs(fit[-1] - true) <= 5e-5, (k, fit[-1], true)
+        # the factor A holds BF16 values only
+        a = U[0].astype(np.float32).ravel()
+        assert np.array_equal(a, (O.bf16_bits(a).astype(np.uint32) << 16).view(np.float32))
 
 
 def test_order4_cp_als():

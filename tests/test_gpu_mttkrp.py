@@ -141,3 +141,85 @@ def test_fp8_all_codes_decode():
         U[1] = U1
         got = run_mttkrp(q, tdx.TDX_FP8_E4M3, s, dims, U, 0)[:, 0]
         assert np.array_equal(got, O.e4m3_decode(X[j * I:(j + 1) * I])), j
+
+
+# ---------------------------------------------------------------------------
+# shared pass of an order-3 sweep: M1 and M2 from T = X x_0 A^T
+
+def run_dtree(Xdev, dtype, scales, dims, U):
+    R = U[0].shape[1]
+    Ud = [dev(u.ravel(order="F")) for u in U]
+    M1, M2 = empty(dims[1] * R), empty(dims[2] * R)
+    tdx.mttkrp_dtree(Xdev, dtype, scales, dims, Ud, R, M1, M2)
+    return (host(M1).reshape((dims[1], R), order="F"), host(M2).reshape((dims[2], R), order="F"))
+
+
+@pytest.mark.parametrize("dims,R", [((256, 64, 40), 16), ((256, 160, 9), 64), ((512, 32, 70), 24)])
+def test_shared_pass_integer_data_bit_exact(dims, R):
+    """Small integers: BF16 storage and BF16 A operands are exact, every partial sum is an
+    integer below 2^24, so M1 and M2 equal the oracle's MTTKRPs bit for bit."""
+    import torch
+    rng = np.random.default_rng(sum(dims) + 3 * R)
+    X = rng.integers(-3, 4, size=int(np.prod(dims))).astype(np.float32)
+    U = factors(rng, dims, R, "int")
+    q = torch.empty(X.size, dtype=torch.int16, device="cuda")
+    tdx.quantize_bf16(dev(X), q)
+    M1, M2 = run_dtree(q, tdx.TDX_BF16, None, dims, U)
+    U64 = [u.astype(np.float64) for u in U]
+    assert np.array_equal(M1, O.mttkrp(X, dims, U64, 1).astype(np.float32))
+    assert np.array_equal(M2, O.mttkrp(X, dims, U64, 2).astype(np.float32))
+
+
+@pytest.mark.parametrize("dims,R", [((256, 96, 48), 16), ((512, 64, 40), 32), ((256, 128, 24), 64), ((256, 32, 5), 10),
+                                    ((2048, 32, 6), 32)])
+def test_shared_pass_bf16_storage(dims, R):
+    """Within 3e-3 normwise of the oracle on the stored values (BF16 rounding of A), and
+    within the same bar of the fused per-mode kernel."""
+    import torch
+    rng = np.random.default_rng(len(dims) * 17 + R)
+    X = rng.standard_normal(int(np.prod(dims))).astype(np.float32)
+    U = factors(rng, dims, R)
+    q = torch.empty(X.size, dtype=torch.int16, device="cuda")
+    tdx.quantize_bf16(dev(X), q)
+    Xb = (O.bf16_bits(X).astype(np.uint32) << 16).view(np.float32)
+    M1, M2 = run_dtree(q, tdx.TDX_BF16, None, dims, U)
+    U64 = [u.astype(np.float64) for u in U]
+    for mode, got in ((1, M1), (2, M2)):
+        ref = O.mttkrp(Xb, dims, U64, mode)
+        assert relnorm(got, ref) <= 3e-3, (dims, R, mode, relnorm(got, ref))
+        assert relnorm(got, run_mttkrp(q, tdx.TDX_BF16, None, dims, U, mode)) <= 3e-3
+
+
+@pytest.mark.parametrize("dims,R", [((512, 64, 40), 16), ((512, 96, 33), 64), ((1024, 32, 24), 32)])
+def test_shared_pass_fp8_storage(dims, R):
+    import torch
+    rng = np.random.default_rng(31 + R)
+    X = rng.standard_normal(int(np.prod(dims))).astype(np.float32)
+    U = factors(rng, dims, R)
+    I = dims[0]
+    q = torch.empty(X.size, dtype=torch.uint8, device="cuda")
+    s = empty(I)
+    tdx.quantize_fp8_mode1(dev(X), q, s, I)
+    Qo, so = O.quantize_e4m3_mode1(X, I)
+    dec = O.e4m3_decode(Qo)
+    M1, M2 = run_dtree(q, tdx.TDX_FP8_E4M3, s, dims, U)
+    U64 = [u.astype(np.float64) for u in U]
+    for mode, got in ((1, M1), (2, M2)):
+        ref = O.mttkrp(dec, dims, U64, mode, scales=so)
+        assert relnorm(got, ref) <= 3e-3, (dims, R, mode, relnorm(got, ref))
+
+
+@pytest.mark.parametrize("dims,R,dtype", [((128, 64, 40), 16, 1), ((384, 64, 40), 16, 1), ((256, 48, 40), 16, 1),
+                                          ((256, 64, 40), 16, 2), ((256, 64, 40), 48, 1), ((256, 64, 40), 80, 1)])
+def test_shared_pass_rejects_uncovered_shapes(dims, R, dtype):
+    """I0 not 128 k (BF16) / 256 k (FP8) with k in {2, 4, 8, 16}, I1 not a multiple of 32,
+    padded rank 48 or rank > 64: TDX_ERR_UNSUPPORTED, nothing launched."""
+    import torch
+    dt = tdx.TDX_BF16 if dtype == 1 else tdx.TDX_FP8_E4M3
+    n = int(np.prod(dims))
+    q = torch.zeros(n, dtype=torch.int16 if dtype == 1 else torch.uint8, device="cuda")
+    s = dev(np.ones(dims[0], np.float32)) if dtype == 2 else None
+    U = [np.ones((d, R), np.float32) for d in dims]
+    with pytest.raises(tdx.TdxError) as e:
+        run_dtree(q, dt, s, dims, U)
+    assert e.value.status == tdx.TDX_ERR_UNSUPPORTED
