@@ -11,7 +11,9 @@ device and is resident in HBM before any timed region.
  * mixed path (the value): BF16 tensor on MFMA with FP32 accumulation for --steps sweeps,
    then 2 FP32 refinement sweeps.  value = median over the init seeds of the device seconds
    until its relative error is within 1e-4 of the reference.
- * MTTKRP per mode for FP32 / BF16 / FP8-E4M3 storage (hipEvents on the library stream).
+ * MTTKRP per mode for FP32 / BF16 / FP8-E4M3 storage, and the two kernels of the shared
+   pass that the BF16/FP8 sweep uses for modes 1 and 2 (hipEvents on the library stream).
+   roofline: the longest kernel of the BF16 sweep, the shared pass over X.
  * cpu_baseline: the oracle's FP64 ALS (OpenMP) on the same tensor, one sweep timed on the
    host cores and extrapolated to the plain path's sweep count to target; plus the 128^3
    rank-16 configuration in full on the oracle beside the GPU (times and per-sweep errors).
@@ -33,11 +35,21 @@ HBM_PEAK_GBS = 8000.0
 # `bench.py --kernels-only` (FETCH_SIZE and WRITE_SIZE, each scaled by its reading for a
 # plain 1 GiB copy); counters cannot be read in the timed run itself
 MTTKRP_BF16_TRAFFIC = 2171913077
+# the same for the shared pass over X (BF16 storage: X read, T written)
+SHARED_PASS_BF16_TRAFFIC = 2530917267
 TARGET_DELTA = 1e-4
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def shared_pass_bytes(s_bytes, N, n, R):
+    """Algorithmic HBM bytes of the shared pass at n^3, rank R: the pass over X reads X (s N
+    bytes) and A, C and writes T (n^2 Rp FP32); the pass over T reads T and B."""
+    Rp = -(-R // 16) * 16
+    t_bytes = 4 * n * n * Rp
+    return s_bytes * N + t_bytes + 2 * 4 * n * R, t_bytes + 4 * n * R
 
 
 def kernels_only(args, X, dims, R, U):
@@ -58,6 +70,9 @@ def kernels_only(args, X, dims, R, U):
                                  ("fp8", Xq, tdx.TDX_FP8_E4M3, sc)):
         res[name] = [round(tdx.bench_mttkrp(Xs, dt, scales, dims, U, R, mode, args.mttkrp_iters), 4)
                      for mode in range(3)]
+        if dt != tdx.TDX_F32:
+            res[name + "_shared_pass"] = [round(v, 4) for v in
+                                          tdx.bench_mttkrp_dtree(Xs, dt, scales, dims, U, R, args.mttkrp_iters)]
     src = X[: 1 << 28]
     dst = torch.empty(1 << 28, dtype=torch.float32, device=X.device)
     for _ in range(3):
@@ -271,6 +286,17 @@ def main():
         mt[name] = {"ms_per_mode": [round(x, 4) for x in per], "GBps": round(alg_bytes / (avg * 1e-3) / 1e9, 1),
                     "TFLOPs": round(flops / (avg * 1e-3) / 1e12, 2),
                     "hbm_frac": round(alg_bytes / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        if dt != tdx.TDX_F32:
+            # shared pass of the order-3 sweep (modes 1 and 2): the pass over X writes T = X x_0 A^T
+            # (n^2 Rp FP32) and the mode-1 partials, the pass over T gives the mode-2 partials
+            ms_x, ms_t = tdx.bench_mttkrp_dtree(Xs, dt, scales, dims, U32, R, args.mttkrp_iters)
+            b_x, b_t = shared_pass_bytes(s_bytes, N, n, R)
+            if name == "bf16":
+                mt[name]["traffic_per_launch"] = MTTKRP_BF16_TRAFFIC
+            mt[name]["shared_pass"] = {
+                "ms_pass_x": round(ms_x, 4), "GBps_pass_x": round(b_x / (ms_x * 1e-3) / 1e9, 1),
+                "hbm_frac_pass_x": round(b_x / (ms_x * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "bytes_pass_x": b_x,
+                "ms_pass_t": round(ms_t, 4), "GBps_pass_t": round(b_t / (ms_t * 1e-3) / 1e9, 1), "bytes_pass_t": b_t}
     del Xb, Xq
 
     # --- CPU baseline: oracle FP64 ALS sweep on the same tensor ---
@@ -318,9 +344,12 @@ def main():
 
     extra = None if args.no_extra else extras(args, log)
 
+    # The BF16 sweep is: mode 0 by the fused MTTKRP, modes 1 and 2 by the shared pass; its
+    # longest kernel is the pass over X, the roofline kernel below.
     bf = mt["bf16"]
-    alg_b = 2 * N + 4 * R * 2 * n + 4 * n * R
-    avg_b_ms = float(np.mean(bf["ms_per_mode"]))
+    sp = bf["shared_pass"]
+    alg_m0 = 2 * N + 4 * R * 2 * n + 4 * n * R
+    sweep_bytes = alg_m0 + sp["bytes_pass_x"] + sp["bytes_pass_t"]
     out = {
         "metric": "CP-ALS sec to rel-err 1e-4 (mixed/learned vs plain FP32); MTTKRP HBM GB/s",
         "value": round(tbm, 5) if tbm is not None else None,
@@ -344,12 +373,17 @@ def main():
         "seeds_reaching_target": {k: sum(1 for r in per_seed if r[k]["s"] is not None) for k in ("fp32", "bf16", "fp8")},
         "per_seed": per_seed,
         "mttkrp": mt,
-        "roofline": {"bound": "hbm", "achieved": round(alg_b / (avg_b_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(alg_b / (avg_b_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                     "traffic": MTTKRP_BF16_TRAFFIC, "kernel": "k_mttkrp_lds BF16 storage (mean of the 3 modes)",
-                     "bytes_per_launch": alg_b,
-                     "traffic_source": "FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per launch (scaled by a 1 GiB copy), "
-                                       "mean of the 3 modes; profiles/mttkrp_1024cubed_r64.md"},
+        "sweep_hbm": {"bytes_per_bf16_sweep": sweep_bytes,
+                      "GBps": round(sweep_bytes / (msb_all * 1e-3) / 1e9, 1),
+                      "note": "algorithmic bytes of one BF16 sweep (fused mode-0 MTTKRP, shared pass over X, "
+                              "pass over T) / measured ms per sweep (includes solves, Grams, fit)"},
+        "roofline": {"bound": "hbm", "achieved": sp["GBps_pass_x"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": sp["hbm_frac_pass_x"],
+                     "traffic": SHARED_PASS_BF16_TRAFFIC,
+                     "kernel": "k_ttm_m1 BF16 storage (shared pass over X: T = X x_0 A^T and the mode-1 MTTKRP)",
+                     "bytes_per_launch": sp["bytes_pass_x"],
+                     "traffic_source": "FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per launch (scaled by a 1 GiB copy); "
+                                       "profiles/shared_pass_1024cubed_r64.md"},
         "cpu_baseline": cpu,
         "other_configs": extra,
     }

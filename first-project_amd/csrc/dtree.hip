@@ -485,4 +485,56 @@ int tdx_mttkrp_dtree(const void *X, int dtype, const float *scales, const long *
     return record(st);
 }
 
+// Times `iters` launches of each kernel of the shared pass (panels and workspace prepared
+// once, one warm-up launch each) with hipEvents on the library stream; *ms_m1 / *ms_m2 receive
+// the mean launch duration of k_ttm_m1 (the pass over X) and of k_m2 (the pass over T).
+int tdx_bench_mttkrp_dtree(const void *X, int dtype, const float *scales, const long *dims,
+                           const float *const *factors, long rank, int iters, float *ms_m1, float *ms_m2) {
+    if (!X || !dims || !factors || !factors[0] || !factors[1] || !factors[2] || !ms_m1 || !ms_m2 || iters <= 0)
+        return record(TDX_ERR_ARG);
+    if (dtype == TDX_FP8_E4M3 && !scales) return record(TDX_ERR_ARG);
+    DtreePlan d;
+    int st = dtree_plan(d, 3, dims, dtype, rank, true);
+    if (st) return st;
+    if (!d.on) return record(TDX_ERR_UNSUPPORTED);
+    hipStream_t s = stream();
+    float *panel[2] = {nullptr, nullptr};
+    for (int m = 0; m < 2; ++m) {
+        void *ws = nullptr;
+        st = workspace(WS_PANEL0 + m, sizeof(float) * panel_floats(dims[m], d.Rp), &ws);
+        if (st) return st;
+        st = pack_panel(factors[m], dims[m], rank, d.Rp, (dtype == TDX_FP8_E4M3 && m == 0) ? scales : nullptr,
+                        (float *)ws, s);
+        if (st) return record(st);
+        panel[m] = (float *)ws;
+    }
+    void *T = nullptr, *part = nullptr;
+    st = workspace(WS_MISC0, sizeof(float) * d.t_floats(), &T);
+    if (st) return st;
+    st = workspace(WS_PARTIAL, sizeof(float) * d.partial_floats(), &part);
+    if (st) return st;
+    auto chk = [](hipError_t e, const char *what) {
+        return e == hipSuccess ? TDX_OK : record_hip(e, what, __FILE__, __LINE__);
+    };
+    hipEvent_t e[3];
+    for (int i = 0; i < 3; ++i) TDX_HIP(hipEventCreate(&e[i]));
+    st = dtree_m1_launch(d, X, panel[0], factors[2], (float *)T, (float *)part, s);
+    if (!st) st = dtree_m2_launch(d, (const float *)T, panel[1], (float *)part, s);
+    if (!st) st = chk(hipEventRecord(e[0], s), "hipEventRecord");
+    for (int i = 0; i < iters && !st; ++i) st = dtree_m1_launch(d, X, panel[0], factors[2], (float *)T, (float *)part, s);
+    if (!st) st = chk(hipEventRecord(e[1], s), "hipEventRecord");
+    for (int i = 0; i < iters && !st; ++i) st = dtree_m2_launch(d, (const float *)T, panel[1], (float *)part, s);
+    if (!st) st = chk(hipEventRecord(e[2], s), "hipEventRecord");
+    if (!st) st = chk(hipEventSynchronize(e[2]), "hipEventSynchronize");
+    float t1 = 0.f, t2 = 0.f;
+    if (!st) {
+        (void)hipEventElapsedTime(&t1, e[0], e[1]);
+        (void)hipEventElapsedTime(&t2, e[1], e[2]);
+        *ms_m1 = t1 / iters;
+        *ms_m2 = t2 / iters;
+    }
+    for (int i = 0; i < 3; ++i) (void)hipEventDestroy(e[i]);
+    return record(st);
+}
+
 }  // extern "C"

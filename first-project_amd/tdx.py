@@ -74,6 +74,8 @@ _SIGS = {
     "tdx_bench_mttkrp": [_P, ctypes.c_int, _P, ctypes.c_int, ctypes.POINTER(_L), ctypes.POINTER(_P), _L,
                          ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)],
     "tdx_mttkrp_dtree": [_P, ctypes.c_int, _P, ctypes.POINTER(_L), ctypes.POINTER(_P), _L, _P, _P],
+    "tdx_bench_mttkrp_dtree": [_P, ctypes.c_int, _P, ctypes.POINTER(_L), ctypes.POINTER(_P), _L, ctypes.c_int,
+                               ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)],
     "tdx_ttm": [_P, _P, _P, _L, _L, _L, _L, _L],
     "tdx_ttm_t": [_P, ctypes.c_int, ctypes.POINTER(_L), ctypes.c_int, _P, _L, _P],
     "tdx_unfold_gram": [_P, ctypes.c_int, ctypes.POINTER(_L), ctypes.c_int, _P],
@@ -237,6 +239,14 @@ def mttkrp_ex(X, dtype, scales, dims, factors, rank, mode, out):
 def mttkrp_dtree(X, dtype, scales, dims, factors, rank, M1, M2):
     """Order 3: M1 (mode 1) and M2 (mode 2) from the shared pass (T = X x_0 A^T)."""
     _call("tdx_mttkrp_dtree", ptr(X), dtype, ptr(scales), dims_arr(dims), ptr_arr(factors), rank, ptr(M1), ptr(M2))
+
+
+def bench_mttkrp_dtree(X, dtype, scales, dims, factors, rank, iters):
+    """Mean launch ms of the shared pass's two kernels: (pass over X, pass over T)."""
+    m1, m2 = ctypes.c_float(), ctypes.c_float()
+    _call("tdx_bench_mttkrp_dtree", ptr(X), dtype, ptr(scales), dims_arr(dims), ptr_arr(factors), rank, iters,
+          ctypes.byref(m1), ctypes.byref(m2))
+    return m1.value, m2.value
 
 
 def bench_mttkrp(X, dtype, scales, dims, factors, rank, mode, iters):

@@ -99,6 +99,11 @@ int tdx_bench_mttkrp(const void *X, int dtype, const float *scales, int order, c
  * with rank rounded up to 16 in {16, 32, 64}; otherwise TDX_ERR_UNSUPPORTED. */
 int tdx_mttkrp_dtree(const void *X, int dtype, const float *scales, const long *dims,
                      const float *const *factors, long rank, float *M1, float *M2);
+/* new: mean launch duration (ms, hipEvents on the library stream, `iters` launches each) of the
+ * two kernels of the shared pass: the pass over X (T and the mode-1 partials) and the pass
+ * over T (the mode-2 partials) */
+int tdx_bench_mttkrp_dtree(const void *X, int dtype, const float *scales, const long *dims,
+                           const float *const *factors, long rank, int iters, float *ms_m1, float *ms_m2);
 /* ttm: src/ttm.cu:4-47 -- XU = X x_flag U with U (d x mode size) col-major;
  * outputs d x b x c, a x d x c or a x b x d.  FP32 in/out. */
 int tdx_ttm(const float *X, const float *U, float *XU, long flag, long a, long b, long c, long d);
