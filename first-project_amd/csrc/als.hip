@@ -11,9 +11,10 @@
 // FP32 LU (getrf/getrs) with a cudaMalloc per call, and a transpose (Sgeam).
 //
 // Kernels per mode: k_vinv (one workgroup, symmetric sweep-operator inverse, Jacobi
-// pseudo-inverse fallback), the fused MTTKRP, then k_solve (one workgroup per 16 rows: FP64
-// sum of the split-K partials, U = M V^{-1}, panel, fit and Gram partials), k_gram_psum
-// (Gram of the new factor) and, once per sweep, k_fit.
+// pseudo-inverse fallback), the fused MTTKRP (or the shared pass, order 3 BF16/FP8), then
+// k_solve (one workgroup per 16 rows: FP64 sum of the split-K partials, U = M V^{-1}, panel,
+// fit and Gram partials) and k_gram_psum (Gram of the new factor); after the last mode
+// k_gram_psum_fit (its Gram and the sweep's fit in one kernel).
 // A sweep is captured once into a hipGraph and replayed.
 #include "mttkrp.hpp"
 #include "linalg.hpp"
@@ -216,33 +217,63 @@ __global__ __launch_bounds__(256) void k_gram_psum(const double *__restrict__ gr
 // The fast-fit relative error of Tensorlab cpd_als.m:281-283,
 //   ||X - Xhat||^2 = ||X||^2 - 2 <M, U_n> + sum(hadamard of all Grams),
 // appended to fit_hist[*counter]; *counter is incremented.
-__global__ __launch_bounds__(256) void k_fit(const double *__restrict__ ip_part, int nblk, int R, const double *G0,
-                                             const double *G1, const double *G2, const double *G3, int order,
-                                             const double *__restrict__ normx2, double *__restrict__ fit_hist,
-                                             int *__restrict__ counter) {
+
+// Gram partials U_blk^T U_blk (FP64) of kGramRows-row blocks of a column-major I x R factor
+// The last mode's Gram (as k_gram_psum) fused with the sweep's fit: each block writes its
+// Gram entries and the partial sum of the Hadamard product of all Grams over them; the last
+// block to finish (atomic ticket, reset by that block) adds the partials in block order and
+// <M, U> (ip_part, in block order) and appends the relative error to fit_hist.
+__global__ __launch_bounds__(256) void k_gram_psum_fit(const double *__restrict__ gram_part, int nblk, int R,
+                                                       int order, int mode, double *Gout, const double *G0,
+                                                       const double *G1, const double *G2, const double *G3,
+                                                       const double *__restrict__ ip_part, int nblk_ip,
+                                                       const double *__restrict__ normx2, double *__restrict__ fit_hist,
+                                                       int *__restrict__ counter, double *__restrict__ hpart,
+                                                       int *__restrict__ done) {
     __shared__ double sh[256];
-    const int tid = threadIdx.x, RR = R * R;
+    __shared__ int last;
+    const int RR = R * R, tid = threadIdx.x;
+    const int e = blockIdx.x * 256 + tid;
     const double *G[4] = {G0, G1, G2, G3};
-    double hx = 0.0;
-    for (int e = tid; e < RR; e += 256) {
-        double v = 1.0;
-        for (int m = 0; m < order; ++m) v *= G[m][e];
-        hx += v;
+    double h = 0.0;
+    if (e < RR) {
+        double s = 0.0;
+        int b = 0;
+        for (; b + 4 <= nblk; b += 4) {
+            double v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = gram_part[(long)(b + u) * RR + e];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) s += v[u];
+        }
+        for (; b < nblk; ++b) s += gram_part[(long)b * RR + e];
+        Gout[e] = s;
+        h = s;
+        for (int m = 0; m < order; ++m)
+            if (m != mode) h *= G[m][e];
     }
-    hx = block_sum(hx, sh);
-    double ip = 0.0;
-    for (int b = tid; b < nblk; b += 256) ip += ip_part[b];
-    ip = block_sum(ip, sh);
+    h = block_sum(h, sh);
     if (tid == 0) {
+        hpart[blockIdx.x] = h;
+        __threadfence();
+        last = (atomicAdd(done, 1) == (int)gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    if (tid == 0) {
+        __threadfence();
+        double hx = 0.0, ip = 0.0;
+        for (int b = 0; b < (int)gridDim.x; ++b) hx += hpart[b];
+        for (int b = 0; b < nblk_ip; ++b) ip += ip_part[b];
         const double nx2 = *normx2;
         const double r2 = nx2 - 2.0 * ip + hx;
         const int c = *counter;
         fit_hist[c] = sqrt(fabs(r2) / nx2);
         *counter = c + 1;
+        *done = 0;
     }
 }
 
-// Gram partials U_blk^T U_blk (FP64) of kGramRows-row blocks of a column-major I x R factor
 __global__ __launch_bounds__(256) void k_gram_rows(const float *__restrict__ U, long I, int R,
                                                    double *__restrict__ gram_part) {
     __shared__ double Us[kGramRows * kMaxR];
@@ -550,6 +581,8 @@ struct CpPlan {
     double *normx2 = nullptr;
     double *scratch = nullptr;
     double *vinv_ws = nullptr;  // k_vinv fallback workspace, 2 R^2 doubles
+    double *hpart = nullptr;    // k_gram_psum_fit: per-block partials (R^2 / 256 doubles)
+    int *fit_done = nullptr;    // k_gram_psum_fit: ticket (0 between launches)
     bool shared_pass = true;    // modes 1 and 2 from one pass over X where eligible
     float *T = nullptr;         // shared-pass intermediate X x_0 A^T (I2 x I1 x Rp)
     size_t t_floats = 0;
@@ -595,11 +628,13 @@ struct CpPlan {
             if (Vinv[m]) (void)hipFree(Vinv[m]);
             panel[m] = nullptr; G[m] = nullptr; Vinv[m] = nullptr;
         }
-        void *ps[] = {partial, msum, ip_part, gram_part, fit_hist, counter, status, normx2, scratch, vinv_ws, T};
+        void *ps[] = {partial, msum, ip_part, gram_part, fit_hist, counter, status, normx2, scratch, vinv_ws, T,
+                      hpart, fit_done};
         for (void *p : ps)
             if (p) (void)hipFree(p);
         partial = nullptr; msum = nullptr; ip_part = nullptr; gram_part = nullptr; fit_hist = nullptr;
         counter = nullptr; status = nullptr; normx2 = nullptr; scratch = nullptr; vinv_ws = nullptr; T = nullptr;
+        hpart = nullptr; fit_done = nullptr;
         t_floats = 0;
         if (events) {
             for (int m = 0; m < 4; ++m) {
@@ -644,6 +679,9 @@ static int plan_setup(CpPlan &P, int order, const long *dims, long R, float *con
     TDX_HIP(hipMalloc(&P.normx2, sizeof(double)));
     TDX_HIP(hipMalloc(&P.scratch, sizeof(double) * 4096));
     TDX_HIP(hipMalloc(&P.vinv_ws, sizeof(double) * 2 * R * R));
+    TDX_HIP(hipMalloc(&P.hpart, sizeof(double) * ceil_div(R * R, 256L)));
+    TDX_HIP(hipMalloc(&P.fit_done, sizeof(int)));
+    TDX_HIP(hipMemset(P.fit_done, 0, sizeof(int)));
     if (P.ls_mode) {
         auto alloc = [&](void **p, size_t bytes) -> int {
             TDX_HIP(hipMalloc(p, bytes));
@@ -806,13 +844,18 @@ static int enqueue_sweep(CpPlan &P, const Storage &S, hipStream_t s, bool fit) {
         const int quant = (S.dt.on && n == 0) ? 1 : 0;
         k_solve<<<nblk, 256, 0, s>>>(P.partial, nsplit, rows_pad, P.Rp, In, R, P.Vinv[n], P.U[n], P.panel[n], pscale,
                                      P.ip_part, P.gram_part, skip ? &P.ls->skip : nullptr, quant);
-        k_gram_psum<<<(int)ceil_div((long)R * R, 256L), 256, 0, s>>>(P.gram_part, nblk, R * R, P.G[n]);
-        TDX_LAUNCH_CHECK();
-        if (fit && n == P.order - 1) {
-            k_fit<<<1, 256, 0, s>>>(P.ip_part, nblk, R, P.G[0], P.G[1], P.order > 2 ? P.G[2] : P.G[0],
-                                    P.order > 3 ? P.G[3] : P.G[0], P.order, P.normx2, P.fit_hist, P.counter);
-            TDX_LAUNCH_CHECK();
+        // G_n; after the last mode together with the fit of the sweep
+        {
+            const int gb = (int)ceil_div((long)R * R, 256L);
+            if (fit && n == P.order - 1)
+                k_gram_psum_fit<<<gb, 256, 0, s>>>(P.gram_part, nblk, R, P.order, n, P.G[n], P.G[0], P.G[1],
+                                                   P.order > 2 ? P.G[2] : P.G[0], P.order > 3 ? P.G[3] : P.G[0],
+                                                   P.ip_part, nblk, P.normx2, P.fit_hist, P.counter, P.hpart,
+                                                   P.fit_done);
+            else
+                k_gram_psum<<<gb, 256, 0, s>>>(P.gram_part, nblk, R * R, P.G[n]);
         }
+        TDX_LAUNCH_CHECK();
     }
     if (P.ls_mode) return enqueue_ls(P, S, s);
     return TDX_OK;
