@@ -97,6 +97,25 @@ def test_fit_identity_vs_direct_residual_on_device():
     assert abs(direct - O.cp_relerr(host(X), dims, [u.astype(np.float64) for u in U])) < 1e-7
 
 
+@pytest.mark.parametrize("storage", [tdx.TDX_F32, tdx.TDX_BF16])
+def test_fit_recording_does_not_change_the_iterate(storage):
+    """With the fit off, the last mode's Gram comes from k_gram_psum instead of the kernel
+    that also appends the fit: the factors must be bit-identical either way."""
+    dims, R = (256, 64, 48), 16
+    X, _ = make_cp_tensor(dims, R, eta=1e-2, kind="normal")
+    U0 = init_factors(dims, R, seed=11)
+    out = []
+    for rec in (1, 0):
+        Ud = [dev(u.ravel(order="F")) for u in U0]
+        o = tdx.default_cp_options()
+        o.max_sweeps, o.refine_sweeps, o.storage, o.init, o.record_fit = 6, 1, storage, 1, rec
+        st, rep, fit, ms = tdx.cp_als_ex(X, dims, R, Ud, o)
+        assert st == tdx.TDX_OK
+        out.append([host(u) for u in Ud])
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+
+
 def test_graph_and_eager_identical():
     dims, R = (32, 40, 24), 6
     X, _ = make_cp_tensor(dims, R)
