@@ -101,25 +101,49 @@ __global__ __launch_bounds__(256) void k_solve(const float *__restrict__ partial
     const int tid = threadIdx.x;
     const long row0 = (long)blockIdx.x * kSolveRows;
     const int RR = R * R;
-    for (int e = tid; e < RR; e += 256) Vs[e] = Vinv[e];
-    const long pstride = rows_pad * Rp;
-    for (int e = tid; e < kSolveRows * Rp; e += 256) {
-        const int r = e / Rp, b = e % Rp;
+    // V^{-1} into registers (all of a thread's loads in flight at once), the split sums, then
+    // V^{-1} into LDS: the two read phases overlap instead of chaining their latencies
+    constexpr int kVPer = kMaxR * kMaxR / 256;
+    double vreg[kVPer];
+#pragma unroll
+    for (int k = 0; k < kVPer; ++k) {
+        const int e = tid + 256 * k;
+        vreg[k] = e < RR ? Vinv[e] : 0.0;
+    }
+    // thread (row r, columns b .. b + 3): one 16-byte load per split (Rp % 16 == 0, so
+    // kSolveRows Rp / 4 <= 256 threads cover the block), 16 splits in flight
+    const int q4 = Rp / 4;
+    if (tid < kSolveRows * q4) {
+        const int r = tid / q4, b = 4 * (tid % q4);
         const long i = row0 + r;
-        double acc = 0.0;
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
         if (i < In) {
-            const float *pp = partial + i * Rp + b;
-            int sp = 0;
-            for (; sp + 8 <= nsplit; sp += 8) {
-                float v[8];
+            const float4 *pp = reinterpret_cast<const float4 *>(partial + i * Rp + b);
+            const long ps4 = rows_pad * Rp / 4;
+            for (int sp = 0; sp < nsplit; sp += 16) {
+                float4 v[16];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) v[u] = pp[(long)(sp + u) * pstride];
+                for (int u = 0; u < 16; ++u) {
+                    const int sq = sp + u < nsplit ? sp + u : nsplit - 1;  // valid; not summed
+                    v[u] = pp[(long)sq * ps4];
+                }
 #pragma unroll
-                for (int u = 0; u < 8; ++u) acc += (double)v[u];
+                for (int u = 0; u < 16; ++u)
+                    if (sp + u < nsplit) {  // split order, as k_split_sum
+                        acc[0] += (double)v[u].x;
+                        acc[1] += (double)v[u].y;
+                        acc[2] += (double)v[u].z;
+                        acc[3] += (double)v[u].w;
+                    }
             }
-            for (; sp < nsplit; ++sp) acc += (double)pp[(long)sp * pstride];
         }
-        Ms[r * kMaxR + b] = acc;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) Ms[r * kMaxR + b + q] = acc[q];
+    }
+#pragma unroll
+    for (int k = 0; k < kVPer; ++k) {
+        const int e = tid + 256 * k;
+        if (e < RR) Vs[e] = vreg[k];
     }
     __syncthreads();
     // U = M V^{-1}: thread (row r, columns b0 .. b0 + 3)
