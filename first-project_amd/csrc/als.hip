@@ -220,22 +220,27 @@ __global__ __launch_bounds__(256) void k_solve(const float *__restrict__ partial
     }
 }
 
+// p[0] + p[stride] + ... + p[(n - 1) stride], added in index order, 16 loads in flight (the
+// ones past n re-read a valid entry and are not added)
+__device__ __forceinline__ double ordered_sum(const double *p, long stride, int n) {
+    double s = 0.0;
+    for (int b = 0; b < n; b += 16) {
+        double v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = p[(long)(b + u < n ? b + u : n - 1) * stride];
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+            if (b + u < n) s += v[u];
+    }
+    return s;
+}
+
 // G = sum of the block Gram partials, in block order; one thread per entry
 __global__ __launch_bounds__(256) void k_gram_psum(const double *__restrict__ gram_part, int nblk, int RR,
                                                    double *__restrict__ Gout) {
     const int e = blockIdx.x * 256 + threadIdx.x;
     if (e >= RR) return;
-    double s = 0.0;
-    int b = 0;
-    for (; b + 4 <= nblk; b += 4) {
-        double v[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) v[u] = gram_part[(long)(b + u) * RR + e];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) s += v[u];
-    }
-    for (; b < nblk; ++b) s += gram_part[(long)b * RR + e];
-    Gout[e] = s;
+    Gout[e] = ordered_sum(gram_part + e, RR, nblk);
 }
 
 // The fast-fit relative error of Tensorlab cpd_als.m:281-283,
@@ -261,16 +266,7 @@ __global__ __launch_bounds__(256) void k_gram_psum_fit(const double *__restrict_
     const double *G[4] = {G0, G1, G2, G3};
     double h = 0.0;
     if (e < RR) {
-        double s = 0.0;
-        int b = 0;
-        for (; b + 4 <= nblk; b += 4) {
-            double v[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) v[u] = gram_part[(long)(b + u) * RR + e];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) s += v[u];
-        }
-        for (; b < nblk; ++b) s += gram_part[(long)b * RR + e];
+        const double s = ordered_sum(gram_part + e, RR, nblk);
         Gout[e] = s;
         h = s;
         for (int m = 0; m < order; ++m)
@@ -286,9 +282,7 @@ __global__ __launch_bounds__(256) void k_gram_psum_fit(const double *__restrict_
     if (!last) return;
     if (tid == 0) {
         __threadfence();
-        double hx = 0.0, ip = 0.0;
-        for (int b = 0; b < (int)gridDim.x; ++b) hx += hpart[b];
-        for (int b = 0; b < nblk_ip; ++b) ip += ip_part[b];
+        const double hx = ordered_sum(hpart, 1, (int)gridDim.x), ip = ordered_sum(ip_part, 1, nblk_ip);
         const double nx2 = *normx2;
         const double r2 = nx2 - 2.0 * ip + hx;
         const int c = *counter;
