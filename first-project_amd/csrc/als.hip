@@ -818,8 +818,10 @@ static int enqueue_sweep(CpPlan &P, const Storage &S, hipStream_t s, bool fit) {
         // stream, since it would hold a CU the MTTKRP grid needs.
         // Shared pass (order 3, BF16/FP8): mode 1 from one pass over X that also writes T,
         // mode 2 from T.  The inverse (4 KB of LDS) fits beside either kernel's workgroups.
+        // The fork and join between the streams cost ~15 us of replay time per mode, more
+        // than the inverse itself at R <= 16 (~7 us), so small ranks keep it in line.
         const bool dt = S.dt.on && n > 0;
-        const bool beside = dt || S.mp[n].rowtiles * (long)S.mp[n].nsplit <= device_cus();
+        const bool beside = R > 16 && (dt || S.mp[n].rowtiles * (long)S.mp[n].nsplit <= device_cus());
         if (beside) {
             TDX_HIP(hipEventRecord(P.ev_fork[n], s));
             TDX_HIP(hipStreamWaitEvent(P.side, P.ev_fork[n], 0));
