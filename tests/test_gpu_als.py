@@ -116,12 +116,16 @@ def test_fit_recording_does_not_change_the_iterate(storage):
         assert np.array_equal(a, b)
 
 
-def test_graph_and_eager_identical():
-    dims, R = (32, 40, 24), 6
+@pytest.mark.parametrize("dims,R,storage", [((32, 40, 24), 6, tdx.TDX_F32), ((32, 40, 24), 24, tdx.TDX_F32),
+                                            ((256, 64, 40), 24, tdx.TDX_BF16)])
+def test_graph_and_eager_identical(dims, R, storage):
+    """Replayed graph and eager launches give identical fits and factors: rank <= 16 with the
+    inverse in line, rank 24 with the inverse on the side stream (FP32 per-mode MTTKRP, and
+    BF16 storage with the shared pass)."""
     X, _ = make_cp_tensor(dims, R)
     U0 = init_factors(dims, R, seed=4)
-    _, _, f1, U1 = run_gpu_als(X, dims, R, U0, 8, graph=True)
-    _, _, f2, U2 = run_gpu_als(X, dims, R, U0, 8, graph=False)
+    _, _, f1, U1 = run_gpu_als(X, dims, R, U0, 8, storage=storage, graph=True)
+    _, _, f2, U2 = run_gpu_als(X, dims, R, U0, 8, storage=storage, graph=False)
     assert np.array_equal(f1, f2)
     for a, b in zip(U1, U2):
         assert np.array_equal(a, b)
