@@ -811,33 +811,42 @@ static int enqueue_sweep(CpPlan &P, const Storage &S, hipStream_t s, bool fit) {
         k_copy_set<<<256, 256, 0, s>>>(P.set_of(P.U), P.set_of(P.U1));
         TDX_LAUNCH_CHECK();
     }
-    for (int n = 0; n < P.order; ++n) {
-        // V^{-1} (one workgroup) and the MTTKRP depend only on the previous mode's update.
-        // When the MTTKRP grid leaves a workgroup slot free on every CU (one workgroup per CU)
-        // the inverse runs beside it on the side stream; otherwise ahead of it on the same
-        // stream, since it would hold a CU the MTTKRP grid needs.
-        // Shared pass (order 3, BF16/FP8): mode 1 from one pass over X that also writes T,
-        // mode 2 from T.  The inverse (4 KB of LDS) fits beside either kernel's workgroups.
-        // The fork and join between the streams cost ~15 us of replay time per mode, more
-        // than the inverse itself at R <= 16 (~7 us), so small ranks keep it in line.
+    // V^{-1} (one workgroup) and the MTTKRP depend only on the previous mode's update.
+    // When the MTTKRP grid leaves a workgroup slot free on every CU (one workgroup per CU)
+    // the inverse runs beside it on the side stream; otherwise ahead of it on the same
+    // stream, since it would hold a CU the MTTKRP grid needs.
+    // Shared pass (order 3, BF16/FP8): mode 1 from one pass over X that also writes T,
+    // mode 2 from T.  The inverse (4 KB of LDS) fits beside either kernel's workgroups.
+    // The fork and join between the streams cost ~15 us of replay time per mode, more
+    // than the inverse itself at R <= 16 (~7 us), so small ranks keep it in line.
+    auto beside_of = [&](int n) {
         const bool dt = S.dt.on && n > 0;
-        const bool beside = R > 16 && (dt || S.mp[n].rowtiles * (long)S.mp[n].nsplit <= device_cus());
-        if (beside) {
-            TDX_HIP(hipEventRecord(P.ev_fork[n], s));
-            TDX_HIP(hipStreamWaitEvent(P.side, P.ev_fork[n], 0));
+        return R > 16 && (dt || S.mp[n].rowtiles * (long)S.mp[n].nsplit <= device_cus());
+    };
+    auto launch_vinv = [&](int n, hipStream_t vs) {
+        const double *g2 = P.order > 2 ? P.G[2] : P.G[0], *g3 = P.order > 3 ? P.G[3] : P.G[0];
+        if (R <= 16) k_vinv<4><<<1, 256, 0, vs>>>(P.G[0], P.G[1], g2, g3, P.order, n, R, P.Vinv[n], P.status,
+                                                P.vinv_ws);
+        else if (R <= 32) k_vinv<8><<<1, 256, 0, vs>>>(P.G[0], P.G[1], g2, g3, P.order, n, R, P.Vinv[n], P.status,
+                                                     P.vinv_ws);
+        else k_vinv<16><<<1, 256, 0, vs>>>(P.G[0], P.G[1], g2, g3, P.order, n, R, P.Vinv[n], P.status,
+                                         P.vinv_ws);
+    };
+    // forked[n]: mode n's inverse was already forked to the side stream (after the previous
+    // mode's solve, behind that mode's Gram sum)
+    bool forked[4] = {false, false, false, false};
+    for (int n = 0; n < P.order; ++n) {
+        const bool dt = S.dt.on && n > 0;
+        const bool beside = beside_of(n);
+        if (!forked[n]) {
+            if (beside) {
+                TDX_HIP(hipEventRecord(P.ev_fork[n], s));
+                TDX_HIP(hipStreamWaitEvent(P.side, P.ev_fork[n], 0));
+            }
+            launch_vinv(n, beside ? P.side : s);
+            TDX_LAUNCH_CHECK();
+            if (beside) TDX_HIP(hipEventRecord(P.ev_join[n], P.side));
         }
-        {
-            const hipStream_t vs = beside ? P.side : s;
-            const double *g2 = P.order > 2 ? P.G[2] : P.G[0], *g3 = P.order > 3 ? P.G[3] : P.G[0];
-            if (R <= 16) k_vinv<4><<<1, 256, 0, vs>>>(P.G[0], P.G[1], g2, g3, P.order, n, R, P.Vinv[n], P.status,
-                                                    P.vinv_ws);
-            else if (R <= 32) k_vinv<8><<<1, 256, 0, vs>>>(P.G[0], P.G[1], g2, g3, P.order, n, R, P.Vinv[n], P.status,
-                                                    P.vinv_ws);
-            else k_vinv<16><<<1, 256, 0, vs>>>(P.G[0], P.G[1], g2, g3, P.order, n, R, P.Vinv[n], P.status,
-                                                    P.vinv_ws);
-        }
-        TDX_LAUNCH_CHECK();
-        if (beside) TDX_HIP(hipEventRecord(P.ev_join[n], P.side));
         const float *panels[4] = {P.panel[0], P.panel[1], P.panel[2], P.panel[3]};
         const float *row_scale = (S.dtype == TDX_FP8_E4M3 && n == 0) ? S.scales : nullptr;
         const int *skip = (P.ls_mode && n == 0) ? &P.ls->skip : nullptr;
@@ -864,16 +873,32 @@ static int enqueue_sweep(CpPlan &P, const Storage &S, hipStream_t s, bool fit) {
         const int quant = (S.dt.on && n == 0) ? 1 : 0;
         k_solve<<<nblk, 256, 0, s>>>(P.partial, nsplit, rows_pad, P.Rp, In, R, P.Vinv[n], P.U[n], P.panel[n], pscale,
                                      P.ip_part, P.gram_part, skip ? &P.ls->skip : nullptr, quant);
-        // G_n; after the last mode together with the fit of the sweep
+        // G_n; after the last mode together with the fit of the sweep.  The next mode's MTTKRP
+        // needs only this mode's factor, so when its inverse runs beside it, G_n is summed on
+        // the side stream ahead of that inverse, off the MTTKRP's path (the next solve, which
+        // rewrites gram_part, waits for that inverse).  Not before the pass over T: it is as
+        // short as the inverse, and launched together with it the inverse ran 76 us instead
+        // of ~51 (1024^3, R 64) and became the longer branch.
+        const bool early = n + 1 < P.order && beside_of(n + 1) && !(S.dt.on && n + 1 == 2);
         {
             const int gb = (int)ceil_div((long)R * R, 256L);
-            if (fit && n == P.order - 1)
+            if (fit && n == P.order - 1) {
                 k_gram_psum_fit<<<gb, 256, 0, s>>>(P.gram_part, nblk, R, P.order, n, P.G[n], P.G[0], P.G[1],
                                                    P.order > 2 ? P.G[2] : P.G[0], P.order > 3 ? P.G[3] : P.G[0],
                                                    P.ip_part, nblk, P.normx2, P.fit_hist, P.counter, P.hpart,
                                                    P.fit_done);
-            else
+            } else if (early) {
+                TDX_HIP(hipEventRecord(P.ev_fork[n + 1], s));
+                TDX_HIP(hipStreamWaitEvent(P.side, P.ev_fork[n + 1], 0));
+                k_gram_psum<<<gb, 256, 0, P.side>>>(P.gram_part, nblk, R * R, P.G[n]);
+                TDX_LAUNCH_CHECK();
+                launch_vinv(n + 1, P.side);
+                TDX_LAUNCH_CHECK();
+                TDX_HIP(hipEventRecord(P.ev_join[n + 1], P.side));
+                forked[n + 1] = true;
+            } else {
                 k_gram_psum<<<gb, 256, 0, s>>>(P.gram_part, nblk, R * R, P.G[n]);
+            }
         }
         TDX_LAUNCH_CHECK();
     }
