@@ -34,9 +34,9 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per BF16 MTTKRP launch (mean of the 3 modes) from the two counter passes of
 # `bench.py --kernels-only` (FETCH_SIZE and WRITE_SIZE, each scaled by its reading for a
 # plain 1 GiB copy); counters cannot be read in the timed run itself
-MTTKRP_BF16_TRAFFIC = 2171587296
+MTTKRP_BF16_TRAFFIC = 2171970286
 # the same for the shared pass over X (BF16 storage: X read, T written)
-SHARED_PASS_BF16_TRAFFIC = 2576021109
+SHARED_PASS_BF16_TRAFFIC = 2575748744
 TARGET_DELTA = 1e-4
 
 
