@@ -24,6 +24,7 @@ import argparse
 import json
 import os
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -116,7 +117,9 @@ def extras(args, log):
     X, b = W.c4()
     U = torch.empty(b.u_floats, device="cuda")
     tdx.cp_als_batched(X, U, b, 0, init=0, seed=1)
-    cache = os.path.join(ROOT, "gpurun_out", "batched_tuning.txt")
+    # Autotuner cache: the host's temporary directory unless TDX_TUNING_CACHE names a file.
+    cache = os.environ.get("TDX_TUNING_CACHE",
+                           os.path.join(tempfile.gettempdir(), "tdx", "batched_tuning.txt"))
     os.makedirs(os.path.dirname(cache), exist_ok=True)
     best, times = tdx.cp_batched_autotune(X, U, b, trial_sweeps=2, cache_path=cache)
     st, fit, ms = tdx.cp_als_batched(X, U, b, 30, init=0, seed=1, variant=best)
