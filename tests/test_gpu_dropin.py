@@ -1,0 +1,62 @@
+"""A C++ program written against the reference's surface (include/cudecompose.h, which
+restates inc/cudecompose.h:23-52) runs on the GPU with its own hipMalloc'd buffers, and
+every output matches the oracle bit for bit.  The program (tests/dropin_caller.cpp,
+built by __graft_entry__.build()) writes its integer inputs and its outputs to one file.
+
+Oracle functions used: t2m (src/tensor2matrix.cu:38-52), kr (src/kr.cu:3-11), hadmard
+and hadmard3 (src/hadmard.cu:10-30), tensor_product (src/tensorproduct.cu), mttkrp
+(src/mttkrp.cu:3-80), ttm (src/ttm.cu:4-47).  rse (src/rse.cu:4-30) is checked against
+its closed form, 2 / ||X|| for one entry changed by 2."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "first-project_amd", "_build", "dropin_caller")
+M, N, K, R, D = 5, 4, 3, 3, 2
+
+
+@pytest.mark.gpu
+def test_cpp_caller_through_cudecompose_h(tmp_path):
+    assert os.path.exists(EXE), "build first: python -c 'import __graft_entry__ as g; g.build()'"
+    out = tmp_path / "dropin.bin"
+    print("dropin_caller", out, flush=True)
+    sys.stdout.flush()
+    r = subprocess.run([EXE, str(out)], capture_output=True, text=True, timeout=90)
+    assert r.returncode == 0, (r.returncode, r.stdout, r.stderr)
+    assert "dropin-caller-ok" in r.stdout
+    buf = np.fromfile(str(out), dtype=np.float32)
+    pos = 0
+
+    def take(n):
+        nonlocal pos
+        v = buf[pos:pos + n]
+        pos += n
+        assert v.size == n, "file too short"
+        return v
+
+    dims = (M, N, K)
+    X, A, A2, A3, B, C = take(M * N * K), take(M * R), take(M * R), take(M * R), take(N * R), take(K * R)
+    U = [take(D * d) for d in dims]
+    assert np.array_equal(X, np.round(X)) and np.abs(X).max() <= 4  # integer inputs as generated
+    for flag in (1, 2, 3):
+        assert np.array_equal(take(M * N * K), O.t2m(X, flag, M, N, K)), ("t2m", flag)
+    assert np.array_equal(take(M * N * R), O.kr(A, B, M, N, R))
+    assert np.array_equal(take(M * R), O.hadamard(A, A2))
+    assert np.array_equal(take(M * R), O.hadamard3(A, A2, A3))
+    assert np.array_equal(take(M * R * N * R), O.tensor_product(A, B, M, R, N, R))
+    F = [a.reshape((d, R), order="F").astype(np.float64) for a, d in zip((A, B, C), dims)]
+    for flag in (1, 2, 3):
+        ref = O.mttkrp(X, dims, F, flag - 1).ravel(order="F").astype(np.float32)
+        assert np.array_equal(take(dims[flag - 1] * R), ref), ("mttkrp", flag)
+    for flag in (1, 2, 3):
+        ref = O.ttm(X, U[flag - 1], flag, M, N, K, D).astype(np.float32)
+        assert np.array_equal(take(M * N * K // dims[flag - 1] * D), ref), ("ttm", flag)
+    err = take(1)[0]
+    assert pos == buf.size
+    assert abs(err - 2.0 / np.linalg.norm(X.astype(np.float64))) < 1e-6
