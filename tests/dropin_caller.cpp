@@ -6,8 +6,9 @@
 //
 // Usage: dropin_caller <out.bin>.  Inputs are small integers, so every product and sum is
 // exact in FP32.  The file holds the inputs, then the outputs, as raw float32 arrays in the
-// order written below, then the rse error.  Exit status 0 on success; otherwise the
-// failing call is named on stderr.
+// order written below, then the rse error, then the relative errors of the cp_als and
+// tucker_hosvd reconstructions.  Exit status 0 on success; otherwise the failing call is
+// named on stderr.
 #include <hip/hip_runtime_api.h>
 
 #include <cstdio>
@@ -103,8 +104,39 @@ int main(int argc, char **argv) {
         return 4;
     }
     fwrite(&err, sizeof(dt), 1, out_file);
+
+    // Drivers, as the reference's test/test_cp.cpp and test/test_tucker.cpp use them: build an
+    // exact low-rank tensor, decompose it, rebuild it from the result and take rse.
+    const long P = 12, Q = 10, S = 8, RC = 2, RT = 2;
+    dt *dXc = dev(std::vector<dt>(P * Q * S, 0.f)), *drec = dev(std::vector<dt>(P * Q * S, 0.f));
+    dt *dFa = dev(ints(P * RC, 5, 2)), *dFb = dev(ints(Q * RC, 7, 3)), *dFc = dev(ints(S * RC, 4, 1));
+    dt *dOa = dev(std::vector<dt>(P * RC, 0.f)), *dOb = dev(std::vector<dt>(Q * RC, 0.f)),
+       *dOc = dev(std::vector<dt>(S * RC, 0.f));
+    dt errs[2] = {-1.f, -1.f};
+    gencp(dXc, dFa, dFb, dFc, P, Q, S, RC);
+    cp_als(dXc, dOa, dOb, dOc, P, Q, S, RC);
+    gencp(drec, dOa, dOb, dOc, P, Q, S, RC);
+    rse(dXc, drec, P * Q * S, &errs[0]);
+    if (tdx_last_status() != TDX_OK) {
+        fprintf(stderr, "cp_als chain: status %d\n", tdx_last_status());
+        return 4;
+    }
+    dt *dG = dev(ints(RT * RT * RT, 5, 4)), *dT1 = dev(std::vector<dt>(P * RT, 0.f)),
+       *dT2 = dev(std::vector<dt>(Q * RT, 0.f)), *dT3 = dev(std::vector<dt>(S * RT, 0.f)),
+       *dG2 = dev(std::vector<dt>(RT * RT * RT, 0.f));
+    gentucker(dXc, dG, dFa, dFb, dFc, P, Q, S, RT, RT, RT);
+    tucker_hosvd(dXc, dG2, dT1, dT2, dT3, P, Q, S, RT, RT, RT);
+    gentucker(drec, dG2, dT1, dT2, dT3, P, Q, S, RT, RT, RT);
+    rse(dXc, drec, P * Q * S, &errs[1]);
+    if (tdx_last_status() != TDX_OK) {
+        fprintf(stderr, "tucker_hosvd chain: status %d\n", tdx_last_status());
+        return 4;
+    }
+    fwrite(errs, sizeof(dt), 2, out_file);
     fclose(out_file);
-    for (dt *p : {dX, dA, dA2, dA3, dB, dC, dU[0], dU[1], dU[2], dout, dX1}) hipFree(p);
+    for (dt *p : {dX, dA, dA2, dA3, dB, dC, dU[0], dU[1], dU[2], dout, dX1, dXc, drec, dFa, dFb, dFc, dOa, dOb,
+                  dOc, dG, dT1, dT2, dT3, dG2})
+        if (hipFree(p) != hipSuccess) return 3;
     printf("dropin-caller-ok\n");
     return 0;
 }

@@ -6,7 +6,9 @@ built by __graft_entry__.build()) writes its integer inputs and its outputs to o
 Oracle functions used: t2m (src/tensor2matrix.cu:38-52), kr (src/kr.cu:3-11), hadmard
 and hadmard3 (src/hadmard.cu:10-30), tensor_product (src/tensorproduct.cu), mttkrp
 (src/mttkrp.cu:3-80), ttm (src/ttm.cu:4-47).  rse (src/rse.cu:4-30) is checked against
-its closed form, 2 / ||X|| for one entry changed by 2."""
+its closed form, 2 / ||X|| for one entry changed by 2.  The drivers cp_als
+(src/cp_als.cu:3-134) and tucker_hosvd (src/tucker_hosvd.cu:3-103) are checked by exact
+low-rank recovery through gencp / gentucker (src/rse.cu:17-51) and rse."""
 import os
 import subprocess
 import sys
@@ -58,5 +60,12 @@ def test_cpp_caller_through_cudecompose_h(tmp_path):
         ref = O.ttm(X, U[flag - 1], flag, M, N, K, D).astype(np.float32)
         assert np.array_equal(take(M * N * K // dims[flag - 1] * D), ref), ("ttm", flag)
     err = take(1)[0]
-    assert pos == buf.size
     assert abs(err - 2.0 / np.linalg.norm(X.astype(np.float64))) < 1e-6
+    # exact rank-2 CP and multilinear-rank-(2,2,2) tensors (12 x 10 x 8, integer factors with
+    # column congruence <= 0.46): cp_als and tucker_hosvd recover them, so the rebuilt
+    # tensor's relative error is at FP32 rounding level
+    e_cp, e_tucker = take(2)
+    assert pos == buf.size
+    print("cp_als relerr", e_cp, "tucker_hosvd relerr", e_tucker)
+    assert 0 <= e_cp < 1e-4
+    assert 0 <= e_tucker < 1e-5
