@@ -458,19 +458,33 @@ int tdx_cp_batched_autotune(const float *X, const float *U, const long *dims, co
                             const long *uoff, long ntensors, long u_floats, int trial_sweeps, const char *cache_path,
                             int *best, float *ms_per_sweep) {
     if (!X || !U || !best || ntensors <= 0 || u_floats <= 0 || trial_sweeps <= 0) return record(TDX_ERR_ARG);
-    // cache key: device name + tensor count + total elements + rank histogram
+    // cache key: device name + tensor count + total elements + the full rank histogram (1..16)
+    // + an FNV-1a digest of the (extents, rank) list in order
     hipDeviceProp_t prop;
     int dev = 0;
     TDX_HIP(hipGetDevice(&dev));
     TDX_HIP(hipGetDeviceProperties(&prop, dev));
     long tot = 0, rh[17] = {0};
+    uint64_t digest = 1469598103934665603ull;
+    auto mix = [&](long v) {
+        for (int b = 0; b < 8; ++b) {
+            digest ^= (uint64_t)((v >> (8 * b)) & 0xff);
+            digest *= 1099511628211ull;
+        }
+    };
     for (long t = 0; t < ntensors; ++t) {
         tot += dims[3 * t] * dims[3 * t + 1] * dims[3 * t + 2];
-        if (ranks[t] >= 0 && ranks[t] <= 16) rh[ranks[t]]++;
+        if (ranks[t] >= 1 && ranks[t] <= 16) rh[ranks[t]]++;
+        mix(dims[3 * t]);
+        mix(dims[3 * t + 1]);
+        mix(dims[3 * t + 2]);
+        mix(ranks[t]);
     }
     char key[512];
-    snprintf(key, sizeof(key), "%s|n%ld|e%ld|r4:%ld,r8:%ld,r16:%ld", prop.gcnArchName, ntensors, tot, rh[4], rh[8],
-             rh[16]);
+    int kl = snprintf(key, sizeof(key), "%s|n%ld|e%ld|r", prop.gcnArchName, ntensors, tot);
+    for (int r = 1; r <= 16 && kl > 0 && kl < (int)sizeof(key) - 24; ++r)
+        kl += snprintf(key + kl, sizeof(key) - kl, "%s%ld", r > 1 ? "," : "", rh[r]);
+    if (kl > 0 && kl < (int)sizeof(key) - 24) snprintf(key + kl, sizeof(key) - kl, "|d%016llx", (unsigned long long)digest);
     if (cache_path) {
         FILE *f = fopen(cache_path, "r");
         if (f) {

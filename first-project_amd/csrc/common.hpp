@@ -38,6 +38,8 @@ enum WsSlot {
     WS_MISC1,
     WS_MISC2,
     WS_MISC3,
+    WS_STORE,        // CP-ALS: the BF16 / FP8 copy of the tensor for the main phase
+    WS_STORE_SCALES, // CP-ALS: its FP8 slice scales
     WS_COUNT
 };
 int workspace(int slot, size_t bytes, void **ptr);

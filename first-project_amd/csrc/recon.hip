@@ -336,11 +336,13 @@ static int gentucker_n(float *rec, const float *core, const float *const *U, int
     hipStream_t s = stream();
     long cur[4];
     for (int m = 0; m < order; ++m) cur[m] = ranks[m];
+    // the intermediates of steps 0 .. order-2 ping-pong between two workspaces; the last
+    // step writes into rec, so it does not size them
     long maxsz = 1;
     {
         long sz = 1;
         for (int m = 0; m < order; ++m) sz *= ranks[m];
-        for (int m = 0; m < order; ++m) {
+        for (int m = 0; m < order - 1; ++m) {
             sz = sz / ranks[m] * dims[m];
             if (sz > maxsz) maxsz = sz;
         }

@@ -49,7 +49,7 @@ class CpOptions(ctypes.Structure):
 
 class CpReport(ctypes.Structure):
     _fields_ = [("sweeps_done", ctypes.c_int), ("status", ctypes.c_int), ("relerr_fit", ctypes.c_double),
-                ("seconds", ctypes.c_double)]
+                ("seconds", ctypes.c_double), ("setup_seconds", ctypes.c_double)]
 
 
 _SIGS = {

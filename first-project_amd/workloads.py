@@ -69,16 +69,18 @@ def c2(n=1024, R=64):
     return X, dims, R
 
 
-def c3(n=128, r=32, order=4, eta=5e-3):
-    """Tucker tensor core x_1 U_1 ... x_d U_d + noise (noise scaled to eta ||X0||)."""
+def c3(n=128, r=32, order=4, eta=5e-3, with_factors=False):
+    """Tucker tensor core x_1 U_1 ... x_d U_d + noise (noise scaled to eta ||X0||); with
+    with_factors=True also the generating orthonormal factors (host FP64, n x r)."""
     import torch
     dims = (n,) * order
     ranks = (r,) * order
     core = _normal(r ** order, 0, 0)
-    Us = []
+    Us, Qs = [], []
     for m in range(order):
         G = _normal(n * r, m + 1, 0).cpu().numpy().reshape((n, r), order="F").astype(np.float64)
         Q, _ = np.linalg.qr(G)
+        Qs.append(Q)
         Us.append(torch.from_numpy(np.ascontiguousarray(Q.ravel(order="F"), np.float32)).cuda())
     X0 = torch.empty(n ** order, device="cuda")
     if order == 4:
@@ -87,6 +89,8 @@ def c3(n=128, r=32, order=4, eta=5e-3):
         tdx.gentucker(X0, core, *Us, *dims, *ranks)
     E = _normal(n ** order, 5, 0)
     X = X0 + (eta * X0.double().norm() / E.double().norm()).float() * E
+    if with_factors:
+        return X.contiguous(), dims, ranks, Qs
     return X.contiguous(), dims, ranks
 
 

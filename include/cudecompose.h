@@ -7,8 +7,14 @@
  * Differences from the reference, all deliberate:
  *  - no CUDA/HIP headers are pulled in and no kernels are re-exported (the reference
  *    declares 11 __global__ symbols, inc/cudecompose.h:22,27,31-33,35,45,47,55-57);
- *  - the functions return void as before, and record a status readable with
- *    tdx_last_status();
+ *  - the functions return void as before and record a status readable with
+ *    tdx_last_status(); a status other than TDX_OK is also printed to stderr
+ *    ("cudecompose: <function>: <reason>"), as the reference prints its own errors
+ *    (src/mttkrp.cu:75-77), and the outputs are then left unwritten;
+ *  - limits the reference does not have: cp_als, cp_als4 and cp_tensorcore take
+ *    rank r <= 64; tucker_hosvd, tucker_hosvd4 and tucker_tensorcore take mode sizes
+ *    a, b, c (, d) <= 128 (the FP64 eigensolve runs in one workgroup) and ranks
+ *    r_n <= the mode size.  Larger inputs return TDX_ERR_UNSUPPORTED (printed);
  *  - f2h takes the 16-bit type `half` defined below (IEEE binary16 bit patterns);
  *  - rse does not overwrite its second argument (src/rse.cu:9 does);
  *  - cp_tensorcore never frees the caller's buffers (src/tensorcore_cp.cu:136-139 does),
@@ -26,69 +32,74 @@
 typedef float dt;
 typedef uint16_t half;
 
-inline void hadmard(dt *d_A, dt *d_B, dt *d_C, long a, long b) { tdx_hadamard(d_A, d_B, d_C, a, b); }
+/* status of a void wrapper: printed to stderr unless TDX_OK (still kept in tdx_last_status) */
+inline void tdx_report_(const char *fn, int st) {
+    if (st != TDX_OK) fprintf(stderr, "cudecompose: %s: %s (status %d)\n", fn, tdx_status_string(st), st);
+}
+
+inline void hadmard(dt *d_A, dt *d_B, dt *d_C, long a, long b) { tdx_report_("hadmard", tdx_hadamard(d_A, d_B, d_C, a, b)); }
 inline void hadmard3(dt *d_A, dt *d_B, dt *d_C, dt *d_ABC, long a, long b) {
-    tdx_hadamard3(d_A, d_B, d_C, d_ABC, a, b);
+    tdx_report_("hadmard3", tdx_hadamard3(d_A, d_B, d_C, d_ABC, a, b));
 }
-inline void kr(dt *d_A, dt *d_B, dt *d_AkrB, long a, long b, long r) { tdx_kr(d_A, d_B, d_AkrB, a, b, r); }
+inline void kr(dt *d_A, dt *d_B, dt *d_AkrB, long a, long b, long r) { tdx_report_("kr", tdx_kr(d_A, d_B, d_AkrB, a, b, r)); }
 inline void tensor_product(dt *d_A, dt *d_B, dt *d_C, long a, long b, long c, long d) {
-    tdx_tensor_product(d_A, d_B, d_C, a, b, c, d);
+    tdx_report_("tensor_product", tdx_tensor_product(d_A, d_B, d_C, a, b, c, d));
 }
-inline void f2h(dt *A, half *B, long num) { tdx_f2h(A, B, num); }
+inline void f2h(dt *A, half *B, long num) { tdx_report_("f2h", tdx_f2h(A, B, num)); }
 inline void ttm(dt *d_X, dt *d_U, dt *d_XU, long flag, long a, long b, long c, long d) {
-    tdx_ttm(d_X, d_U, d_XU, flag, a, b, c, d);
-    tdx_synchronize();
+    tdx_report_("ttm", tdx_ttm(d_X, d_U, d_XU, flag, a, b, c, d));
+    if (tdx_last_status() == TDX_OK) tdx_report_("ttm", tdx_synchronize());
 }
 inline void mttkrp(dt *d_X, dt *left, dt *d_right, dt *d_XU, long flag, long m, long n, long k, long r) {
-    tdx_mttkrp(d_X, left, d_right, d_XU, flag, m, n, k, r);
-    tdx_synchronize();
+    tdx_report_("mttkrp", tdx_mttkrp(d_X, left, d_right, d_XU, flag, m, n, k, r));
+    if (tdx_last_status() == TDX_OK) tdx_report_("mttkrp", tdx_synchronize());
 }
-inline void t2m(dt *d_T1, dt *d_T2, long flag, long m, long n, long k) { tdx_t2m(d_T1, d_T2, flag, m, n, k); }
+inline void t2m(dt *d_T1, dt *d_T2, long flag, long m, long n, long k) { tdx_report_("t2m", tdx_t2m(d_T1, d_T2, flag, m, n, k)); }
 inline void tminust(dt *d_left, dt *d_right, long m) {
-    tdx_tminust(d_left, d_right, m);
-    tdx_synchronize();
+    tdx_report_("tminust", tdx_tminust(d_left, d_right, m));
+    if (tdx_last_status() == TDX_OK) tdx_report_("tminust", tdx_synchronize());
 }
-inline void rse(dt *d_X, dt *d_X1, long m, dt *error) { tdx_rse(d_X, d_X1, m, error); }
+inline void rse(dt *d_X, dt *d_X1, long m, dt *error) { tdx_report_("rse", tdx_rse(d_X, d_X1, m, error)); }
 inline void gencp(dt *d_rec, dt *d_A, dt *d_B, dt *d_C, long a, long b, long c, long r) {
-    tdx_gencp(d_rec, d_A, d_B, d_C, a, b, c, r);
-    tdx_synchronize();
+    tdx_report_("gencp", tdx_gencp(d_rec, d_A, d_B, d_C, a, b, c, r));
+    if (tdx_last_status() == TDX_OK) tdx_report_("gencp", tdx_synchronize());
 }
 inline void gentucker(dt *d_rec, dt *d_core, dt *d_A, dt *d_B, dt *d_C, long a, long b, long c, long r1, long r2,
                       long r3) {
-    tdx_gentucker(d_rec, d_core, d_A, d_B, d_C, a, b, c, r1, r2, r3);
-    tdx_synchronize();
+    tdx_report_("gentucker", tdx_gentucker(d_rec, d_core, d_A, d_B, d_C, a, b, c, r1, r2, r3));
+    if (tdx_last_status() == TDX_OK) tdx_report_("gentucker", tdx_synchronize());
 }
 inline void gencp4(dt *d_T, dt *d_AA, dt *d_BB, dt *d_CC, dt *d_DD, long a, long b, long c, long d, long r) {
-    tdx_gencp4(d_T, d_AA, d_BB, d_CC, d_DD, a, b, c, d, r);
-    tdx_synchronize();
+    tdx_report_("gencp4", tdx_gencp4(d_T, d_AA, d_BB, d_CC, d_DD, a, b, c, d, r));
+    if (tdx_last_status() == TDX_OK) tdx_report_("gencp4", tdx_synchronize());
 }
 inline void gentucker4(dt *d_T, dt *d_G, dt *d_A, dt *d_B, dt *d_C, dt *d_D, long a, long b, long c, long d,
                        long r1, long r2, long r3, long r4) {
-    tdx_gentucker4(d_T, d_G, d_A, d_B, d_C, d_D, a, b, c, d, r1, r2, r3, r4);
-    tdx_synchronize();
+    tdx_report_("gentucker4", tdx_gentucker4(d_T, d_G, d_A, d_B, d_C, d_D, a, b, c, d, r1, r2, r3, r4));
+    if (tdx_last_status() == TDX_OK) tdx_report_("gentucker4", tdx_synchronize());
 }
 inline void cp_als(dt *d_X, dt *d_A, dt *d_B, dt *d_C, long a, long b, long c, long r) {
-    tdx_cp_als(d_X, d_A, d_B, d_C, a, b, c, r);
+    tdx_report_("cp_als", tdx_cp_als(d_X, d_A, d_B, d_C, a, b, c, r));
 }
 inline void cp_als4(dt *d_X, dt *d_A, dt *d_B, dt *d_C, dt *d_D, long a, long b, long c, long d, long r) {
-    tdx_cp_als4(d_X, d_A, d_B, d_C, d_D, a, b, c, d, r);
+    tdx_report_("cp_als4", tdx_cp_als4(d_X, d_A, d_B, d_C, d_D, a, b, c, d, r));
 }
 inline void cp_tensorcore(dt *d_X, dt *d_A, dt *d_B, dt *d_C, long a, long b, long c, long r) {
-    tdx_cp_tensorcore(d_X, d_A, d_B, d_C, a, b, c, r);
+    tdx_report_("cp_tensorcore", tdx_cp_tensorcore(d_X, d_A, d_B, d_C, a, b, c, r));
 }
 inline void tucker_hosvd(dt *d_X, dt *d_core, dt *d_U1, dt *d_U2, dt *d_U3, long a, long b, long c, long r1,
                          long r2, long r3) {
-    tdx_tucker_hosvd(d_X, d_core, d_U1, d_U2, d_U3, a, b, c, r1, r2, r3);
+    tdx_report_("tucker_hosvd", tdx_tucker_hosvd(d_X, d_core, d_U1, d_U2, d_U3, a, b, c, r1, r2, r3));
 }
 inline void tucker_tensorcore(dt *d_X, dt *d_core, dt *d_U1, dt *d_U2, dt *d_U3, long a, long b, long c, long r1,
                               long r2, long r3) {
-    tdx_tucker_tensorcore(d_X, d_core, d_U1, d_U2, d_U3, a, b, c, r1, r2, r3);
+    tdx_report_("tucker_tensorcore", tdx_tucker_tensorcore(d_X, d_core, d_U1, d_U2, d_U3, a, b, c, r1, r2, r3));
 }
 inline void tucker_hosvd4(dt *d_X, dt *d_core, dt *d_U1, dt *d_U2, dt *d_U3, dt *d_U4, long a, long b, long c,
                           long d, long r1, long r2, long r3, long r4) {
-    tdx_tucker_hosvd4(d_X, d_core, d_U1, d_U2, d_U3, d_U4, a, b, c, d, r1, r2, r3, r4);
+    tdx_report_("tucker_hosvd4", tdx_tucker_hosvd4(d_X, d_core, d_U1, d_U2, d_U3, d_U4, a, b, c, d, r1, r2, r3, r4));
 }
-inline void initMat(dt *d_A, long m) { tdx_init_identity(d_A, m); }
+inline void initMat(dt *d_A, long m) { tdx_report_("initMat", tdx_init_identity(d_A, m)); }
 
 /* printTensor (src/kr.cu:13-30): host copy and column-major print, one frontal slice
  * per block. */

@@ -205,17 +205,24 @@ typedef struct {
     int sweeps_done;
     int status;              /* TDX_OK or TDX_WARN_SINGULAR */
     double relerr_fit;       /* last fast-fit relative error (FP64 identity) */
-    double seconds;          /* device time of the sweeps (hipEvents) */
+    double seconds;          /* device time of the whole call from the first device work on:
+                                norm of X, the BF16 / FP8 conversion of X, factor panels and
+                                every sweep (hipEvents) */
+    double setup_seconds;    /* the part of `seconds` before the first sweep (norm, conversion,
+                                panels); sweep_ms covers the rest */
 } tdx_cp_report;
 
 void tdx_cp_default_options(tdx_cp_options *opt);
-/* new: CP-ALS of an order-3/4 FP32 tensor; factors[m] (I_m x rank) in/out.
+/* new: CP-ALS of an order-3/4 FP32 tensor; factors[m] (I_m x rank) in/out.  The BF16 / FP8
+ * copy of X for the main phase lives in the library's workspace pool (allocated by the first
+ * call that needs it, reused after; tdx_release_workspace frees it).  rank <= 64
+ * (larger ranks return TDX_ERR_UNSUPPORTED; the reference has no limit).
  * fit_hist (host, may be NULL) receives the per-sweep fast-fit relative error;
  * sweep_ms (host, may be NULL) receives the per-sweep device time. */
 int tdx_cp_als_ex(const float *X, int order, const long *dims, long rank, float *const *factors,
                   const tdx_cp_options *opt, tdx_cp_report *report, double *fit_hist, float *sweep_ms);
 /* cp_als: src/cp_als.cu:3-134 -- 500 FP32 sweeps, B and C drawn from the seeded generator
- * (seed 1234) as in the reference; A, B, C are outputs. */
+ * (seed 1234) as in the reference; A, B, C are outputs.  r <= 64 (as tdx_cp_als_ex). */
 int tdx_cp_als(const float *X, float *A, float *B, float *C, long a, long b, long c, long r);
 /* cp_als4: src/cp4_als.cu:3-186 */
 int tdx_cp_als4(const float *X, float *A, float *B, float *C, float *D, long a, long b, long c, long d, long r);
@@ -266,6 +273,10 @@ int tdx_mlp_train(float *params, const float *X, const float *T, long nsamples, 
                   int epochs, long batch, float lr, uint64_t seed, float *loss_hist);
 
 /* ---- Tucker ---- */
+/* Tucker limits (all entry points below): every mode size <= 128 (the Gram kernel's block and
+ * the one-workgroup FP64 eigensolve), 1 <= r_n <= mode size; larger modes return
+ * TDX_ERR_UNSUPPORTED (the reference has no limit, decomposition/GPU/normal/dprice/de_tucker.cpp:31
+ * runs 1120^3). */
 /* tucker_hosvd: src/tucker_hosvd.cu:3-103 */
 int tdx_tucker_hosvd(const float *X, float *core, float *U1, float *U2, float *U3,
                      long a, long b, long c, long r1, long r2, long r3);

@@ -304,3 +304,118 @@ def mlp_backward(x, t, params, nin, width, nout):
 
 def adam(p, g, m, v, t, lr, b1=0.9, b2=0.999, eps=1e-8):
     lib().orc_adam(p, np.ascontiguousarray(g, np.float64), m, v, p.size, t, lr, b1, b2, eps)
+
+
+def cp_als3_gemm(X, dims, U0, sweeps, dtype=np.float64, log=None, stop_at=None):
+    """Order-3 CP-ALS with the MTTKRPs as GEMMs over the native layout: the formulation of the
+    reference's CPU path, decomposition/CPU/python/mycp.py:11-22 (Hadamard of the other Grams,
+    TensorLy's unfolding_dot_khatri_rao = X(n) (kr of the others), a linear solve per mode),
+    with the reference library's mode order and conventions (src/cp_als.cu:44-119).
+
+    Mode 0: M0 = X(0) (C (.) B), one GEMM.  Modes 1 and 2 contract A_new over i0 first:
+    T = A_new^T X(0) (R x J K, one GEMM), then M1[j,r] = sum_k T[r,j,k] C[k,r] and
+    M2[k,r] = sum_j T[r,j,k] B_new[j,r] -- the same sums as X(1) (C (.) A) and X(2) (B (.) A)
+    in another order (exact algebra; FP64 rounding differences only).  The solve is the
+    Cholesky solve of orc_cp_als (pseudo-inverse fallback).  Arithmetic in `dtype` (FP64: the
+    oracle; FP32: the single-precision CPU baseline).  Returns (U list, per-sweep fast-fit
+    relative error (Tensorlab cpd_als.m:281-283, accumulated in FP64), seconds per sweep).
+    stop_at: stop after the first sweep whose relative error is <= stop_at."""
+    import time
+    I, J, K = (int(d) for d in dims)
+    R = U0[0].shape[1]
+    U = [np.array(u, dtype=np.float64, order="F") for u in U0]
+    Xd = np.asarray(X).reshape(-1)
+    if Xd.dtype != dtype:
+        Xd = Xd.astype(dtype)
+    X0 = Xd.reshape((I, J * K), order="F")           # mode-0 unfolding, a view
+    normx2 = float(np.dot(Xd.astype(np.float64, copy=False), Xd.astype(np.float64, copy=False))) \
+        if dtype == np.float64 else float(np.sum(np.square(Xd, dtype=np.float64)))
+
+    def solve(V, M):
+        try:
+            L = np.linalg.cholesky(V)
+            Y = np.linalg.solve(L, M.T)
+            return np.linalg.solve(L.T, Y).T
+        except np.linalg.LinAlgError:
+            return (np.linalg.pinv(V, rcond=1e-12) @ M.T).T
+
+    G = [u.T @ u for u in U]
+    fit, secs = [], []
+    for it in range(sweeps):
+        t0 = time.perf_counter()
+        A, B, C = U
+        KR = (C.astype(dtype)[:, None, :] * B.astype(dtype)[None, :, :]).reshape(J * K, R)  # row j + k J
+        M0 = np.asarray(X0 @ KR, np.float64)
+        del KR
+        U[0] = solve(G[1] * G[2], M0)
+        G[0] = U[0].T @ U[0]
+        T = (U[0].T.astype(dtype) @ X0).reshape(R, K, J)   # T[r, k, j]
+        M1 = np.matmul(T.transpose(0, 2, 1), U[2].T.astype(dtype)[:, :, None])[:, :, 0].T  # (J, R)
+        U[1] = solve(G[0] * G[2], np.asarray(M1, np.float64))
+        G[1] = U[1].T @ U[1]
+        M2 = np.matmul(T, U[1].T.astype(dtype)[:, :, None])[:, :, 0].T                  # (K, R)
+        M2 = np.asarray(M2, np.float64)
+        del T
+        U[2] = solve(G[0] * G[1], M2)
+        G[2] = U[2].T @ U[2]
+        r2 = normx2 - 2.0 * float(np.sum(M2 * U[2])) + float(np.sum(G[0] * G[1] * G[2]))
+        fit.append(float(np.sqrt(abs(r2) / normx2)))
+        secs.append(time.perf_counter() - t0)
+        if log:
+            log(f"  sweep {it + 1}: relerr {fit[-1]:.12f} ({secs[-1]:.2f} s)")
+        if stop_at is not None and fit[-1] <= stop_at:
+            break
+    return U, np.asarray(fit), np.asarray(secs)
+
+
+def _unfold_gram_gemm(X, dims, mode):
+    """G = X(mode) X(mode)^T in FP64 as one tensordot over the other indices."""
+    P = int(np.prod(dims[:mode]))
+    Q = int(np.prod(dims[mode + 1:]))
+    Y = X.reshape((P, dims[mode], Q), order="F")
+    if P == 1:
+        Y2 = Y[0]                       # (In, Q)
+        return Y2 @ Y2.T
+    if Q == 1:
+        Y2 = Y[:, :, 0]                 # (P, In)
+        return Y2.T @ Y2
+    return np.tensordot(Y, Y, axes=([0, 2], [0, 2]))
+
+
+def _top_eigvecs_lapack(G, r):
+    """The last r eigenvector columns in ascending order with the largest-magnitude entry of
+    each made positive: the convention of orc_sym_eig / top_eigvecs (src/tucker_hosvd.cu:68-70)."""
+    w, V = np.linalg.eigh(G)
+    V = V[:, G.shape[0] - r:].copy()
+    k = np.argmax(np.abs(V), axis=0)
+    V *= np.where(V[k, np.arange(r)] < 0, -1.0, 1.0)
+    return V, w[G.shape[0] - r:]
+
+
+def _ttm_t_gemm(X, dims, mode, U):
+    """Y = X x_mode U^T (U is I_mode x r), F-order flat in, F-order flat out."""
+    P = int(np.prod(dims[:mode]))
+    Q = int(np.prod(dims[mode + 1:]))
+    Y = X.reshape((P, dims[mode], Q), order="F")
+    Z = np.einsum("piq,ir->prq", Y, U, optimize=True) if P > 1 else np.tensordot(U, Y[0], axes=([0], [0]))[None]
+    return np.asarray(Z).reshape(-1, order="F")
+
+
+def tucker_hosvd_gemm(X, dims, ranks):
+    """FP64 HOSVD (src/tucker_hosvd.cu:3-103, src/tucker4_hosvd.cu:3-141) with the Grams and
+    the TTM chain as GEMMs and the eigensolve by LAPACK (eigh): the same quantities as
+    orc_tucker_hosvd (whose per-pair Gram loops do not scale to 128^4), same conventions;
+    tests/test_oracle.py pins one against the other.  Returns (U list, core flat F-order,
+    top eigenvalues per mode)."""
+    Xd = np.asarray(X, np.float64).reshape(-1)
+    U, W = [], []
+    for m in range(len(dims)):
+        V, w = _top_eigvecs_lapack(_unfold_gram_gemm(Xd, dims, m), ranks[m])
+        U.append(V)
+        W.append(w)
+    cur = list(dims)
+    Y = Xd
+    for m in range(len(dims)):
+        Y = _ttm_t_gemm(Y, cur, m, U[m])
+        cur[m] = ranks[m]
+    return U, Y, W

@@ -5,10 +5,12 @@
 // compares every output with the oracle.
 //
 // Usage: dropin_caller <out.bin>.  Inputs are small integers, so every product and sum is
-// exact in FP32.  The file holds the inputs, then the outputs, as raw float32 arrays in the
-// order written below, then the rse error, then the relative errors of the cp_als and
-// tucker_hosvd reconstructions.  Exit status 0 on success; otherwise the failing call is
-// named on stderr.
+// exact in FP32 (f2h alone gets non-integers).  The file holds the inputs, then the outputs,
+// as raw float32 arrays in the order written below, the rse error, the relative errors of
+// the cp_als and tucker_hosvd reconstructions, then the f2h / tminust / initMat / gencp4 /
+// gentucker4 inputs and outputs and the relative errors of the cp_als4, cp_tensorcore,
+// tucker_hosvd4 and tucker_tensorcore reconstructions.  All 22 functions of the reference
+// header run.  Exit status 0 on success; otherwise the failing call is named on stderr.
 #include <hip/hip_runtime_api.h>
 
 #include <cstdio>
@@ -133,9 +135,89 @@ int main(int argc, char **argv) {
         return 4;
     }
     fwrite(errs, sizeof(dt), 2, out_file);
+
+    // The remaining entry points: f2h, tminust, initMat, printTensor, the order-4 builders
+    // gencp4 / gentucker4 (outputs compared with the oracle), and the drivers cp_als4,
+    // cp_tensorcore, tucker_hosvd4, tucker_tensorcore by low-rank recovery as above.
+    half *dh = nullptr;
+    if (hipMalloc((void **)&dh, M * N * K * sizeof(half)) != hipSuccess) return 3;
+    std::vector<dt> Xf = X;
+    for (long i = 0; i < M * N * K; ++i) Xf[i] = X[i] * 0.3f + 1e-3f * (dt)i;  // non-integers: rounding matters
+    dt *dXf = dev(Xf);
+    put(Xf);
+    f2h(dXf, dh, M * N * K);
+    {
+        std::vector<half> hh(M * N * K);
+        if (tdx_last_status() != TDX_OK || hipMemcpy(hh.data(), dh, hh.size() * sizeof(half), hipMemcpyDeviceToHost) != hipSuccess) {
+            fprintf(stderr, "f2h: status %d\n", tdx_last_status());
+            return 4;
+        }
+        std::vector<dt> hf(hh.begin(), hh.end());  // bit patterns as exact floats
+        put(hf);
+    }
+    dt *dL = dev(A), *dRt = dev(A2);
+    tminust(dL, dRt, M * R);
+    take("tminust", dRt, M * R);
+    initMat(dout, 6);
+    take("initMat", dout, 36);
+    printTensor(dA, 2, 2, 2);
+    fflush(stdout);
+    if (tdx_last_status() != TDX_OK) {
+        fprintf(stderr, "printTensor: status %d\n", tdx_last_status());
+        return 4;
+    }
+
+    const long P4 = 7, Q4 = 6, S4 = 5, T4 = 4, R4 = 2;
+    std::vector<dt> Fa4 = ints(P4 * R4, 5, 2), Fb4 = ints(Q4 * R4, 7, 3), Fc4 = ints(S4 * R4, 4, 1),
+                    Fd4 = ints(T4 * R4, 2, 7), G4 = ints(R4 * R4 * R4 * R4, 5, 4);
+    for (const auto *h : {&Fa4, &Fb4, &Fc4, &Fd4, &G4}) put(*h);
+    const long N4 = P4 * Q4 * S4 * T4;
+    dt *dX4 = dev(std::vector<dt>(N4, 0.f)), *drec4 = dev(std::vector<dt>(N4, 0.f));
+    dt *dFa4 = dev(Fa4), *dFb4 = dev(Fb4), *dFc4 = dev(Fc4), *dFd4 = dev(Fd4), *dG4 = dev(G4);
+    dt *dO4[4] = {dev(std::vector<dt>(P4 * R4, 0.f)), dev(std::vector<dt>(Q4 * R4, 0.f)),
+                  dev(std::vector<dt>(S4 * R4, 0.f)), dev(std::vector<dt>(T4 * R4, 0.f))};
+    dt *dGo4 = dev(std::vector<dt>(R4 * R4 * R4 * R4, 0.f));
+    dt errs2[4] = {-1.f, -1.f, -1.f, -1.f};
+    gencp4(dX4, dFa4, dFb4, dFc4, dFd4, P4, Q4, S4, T4, R4);
+    take("gencp4", dX4, N4);
+    cp_als4(dX4, dO4[0], dO4[1], dO4[2], dO4[3], P4, Q4, S4, T4, R4);
+    gencp4(drec4, dO4[0], dO4[1], dO4[2], dO4[3], P4, Q4, S4, T4, R4);
+    rse(dX4, drec4, N4, &errs2[0]);
+    if (tdx_last_status() != TDX_OK) {
+        fprintf(stderr, "cp_als4 chain: status %d\n", tdx_last_status());
+        return 4;
+    }
+    gencp(dXc, dFa, dFb, dFc, P, Q, S, RC);
+    cp_tensorcore(dXc, dOa, dOb, dOc, P, Q, S, RC);
+    gencp(drec, dOa, dOb, dOc, P, Q, S, RC);
+    rse(dXc, drec, P * Q * S, &errs2[1]);
+    if (tdx_last_status() != TDX_OK) {
+        fprintf(stderr, "cp_tensorcore chain: status %d\n", tdx_last_status());
+        return 4;
+    }
+    gentucker4(dX4, dG4, dFa4, dFb4, dFc4, dFd4, P4, Q4, S4, T4, R4, R4, R4, R4);
+    take("gentucker4", dX4, N4);
+    tucker_hosvd4(dX4, dGo4, dO4[0], dO4[1], dO4[2], dO4[3], P4, Q4, S4, T4, R4, R4, R4, R4);
+    gentucker4(drec4, dGo4, dO4[0], dO4[1], dO4[2], dO4[3], P4, Q4, S4, T4, R4, R4, R4, R4);
+    rse(dX4, drec4, N4, &errs2[2]);
+    if (tdx_last_status() != TDX_OK) {
+        fprintf(stderr, "tucker_hosvd4 chain: status %d\n", tdx_last_status());
+        return 4;
+    }
+    gentucker(dXc, dG, dFa, dFb, dFc, P, Q, S, RT, RT, RT);
+    tucker_tensorcore(dXc, dG2, dT1, dT2, dT3, P, Q, S, RT, RT, RT);
+    gentucker(drec, dG2, dT1, dT2, dT3, P, Q, S, RT, RT, RT);
+    rse(dXc, drec, P * Q * S, &errs2[3]);
+    if (tdx_last_status() != TDX_OK) {
+        fprintf(stderr, "tucker_tensorcore chain: status %d\n", tdx_last_status());
+        return 4;
+    }
+    fwrite(errs2, sizeof(dt), 4, out_file);
     fclose(out_file);
+    if (hipFree(dh) != hipSuccess) return 3;
     for (dt *p : {dX, dA, dA2, dA3, dB, dC, dU[0], dU[1], dU[2], dout, dX1, dXc, drec, dFa, dFb, dFc, dOa, dOb,
-                  dOc, dG, dT1, dT2, dT3, dG2})
+                  dOc, dG, dT1, dT2, dT3, dG2, dXf, dL, dRt, dX4, drec4, dFa4, dFb4, dFc4, dFd4, dG4, dO4[0],
+                  dO4[1], dO4[2], dO4[3], dGo4})
         if (hipFree(p) != hipSuccess) return 3;
     printf("dropin-caller-ok\n");
     return 0;

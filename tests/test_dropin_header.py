@@ -93,3 +93,51 @@ def test_reference_caller_compiles_links_and_loads(tmp_path):
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stderr
     assert r.stdout.strip() == "dropin-ok"
+
+
+LIMITS = r"""
+#include <string.h>
+#include "cudecompose.h"
+
+/* Inputs beyond the documented limits (rank > 64 for the CP drivers, a mode longer than 128
+ * for the Tucker drivers) are rejected by argument validation before any device work, so the
+ * pointers are never dereferenced and this runs without a GPU. */
+int main() {
+    static float buf[16];
+    dt *x = buf;
+    int bad = 0;
+    cp_als(x, x, x, x, 100, 100, 100, 65);
+    bad |= tdx_last_status() != TDX_ERR_UNSUPPORTED;
+    cp_als4(x, x, x, x, x, 10, 10, 10, 10, 65);
+    bad |= (tdx_last_status() != TDX_ERR_UNSUPPORTED) << 1;
+    cp_tensorcore(x, x, x, x, 100, 100, 100, 65);
+    bad |= (tdx_last_status() != TDX_ERR_UNSUPPORTED) << 2;
+    tucker_hosvd(x, x, x, x, x, 129, 64, 64, 8, 8, 8);
+    bad |= (tdx_last_status() != TDX_ERR_UNSUPPORTED) << 3;
+    tucker_tensorcore(x, x, x, x, x, 64, 129, 64, 8, 8, 8);
+    bad |= (tdx_last_status() != TDX_ERR_UNSUPPORTED) << 4;
+    tucker_hosvd4(x, x, x, x, x, x, 16, 16, 16, 129, 4, 4, 4, 4);
+    bad |= (tdx_last_status() != TDX_ERR_UNSUPPORTED) << 5;
+    return bad;
+}
+"""
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs a host C++ compiler")
+def test_limits_rejected_and_printed(tmp_path):
+    """rank 65 (CP) and a 129-long mode (Tucker) return TDX_ERR_UNSUPPORTED through the void
+    wrappers, which print the status to stderr and leave the outputs unwritten."""
+    src = tmp_path / "limits.cpp"
+    exe = tmp_path / "limits"
+    src.write_text(LIMITS)
+    libdir = os.path.dirname(tdx.LIB_PATH)
+    cmd = ["g++", "-std=c++11", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+           str(src), "-o", str(exe), "-L", libdir, "-ltdx", "-Wl,-rpath," + libdir]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, (r.returncode, r.stderr)
+    lines = [ln for ln in r.stderr.splitlines() if ln.startswith("cudecompose: ")]
+    names = [ln.split(":")[1].strip() for ln in lines]
+    assert names == ["cp_als", "cp_als4", "cp_tensorcore", "tucker_hosvd", "tucker_tensorcore", "tucker_hosvd4"], r.stderr
+    assert all("status %d" % tdx.TDX_ERR_UNSUPPORTED in ln for ln in lines)

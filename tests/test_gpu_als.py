@@ -233,3 +233,22 @@ def test_reference_entry_points():
     tdx.gencp(rec, *Fd, *dims, R)
     ref = O.cp_reconstruct(dims, [f.astype(np.float64) for f in F])
     assert relnorm(host(rec), ref) < 1e-6
+
+
+def test_nonfinite_tensor_reports_status():
+    """A NaN in X (and an Inf) makes the fit non-finite: the driver returns TDX_ERR_NONFINITE
+    instead of a silently wrong factorisation (the reference ignores every status,
+    src/mttkrp.cu:75-77)."""
+    dims, R = (32, 24, 16), 4
+    for bad in (float("nan"), float("inf")):
+        X, _ = make_cp_tensor(dims, R)
+        Xh = host(X)
+        Xh[1234] = bad
+        U0 = init_factors(dims, R, seed=9)
+        Ud = [dev(u.ravel(order="F")) for u in U0]
+        o = tdx.default_cp_options()
+        o.max_sweeps, o.init, o.record_fit = 3, 1, 1
+        with pytest.raises(tdx.TdxError) as e:
+            tdx.cp_als_ex(dev(Xh), dims, R, Ud, o)
+        assert e.value.status == tdx.TDX_ERR_NONFINITE
+        assert tdx.lib().tdx_last_status() == tdx.TDX_ERR_NONFINITE

@@ -65,7 +65,31 @@ def test_cpp_caller_through_cudecompose_h(tmp_path):
     # column congruence <= 0.46): cp_als and tucker_hosvd recover them, so the rebuilt
     # tensor's relative error is at FP32 rounding level
     e_cp, e_tucker = take(2)
-    assert pos == buf.size
     print("cp_als relerr", e_cp, "tucker_hosvd relerr", e_tucker)
     assert 0 <= e_cp < 1e-4
     assert 0 <= e_tucker < 1e-5
+    # f2h (src/float2half.cu:14-18): IEEE binary16 RNE bit patterns
+    Xf = take(M * N * K)
+    assert np.array_equal(take(M * N * K).astype(np.uint16), O.f16_bits(Xf))
+    # tminust (src/tminust.cu:10-15): right := left - right; initMat (src/tminust.cu:31-36)
+    assert np.array_equal(take(M * R), O.tminust(A, A2))
+    assert np.array_equal(take(36), O.identity(6))
+    # gencp4 / gentucker4 (src/rse.cu:53-96) on integer factors and core: exact
+    P4, Q4, S4, T4, R4 = 7, 6, 5, 4, 2
+    d4 = (P4, Q4, S4, T4)
+    F4 = [take(d * R4).reshape((d, R4), order="F").astype(np.float64) for d in d4]
+    G4 = take(R4 ** 4).astype(np.float64)
+    assert np.array_equal(take(P4 * Q4 * S4 * T4), O.cp_reconstruct(d4, F4).astype(np.float32))
+    rec_t4 = take(P4 * Q4 * S4 * T4)
+    assert np.array_equal(rec_t4, O.tucker_reconstruct(d4, (R4,) * 4, G4, F4).astype(np.float32))
+    e_cp4, e_cptc, e_tucker4, e_tuckertc = take(4)
+    assert pos == buf.size
+    print("cp_als4", e_cp4, "cp_tensorcore", e_cptc, "tucker_hosvd4", e_tucker4, "tucker_tensorcore", e_tuckertc)
+    # exact low-rank recovery: rank-2 order-4 CP (500 FP32 sweeps), rank-2 order-3 CP with BF16
+    # storage (integer entries <= 2^8 are exact in BF16) and 2 FP32 refinement sweeps,
+    # multilinear rank (2,2,2,2); tucker_tensorcore takes its subspaces from the BF16 copy
+    # (entries up to ~2^11 round at 2^-9 relative), so its error is that rounding's first order
+    assert 0 <= e_cp4 < 1e-4
+    assert 0 <= e_cptc < 1e-4
+    assert 0 <= e_tucker4 < 1e-5
+    assert 0 <= e_tuckertc < 1e-2

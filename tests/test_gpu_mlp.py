@@ -82,3 +82,52 @@ def test_training_fits_a_smooth_target():
     y = empty(ns)
     tdx.mlp_forward(p, dev(x.ravel()), ns, NIN, WIDTH, NOUT, y)
     assert np.mean((host(y) - t.ravel()) ** 2) < 0.02
+
+
+# GELU(x) = x Phi(x) and GELU(GELU(x)) at 40 digits (mpmath), the same known answers the
+# oracle is pinned by in tests/test_oracle.py
+GELU_KAT = [(0.0, 0.0), (1.0, 0.84134474606854294859), (-1.0, -0.15865525393145705141),
+            (2.0, 1.9544997361036415856), (-3.0, -0.00404969409489028358), (0.5, 0.34573123063700655182)]
+GELU_GELU_KAT = [(1.0, 0.67301066416940404567), (-1.0, -0.069327625476641394937), (2.0, 1.9050097055649646399)]
+
+
+def _packed(W1, b1, W2, b2, W3, b3, nout):
+    assert W1.shape == (WIDTH, NIN) and W3.shape == (nout, WIDTH)
+    return np.concatenate([W1.ravel(), b1, W2.ravel(), b2, W3.ravel(), b3]).astype(np.float32)
+
+
+def test_forward_gelu_known_answers():
+    """Selector weights isolate single GELUs on the device: W2 = 0, b2[k] = x_k and W3 = the
+    first rows of the identity give y_k = GELU(x_k); unit diagonal weights give GELU(GELU(x_k))."""
+    nout = len(GELU_KAT)
+    sel = np.zeros((nout, WIDTH))
+    sel[np.arange(nout), np.arange(nout)] = 1.0
+    b2 = np.zeros(WIDTH)
+    b2[:nout] = [x for x, _ in GELU_KAT]
+    flat = _packed(np.zeros((WIDTH, NIN)), np.zeros(WIDTH), np.zeros((WIDTH, WIDTH)), b2, sel, np.zeros(nout), nout)
+    y = empty(nout)
+    tdx.mlp_forward(dev(flat), dev(np.zeros(NIN, np.float32)), 1, NIN, WIDTH, nout, y)
+    assert np.allclose(host(y), [v for _, v in GELU_KAT], rtol=1e-6, atol=1e-7), host(y)
+    n2 = len(GELU_GELU_KAT)
+    W1 = np.zeros((WIDTH, NIN))
+    W1[np.arange(n2), np.arange(n2)] = 1.0
+    W2 = np.eye(WIDTH)
+    flat = _packed(W1, np.zeros(WIDTH), W2, np.zeros(WIDTH), sel[:n2], np.zeros(n2), n2)
+    x = np.zeros(NIN, np.float32)
+    x[:n2] = [v for v, _ in GELU_GELU_KAT]
+    y = empty(n2)
+    tdx.mlp_forward(dev(flat), dev(x), 1, NIN, WIDTH, n2, y)
+    assert np.allclose(host(y), [v for _, v in GELU_GELU_KAT], rtol=1e-6, atol=1e-7), host(y)
+
+
+def test_adam_two_step_known_answer():
+    """Kingma & Ba (2015): lr 0.1, p0 = 1, gradients 0.5 then -0.25 -> 0.900000002, then
+    0.87336629870784616 (40-digit mpmath values; FP32 on the device)."""
+    p, m, v = dev(np.ones(4, np.float32)), empty(4), empty(4)
+    m.zero_()
+    v.zero_()
+    tdx.mlp_adam(p, dev(np.full(4, 0.5, np.float32)), m, v, 1, 0.1)
+    assert np.allclose(host(p), 0.90000000199999996, rtol=0, atol=1e-7)
+    tdx.mlp_adam(p, dev(np.full(4, -0.25, np.float32)), m, v, 2, 0.1)
+    assert np.allclose(host(p), 0.87336629870784616256, rtol=0, atol=2e-7)
+    assert np.allclose(host(m), 0.02, rtol=1e-6) and np.allclose(host(v), 3.1225e-4, rtol=1e-6)

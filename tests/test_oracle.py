@@ -341,3 +341,95 @@ def test_adam_matches_paper():
         v_ref = 0.999 * v_ref + 0.001 * g * g
         p_ref = p_ref - 1e-2 * (m_ref / (1 - 0.9 ** t)) / (np.sqrt(v_ref / (1 - 0.999 ** t)) + 1e-8)
     assert np.allclose(p, p_ref, rtol=1e-13)
+
+
+# ---------------------------------------------------------------- MLP known answers
+# GELU(x) = x Phi(x) (Hendrycks & Gimpel 2016) and its derivative Phi(x) + x phi(x), computed
+# independently at 40 digits with mpmath:  x * (1 + mpmath.erf(x / mpmath.sqrt(2))) / 2
+GELU_KAT = {0.0: 0.0, 1.0: 0.84134474606854294859, -1.0: -0.15865525393145705141,
+            2.0: 1.9544997361036415856, -3.0: -0.00404969409489028358, 0.5: 0.34573123063700655182}
+GELU_D_KAT = {0.0: 0.5, 1.0: 1.0833154705876862984, -1.0: -0.083315470587686298383,
+              2.0: 1.0852318010781968967, -3.0: -0.011945647204183927, 0.5: 0.86749512465616284253}
+GELU_GELU_KAT = {1.0: 0.67301066416940404567, -1.0: -0.069327625476641394937, 2.0: 1.9050097055649646399}
+
+
+def _scalar_mlp(w1, b1, w2, b2, w3=1.0, b3=0.0):
+    return [np.array([[w1]]), np.array([b1]), np.array([[w2]]), np.array([b2]), np.array([[w3]]), np.array([b3])]
+
+
+@pytest.mark.parametrize("x", sorted(GELU_KAT))
+def test_mlp_gelu_known_answers(x):
+    """Width-1 networks isolate one GELU: with W2 = 0 and b2 = x the output is GELU(x); with
+    unit weights it is GELU(GELU(x)).  A wrong constant, sign or the tanh approximation fails."""
+    y = O.mlp_forward(np.zeros((1, 1)), _scalar_mlp(0.0, 0.0, 0.0, x), 1, 1, 1)
+    assert abs(y[0, 0] - GELU_KAT[x]) <= 1e-15 * max(1.0, abs(GELU_KAT[x]))
+    if x in GELU_GELU_KAT:
+        y = O.mlp_forward(np.array([[x]]), _scalar_mlp(1.0, 0.0, 1.0, 0.0), 1, 1, 1)
+        assert abs(y[0, 0] - GELU_GELU_KAT[x]) <= 1e-15
+
+
+@pytest.mark.parametrize("x", sorted(GELU_D_KAT))
+def test_mlp_gelu_derivative_known_answers(x):
+    """loss = (y - 0)^2 / 2 with y = GELU(b2): dloss/db2 = GELU(x) GELU'(x)."""
+    loss, g = O.mlp_backward(np.zeros((1, 1)), np.zeros((1, 1)), _scalar_mlp(0.0, 0.0, 0.0, x), 1, 1, 1)
+    assert abs(loss - 0.5 * GELU_KAT[x] ** 2) <= 1e-15
+    assert abs(g[3][0] - GELU_KAT[x] * GELU_D_KAT[x]) <= 1e-15
+
+
+def test_adam_two_step_known_answer():
+    """Kingma & Ba (2015), Algorithm 1, lr 0.1, beta 0.9/0.999, eps 1e-8, p0 = 1, gradients 0.5
+    then -0.25; expected values computed at 40 digits with mpmath (m2 = 0.02, v2 = 3.1225e-4)."""
+    p, m, v = np.array([1.0]), np.zeros(1), np.zeros(1)
+    O.adam(p, np.array([0.5]), m, v, 1, 0.1)
+    assert abs(p[0] - 0.90000000199999996) <= 1e-15
+    assert abs(m[0] - 0.05) <= 1e-16 and abs(v[0] - 2.5e-4) <= 1e-18
+    O.adam(p, np.array([-0.25]), m, v, 2, 0.1)
+    assert abs(p[0] - 0.87336629870784616256) <= 1e-15
+    assert abs(m[0] - 0.02) <= 1e-16 and abs(v[0] - 3.1225e-4) <= 1e-18
+
+
+def test_adam_first_step_is_lr_sign():
+    """Bias correction makes the first step lr g / (|g| + eps) for every g (m1/c1 = g, v1/c2 = g^2)."""
+    g = np.array([3.0, -2e-3, 0.75, -40.0, 1e-6])
+    p = np.zeros(5)
+    O.adam(p, g, np.zeros(5), np.zeros(5), 1, 0.01)
+    assert np.allclose(p, -0.01 * g / (np.abs(g) + 1e-8), rtol=1e-14, atol=0)
+
+
+def test_cp_als3_gemm_matches_loop_oracle():
+    """The GEMM formulation (decomposition/CPU/python/mycp.py's MTTKRP as unfolding x Khatri-Rao)
+    gives the same FP64 iterates as orc_cp_als; its FP32 variant stays within 1e-5."""
+    dims, R = (40, 36, 30), 5
+    F = [O.uniform(d * R, 0, m).reshape((d, R), order="F").astype(np.float64) for m, d in enumerate(dims)]
+    X0 = O.cp_reconstruct(dims, F)
+    E = O.normal(X0.size, 2, 0).astype(np.float64)
+    X = (X0 + 0.01 * np.linalg.norm(X0) / np.linalg.norm(E) * E).astype(np.float32)
+    U0 = [O.uniform(d * R, 1, m).reshape((d, R), order="F").astype(np.float64) for m, d in enumerate(dims)]
+    Uc, fc, dc, st = O.cp_als(X, dims, U0, 15, direct=True)
+    Ug, fg, _ = O.cp_als3_gemm(X, dims, U0, 15)
+    assert st == 0
+    assert np.max(np.abs(dc - fg)) < 1e-12
+    for a, b in zip(Uc, Ug):
+        assert np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(a))
+    _, f32, _ = O.cp_als3_gemm(X, dims, U0, 15, dtype=np.float32)
+    assert np.max(np.abs(f32 - dc)) < 1e-5
+    _, fs, _ = O.cp_als3_gemm(X, dims, U0, 15, stop_at=fg[4])
+    assert len(fs) == 5
+
+
+@pytest.mark.parametrize("dims,ranks", [((20, 16, 12), (4, 3, 5)), ((12, 10, 8, 6), (3, 4, 2, 3))])
+def test_tucker_hosvd_gemm_matches_loop_oracle(dims, ranks):
+    """The GEMM/LAPACK HOSVD used at 128^4 gives the loop oracle's factors (same ascending order
+    and sign convention) and core, orders 3 and 4."""
+    rng = np.random.default_rng(5)
+    core = rng.standard_normal(int(np.prod(ranks)))
+    Us = [np.linalg.qr(rng.standard_normal((d, r)))[0] for d, r in zip(dims, ranks)]
+    X = O.tucker_reconstruct(dims, ranks, core, Us)
+    E = rng.standard_normal(X.size)
+    X = (X + 1e-2 * np.linalg.norm(X) / np.linalg.norm(E) * E).astype(np.float32)
+    Uo, co = O.tucker_hosvd(X, dims, ranks)
+    Ug, cg, w = O.tucker_hosvd_gemm(X, dims, ranks)
+    for a, b in zip(Uo, Ug):
+        assert np.max(np.abs(a - b)) < 1e-12
+    assert np.max(np.abs(co - cg)) < 1e-12 * np.max(np.abs(co))
+    assert all(np.all(np.diff(x) >= 0) for x in w)
