@@ -1,4 +1,4 @@
-"""Headline benchmark: CP-ALS seconds to relative error within 1e-4 of the accurate solution
+"""Headline benchmark: CP-ALS seconds to a relative error within 1e-4 of the FP64 oracle's
 (mixed precision vs plain FP32) and fused-MTTKRP HBM GB/s, on one MI355X.
 
 Workload (dense 1024 x 1024 x 1024 tensor, rank 64): X = [[A,B,C]] + 0.1% Gaussian noise,
@@ -6,17 +6,29 @@ factor entries N(0,1) with columns normalised to unit 2-norm (Philox seed 0), no
 seed 2, initial factors N(0,1) from each of the --seeds.  The tensor is generated on the
 device and is resident in HBM before any timed region.
 
+Accuracy anchor: tests/golden/c2_fp64_anchor.json holds, per init seed, the FP64 oracle's
+relative error after 27 ALS sweeps from the same init on this exact tensor (written on the
+GPU host by tests/golden/make_c2_anchor.py; the bench checks the tensor's fingerprint and
+re-runs the oracle live on the first seed).  A path reaches its target at the first sweep
+whose relative error is <= anchor + 1e-4 (and its final direct error is too).
+
  * plain FP32 path: FP32 tensor, FP32-accumulating MFMA MTTKRP, FP64 solve, 27 sweeps.
-   Its final direct (compensated FP64) relative error is the accuracy reference.
- * mixed path (the value): BF16 tensor on MFMA with FP32 accumulation for --steps sweeps,
-   then 2 FP32 refinement sweeps.  value = median over the init seeds of the device seconds
-   until its relative error is within 1e-4 of the reference.
+ * mixed path (the value): 25 main sweeps + 2 FP32 refinement sweeps, the main sweeps
+   streaming the BF16 copy of X after --opening FP32 sweeps (default 2: rounding the
+   Khatri-Rao operand in the first sweeps settles some runs on another plateau; FP32 ALS on
+   the BF16-rounded tensor does not).  The schedule without opening sweeps and the FP8-E4M3
+   storage are reported beside it.  Time to target = device time from the call's first work
+   (norm of X, the BF16 conversion of X, factor panels) to the end of the sweep that reaches
+   the target.  value = median over the init seeds (a seed that misses counts as infinite).
+ * ms_per_step: --warmup untimed, then exactly --steps timed BF16 sweeps (hipEvents).
  * MTTKRP per mode for FP32 / BF16 / FP8-E4M3 storage, and the two kernels of the shared
    pass that the BF16/FP8 sweep uses for modes 1 and 2 (hipEvents on the library stream).
    roofline: the longest kernel of the BF16 sweep, the shared pass over X.
- * cpu_baseline: the oracle's FP64 ALS (OpenMP) on the same tensor, one sweep timed on the
-   host cores and extrapolated to the plain path's sweep count to target; plus the 128^3
-   rank-16 configuration in full on the oracle beside the GPU (times and per-sweep errors).
+ * cpu_baseline: the oracle's FP64 ALS in GEMM form (the reference CPU path,
+   decomposition/CPU/python/mycp.py, on the host's BLAS threads) on the first seed for 27
+   sweeps: seconds to the same target; plus the FP32 GEMM ALS and the 128^3 rank-16
+   configuration in full on the loop oracle beside the GPU.
+ * other_configs: C1, C3, C4, C5 (learned extrapolation on held-out seeds).
 
 Prints ONE JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
 """
@@ -39,6 +51,8 @@ MTTKRP_BF16_TRAFFIC = 2171970286
 # the same for the shared pass over X (BF16 storage: X read, T written)
 SHARED_PASS_BF16_TRAFFIC = 2575748744
 TARGET_DELTA = 1e-4
+MAIN_SWEEPS, REFINE_SWEEPS, PLAIN_SWEEPS = 25, 2, 27
+ANCHOR_PATH = os.path.join(ROOT, "tests", "golden", "c2_fp64_anchor.json")
 
 
 def log(msg):
@@ -82,9 +96,54 @@ def kernels_only(args, X, dims, R, U):
     print(json.dumps({"kernels_only_ms": res, "copy_floats": 1 << 28}))
 
 
+def c5_leg(n_seeds, log):
+    """Learned extrapolation (96^4 rank 24, congruence 0.9, 0.1% noise) on held-out seeds
+    2000..: device seconds (from the call's first work) to relerr <= eta + 1e-4 for plain ALS,
+    the cpd_lsb rule and the committed MLP weights, all from the same init, 400 sweeps at most."""
+    import numpy as np
+    import torch
+
+    import extrapolation as E
+    import tdx
+    dims, R, eta = (96, 96, 96, 96), 24, 1e-3
+    target = eta + TARGET_DELTA
+    P, meta = E.load(os.path.join(ROOT, "first-project_amd", "extrapolation_mlp"))
+    X = torch.empty(int(np.prod(dims)), device="cuda")
+    rows = []
+    for sd in range(2000, 2000 + n_seeds):
+        E.swamp_tensor(sd, dims, R, eta=eta, out=X)
+        row = {"seed": sd}
+        for name, mode in (("plain", tdx.LS_NONE), ("lsb", tdx.LS_LSB), ("learned", tdx.LS_LEARNED)):
+            U = E.init_factors(sd, dims, R)
+            o = tdx.default_cp_options()
+            o.max_sweeps, o.init, o.record_fit, o.line_search, o.target_relerr = 400, 1, 1, mode, target
+            if mode == tdx.LS_LEARNED:
+                o.mlp_params, o.mlp_width = P.data_ptr(), E.WIDTH
+            st, rep, fit, ms = tdx.cp_als_ex(X, dims, R, U, o, want_elapsed=True)
+            hit = np.nonzero(fit <= target)[0]
+            row[name] = {"sweeps": int(hit[0] + 1) if hit.size else None,
+                         "s": float(rep.elapsed_ms[hit[0]]) / 1e3 if hit.size else None}
+        rows.append(row)
+    out = {"workload": f"96^4 rank 24, pairwise column congruence 0.9, 0.1% noise; held-out seeds 2000..{1999 + n_seeds}; "
+                       f"target relerr <= {target:g}; at most 400 sweeps; MLP trained on seeds 0..{meta['train_seeds'] - 1}",
+           "per_seed": rows}
+    for name in ("plain", "lsb", "learned"):
+        v = [r[name]["s"] if r[name]["s"] is not None else np.inf for r in rows]
+        med = float(np.median(v))
+        out[name] = {"reached": int(np.sum(np.isfinite(v))), "n": len(v),
+                     "median_s": med if np.isfinite(med) else None,
+                     "median_sweeps": float(np.median([r[name]["sweeps"] or np.inf for r in rows]))}
+    pairs = [r["plain"]["s"] / r["learned"]["s"] for r in rows if r["plain"]["s"] and r["learned"]["s"]]
+    out["median_speedup_learned_vs_plain"] = float(np.median(pairs)) if pairs else None
+    log(f"C5: plain reached {out['plain']['reached']}/{n_seeds} median {out['plain']['median_s']}, learned "
+        f"{out['learned']['reached']}/{n_seeds} median {out['learned']['median_s']}")
+    return out
+
+
 def extras(args, log):
     """Secondary configurations: C1 (128^3 plain ALS, 50 sweeps), C3 (128^4 Tucker HOSVD + 10
-    HOOI), C4 (4096-tensor batched ALS, 30 sweeps, vs a per-tensor launch loop on a subsample)."""
+    HOOI), C4 (4096-tensor batched ALS, 30 sweeps, vs a per-tensor launch loop on a subsample),
+    C5 (learned extrapolation)."""
     import numpy as np
     import torch
 
@@ -123,40 +182,91 @@ def extras(args, log):
     os.makedirs(os.path.dirname(cache), exist_ok=True)
     best, times = tdx.cp_batched_autotune(X, U, b, trial_sweeps=2, cache_path=cache)
     st, fit, ms = tdx.cp_als_batched(X, U, b, 30, init=0, seed=1, variant=best)
+    # per-tensor loop baseline on a sample: the single-tensor driver per tensor, eager launches
+    # (a graph per tensor costs more to capture than 30 small sweeps run) and with the graph
     loop_n = 64
     idx = np.linspace(0, b.n - 1, loop_n).astype(int)
-    loop_s = 0.0
-    for t in idx:
-        dims = tuple(int(v) for v in b.shapes[t])
-        R = int(b.ranks[t])
-        Ut = [torch.empty(d * R, device="cuda") for d in dims]
-        xt = X[int(b.xoff[t]):int(b.xoff[t]) + int(np.prod(dims))]
-        oo = tdx.default_cp_options()
-        oo.max_sweeps, oo.record_fit = 30, 1
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        tdx.cp_als_ex(xt, dims, R, Ut, oo)
-        torch.cuda.synchronize()
-        loop_s += time.perf_counter() - t0
+    loop = {}
+    for use_graph in (0, 1):
+        tot = 0.0
+        for t in idx:
+            dims = tuple(int(v) for v in b.shapes[t])
+            R = int(b.ranks[t])
+            Ut = [torch.empty(d * R, device="cuda") for d in dims]
+            xt = X[int(b.xoff[t]):int(b.xoff[t]) + int(np.prod(dims))]
+            oo = tdx.default_cp_options()
+            oo.max_sweeps, oo.record_fit, oo.use_graph = 30, 1, use_graph
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            tdx.cp_als_ex(xt, dims, R, Ut, oo)
+            torch.cuda.synchronize()
+            tot += time.perf_counter() - t0
+        loop["graph" if use_graph else "eager"] = round(tot * b.n / loop_n, 3)
+    bytes_sweep = 3 * 4 * b.x_floats
     out["c4"] = {"tensors": b.n, "elements": b.x_floats, "sweeps": 30, "batched_s": round(float(np.sum(ms)) / 1e3, 5),
                  "batched_ms_per_sweep": round(float(np.mean(ms)), 4), "variant": best, "autotune_ms": times,
-                 "batched_GBps_per_sweep": round(3 * 4 * b.x_floats / (float(np.mean(ms)) * 1e-3) / 1e9, 1),
-                 "per_tensor_loop_s_extrapolated": round(loop_s * b.n / loop_n, 3),
-                 "per_tensor_loop_sample": loop_n,
+                 "batched_GBps_per_sweep": round(bytes_sweep / (float(np.mean(ms)) * 1e-3) / 1e9, 1),
+                 "hbm_frac": round(bytes_sweep / (float(np.mean(ms)) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                 "per_tensor_loop_s_extrapolated": loop, "per_tensor_loop_sample": loop_n,
                  "median_relerr_final": float(np.median(fit[:, -1]))}
     del X, U
+    torch.cuda.empty_cache()
+    if args.c5_seeds > 0:
+        log(f"C5: learned extrapolation, {args.c5_seeds} held-out seeds")
+        out["c5"] = c5_leg(args.c5_seeds, log)
     return out
+
+
+def cpu_leg(X, dims, R, init_host, anchor, seed, log):
+    """The reference CPU path restated (oracle.cp_als3_gemm: Hadamard of Grams, MTTKRP as
+    unfolding x Khatri-Rao GEMMs, Cholesky solves) on this tensor from the seed's init, FP64
+    for 27 sweeps (its trajectory must equal the committed anchor's) and FP32."""
+    import numpy as np
+
+    import oracle as O
+    threads = os.environ.get("OMP_NUM_THREADS") or os.environ.get("OPENBLAS_NUM_THREADS")
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), None)
+    except OSError:
+        pass
+    Xh = X.cpu().numpy()
+    target = anchor[-1] + TARGET_DELTA
+    log(f"CPU baseline: FP64 GEMM ALS, {PLAIN_SWEEPS} sweeps, seed {seed}")
+    t0 = time.perf_counter()
+    X64 = Xh.astype(np.float64)
+    conv = time.perf_counter() - t0
+    _, f64, s64 = O.cp_als3_gemm(X64, dims, init_host, PLAIN_SWEEPS)
+    del X64
+    hit = np.nonzero(f64 <= target)[0]
+    t64 = float(np.sum(s64[:hit[0] + 1])) if hit.size else None
+    log("CPU baseline: FP32 GEMM ALS")
+    _, f32, s32 = O.cp_als3_gemm(Xh, dims, init_host, PLAIN_SWEEPS, dtype=np.float32)
+    hit32 = np.nonzero(f32 <= target)[0]
+    t32 = float(np.sum(s32[:hit32[0] + 1])) if hit32.size else None
+    cores = int(threads) if threads and threads.isdigit() else (os.cpu_count() or 1)
+    return {"value": round(t64, 3) if t64 is not None else None, "unit": "s", "cores": cores, "kind": "port",
+            "sample": f"seed {seed}: FP64 ALS (GEMM form of decomposition/CPU/python/mycp.py) on the {dims[0]}^3 rank-{R} "
+                      f"tensor, {PLAIN_SWEEPS} sweeps ({float(np.sum(s64)):.2f} s, {float(np.mean(s64)):.3f} s/sweep, "
+                      f"FP32->FP64 copy {conv:.2f} s not counted); value = seconds to the sweep reaching the target",
+            "fp64_sweeps_to_target": int(hit[0] + 1) if hit.size else None,
+            "fp64_matches_anchor_max_abs": float(np.max(np.abs(f64 - np.asarray(anchor)))),
+            "fp32": {"s_to_target": round(t32, 3) if t32 is not None else None,
+                     "sweeps_to_target": int(hit32[0] + 1) if hit32.size else None,
+                     "s_per_sweep": round(float(np.mean(s32)), 4), "relerr_final": float(f32[-1])},
+            "cpu_model": model, "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "nproc": os.cpu_count()}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=25, help="BF16 sweeps of the mixed path")
-    ap.add_argument("--warmup", type=int, default=2, help="untimed warm-up sweeps per path")
-    ap.add_argument("--size", type=int, default=1024)
+    ap.add_argument("--steps", type=int, default=25, help="timed BF16 sweeps of the ms_per_step loop")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed sweeps before every timed loop")
     ap.add_argument("--seeds", default="1,2,3,4,5,6,7,8,9", help="initialisation seeds of the C2 runs")
-    ap.add_argument("--rank", type=int, default=64)
+    ap.add_argument("--opening", type=int, default=2, help="FP32 sweeps opening the mixed path's 25 main sweeps")
     ap.add_argument("--mttkrp-iters", type=int, default=10)
+    ap.add_argument("--c5-seeds", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the secondary configurations")
     ap.add_argument("--kernels-only", action="store_true",
@@ -171,118 +281,99 @@ def main():
     import torch
 
     import tdx
+    import workloads as W
 
-    n, R = args.size, args.rank
-    dims = (n, n, n)
-    N = n ** 3
-    dev = torch.device("cuda:0")
     tdx.lib()
     tdx.lib().tdx_init()
+    X, dims, R = W.c2()
+    n = dims[0]
+    N = X.numel()
+    anchor = json.load(open(ANCHOR_PATH))
+    fp = anchor["tensor"]
+    nx2 = float(torch.dot(X.double(), X.double()))
+    same = abs(nx2 - fp["normx2"]) <= 1e-12 * fp["normx2"] and all(
+        float(X[i]) == v for i, v in zip(fp["sample_index"], fp["sample"]))
+    if not same:
+        log("the generated tensor differs from the anchor's: targets unavailable")
 
-    # --- synthetic inputs (Philox streams of the library; identical to the oracle's) ---
-    def normal_cols(seed, m, d):
-        t = torch.empty(d * R, dtype=torch.float32, device=dev)
-        tdx.fill_normal(t, seed, m)
-        M = t.view(R, d).t()  # column-major d x R
-        M = M / M.norm(dim=0, keepdim=True)
-        return M.t().contiguous().view(-1)
-
-    F = [normal_cols(0, m, d) for m, d in enumerate(dims)]
-    X = torch.empty(N, dtype=torch.float32, device=dev)
-    tdx.gen_cp_tensor(X, dims, F, R, 1e-3, 2, 0)
-
-    def init(seed=1):
-        # N(0,1) initial factors (a U(0,1) start stalls on these mixed-sign factors)
-        us = []
-        for m, d in enumerate(dims):
-            t = torch.empty(d * R, dtype=torch.float32, device=dev)
-            tdx.fill_normal(t, seed, m)
-            us.append(t)
-        return us
-
-    def run(storage, sweeps, refine, seed, want_ms=True):
-        U = init(seed)
+    def run(storage, sweeps, refine, seed, opening=0):
+        U = W.normal_init(dims, R, seed)
         o = tdx.default_cp_options()
-        o.max_sweeps, o.refine_sweeps, o.storage = sweeps, refine, storage
+        o.max_sweeps, o.refine_sweeps, o.storage, o.opening_sweeps = sweeps, refine, storage, opening
         o.init, o.record_fit, o.use_graph = 1, 1, 1
-        st, rep, fit, ms = tdx.cp_als_ex(X, dims, R, U, o, want_ms=want_ms)
-        return st, rep, np.asarray(fit), np.asarray(ms, np.float64), U
+        st, rep, fit, ms = tdx.cp_als_ex(X, dims, R, U, o, want_ms=True, want_elapsed=True)
+        return rep, np.asarray(fit), np.asarray(ms, np.float64), U
 
     if args.kernels_only:
-        kernels_only(args, X, dims, R, init())
+        kernels_only(args, X, dims, R, W.normal_init(dims, R, 1))
         return
 
-    # warm-up (also instantiates graphs, workspaces)
+    # warm-up: code objects, graphs, workspaces (the BF16/FP8 copies are pooled)
     log("warm-up")
-    if args.warmup > 0:
-        run(tdx.TDX_F32, args.warmup, 0, 1)
-        run(tdx.TDX_BF16, args.warmup, 1, 1)
-        run(tdx.TDX_FP8_E4M3, args.warmup, 1, 1)
+    w = max(args.warmup, 1)
+    run(tdx.TDX_F32, w, 0, 1)
+    run(tdx.TDX_BF16, w, 1, 1, opening=1)
+    run(tdx.TDX_FP8_E4M3, w, 1, 1, opening=1)
     torch.cuda.synchronize()
 
-    def time_to_target(fit, ms, target):
-        cum = np.cumsum(ms) / 1e3
+    # --- the timed step loop: exactly --steps BF16 sweeps after --warmup untimed ones ---
+    log(f"step loop: {args.warmup} + {args.steps} BF16 sweeps")
+    rep, fit, ms, _ = run(tdx.TDX_BF16, args.warmup + args.steps, 0, 1)
+    step_ms = ms[args.warmup:]
+    assert step_ms.size == args.steps
+
+    def time_to_target(rep, fit, target, final):
         hit = np.nonzero(fit <= target)[0]
-        if hit.size == 0:
+        if hit.size == 0 or final > target:
             return None, None
-        return float(cum[hit[0]]), int(hit[0] + 1)
+        return float(rep.elapsed_ms[hit[0]]) / 1e3, int(hit[0] + 1)
 
-    # --- per init seed: plain FP32 (27 sweeps) sets the target, then the mixed paths ---
-    # ALS on this tensor settles on plateaus whose error is quantised by the number of
-    # components it has not resolved (relerr^2 ~ k/64); which plateau a run reaches at sweep 27
-    # depends on the trajectory, so the value is the median over several initialisations.
     seeds = [int(v) for v in args.seeds.split(",")]
+    paths = {"fp32": (tdx.TDX_F32, PLAIN_SWEEPS, 0, 0),
+             "bf16": (tdx.TDX_BF16, MAIN_SWEEPS, REFINE_SWEEPS, args.opening),
+             "bf16_no_opening": (tdx.TDX_BF16, MAIN_SWEEPS, REFINE_SWEEPS, 0),
+             "fp8": (tdx.TDX_FP8_E4M3, MAIN_SWEEPS, REFINE_SWEEPS, args.opening),
+             "fp8_no_opening": (tdx.TDX_FP8_E4M3, MAIN_SWEEPS, REFINE_SWEEPS, 0)}
     per_seed = []
-    U32 = None
     for seed in seeds:
-        log(f"seed {seed}: plain FP32 27 sweeps, BF16 {args.steps}+2, FP8 {args.steps}+2")
-        torch.cuda.synchronize()
-        st32, rep32, fit32, ms32, U = run(tdx.TDX_F32, 27, 0, seed)
-        ref = tdx.cp_relerr(X, dims, U, R)
-        if U32 is None:
-            U32 = U
-        target = ref + TARGET_DELTA
-        t32, s32 = time_to_target(fit32, ms32, target)
-        stb, repb, fitb, msb, Ub = run(tdx.TDX_BF16, args.steps, 2, seed)
-        relb = tdx.cp_relerr(X, dims, Ub, R)
-        tb, sb = time_to_target(fitb, msb, target)
-        st8, rep8, fit8, ms8, U8 = run(tdx.TDX_FP8_E4M3, args.steps, 2, seed)
-        rel8 = tdx.cp_relerr(X, dims, U8, R)
-        t8, s8 = time_to_target(fit8, ms8, target)
-        per_seed.append({"seed": seed, "target": target,
-                         "fp32": {"s": t32, "sweeps": s32, "relerr": ref, "ms_sweep": float(ms32.mean())},
-                         "bf16": {"s": tb, "sweeps": sb, "relerr": relb, "ms_sweep": float(msb[:args.steps].mean())},
-                         "fp8": {"s": t8, "sweeps": s8, "relerr": rel8, "ms_sweep": float(ms8[:args.steps].mean())}})
-        log(f"  target {target:.6f}: fp32 {t32} s ({s32} sweeps), bf16 {tb} s ({sb}, relerr {relb:.6f}), "
-            f"fp8 {t8} s ({s8}, relerr {rel8:.6f})")
+        a = anchor["seeds"].get(str(seed)) if same else None
+        ref = a["relerr"][-1] if a else None
+        target = ref + TARGET_DELTA if a else None
+        row = {"seed": seed, "fp64_anchor": ref, "target": target}
+        for name, (storage, sweeps, refine, opening) in paths.items():
+            rep, fit, ms, U = run(storage, sweeps, refine, seed, opening)
+            rel = tdx.cp_relerr(X, dims, U, R)
+            t, k = time_to_target(rep, fit, target, rel) if a else (None, None)
+            row[name] = {"s": t, "sweeps": k, "relerr": rel, "diff_vs_fp64": (rel - ref) if a else None,
+                         "setup_ms": round(rep.setup_seconds * 1e3, 4), "total_s": round(rep.seconds, 6),
+                         "ms_sweep_main": round(float(np.mean(ms[opening:sweeps])), 4)}
+        per_seed.append(row)
+        log(f"seed {seed}: anchor {ref}: " + ", ".join(
+            f"{k} {row[k]['s'] and round(row[k]['s'], 5)} s ({row[k]['sweeps']}, {row[k]['relerr']:.6f})" for k in paths))
 
-    def median_time(key):
-        v = sorted((r[key]["s"] if r[key]["s"] is not None else float("inf")) for r in per_seed)
-        med = float(np.median(v))
-        return None if not np.isfinite(med) else med
+    def med(key):
+        v = [r[key]["s"] if r[key]["s"] is not None else np.inf for r in per_seed]
+        m = float(np.median(v))
+        return m if np.isfinite(m) else None
 
-    t32m, tbm, t8m = median_time("fp32"), median_time("bf16"), median_time("fp8")
+    medians = {k: med(k) for k in paths}
+    reached = {k: sum(1 for r in per_seed if r[k]["s"] is not None) for k in paths}
     speedups = sorted((r["fp32"]["s"] / r["bf16"]["s"]) if (r["fp32"]["s"] and r["bf16"]["s"]) else 0.0
                       for r in per_seed)
-    ms32_all = float(np.mean([r["fp32"]["ms_sweep"] for r in per_seed]))
-    msb_all = float(np.mean([r["bf16"]["ms_sweep"] for r in per_seed]))
-    ms8_all = float(np.mean([r["fp8"]["ms_sweep"] for r in per_seed]))
-    s32_med = float(np.median([r["fp32"]["sweeps"] or 27 for r in per_seed]))
+    ms_sweep = {k: round(float(np.mean([r[k]["ms_sweep_main"] for r in per_seed])), 4) for k in paths}
+    U1 = W.normal_init(dims, R, 1)
 
     # --- MTTKRP per mode and storage ---
     log("MTTKRP micro-benchmarks")
-    Xb = torch.empty(N, dtype=torch.int16, device=dev)
+    Xb = torch.empty(N, dtype=torch.int16, device="cuda")
     tdx.quantize_bf16(X, Xb)
-    Xq = torch.empty(N, dtype=torch.uint8, device=dev)
-    sc = torch.empty(n, dtype=torch.float32, device=dev)
+    Xq = torch.empty(N, dtype=torch.uint8, device="cuda")
+    sc = torch.empty(n, dtype=torch.float32, device="cuda")
     tdx.quantize_fp8_mode1(X, Xq, sc, n)
     mt = {}
     for name, Xs, dt, s_bytes, scales in (("fp32", X, tdx.TDX_F32, 4, None), ("bf16", Xb, tdx.TDX_BF16, 2, None),
                                           ("fp8", Xq, tdx.TDX_FP8_E4M3, 1, sc)):
-        per = []
-        for mode in range(3):
-            ms = tdx.bench_mttkrp(Xs, dt, scales, dims, U32, R, mode, args.mttkrp_iters)
-            per.append(ms)
+        per = [tdx.bench_mttkrp(Xs, dt, scales, dims, U1, R, mode, args.mttkrp_iters) for mode in range(3)]
         alg_bytes = s_bytes * N + 4 * R * (2 * n) + 4 * n * R
         flops = 2.0 * N * R
         avg = float(np.mean(per))
@@ -290,9 +381,7 @@ def main():
                     "TFLOPs": round(flops / (avg * 1e-3) / 1e12, 2),
                     "hbm_frac": round(alg_bytes / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
         if dt != tdx.TDX_F32:
-            # shared pass of the order-3 sweep (modes 1 and 2): the pass over X writes T = X x_0 A^T
-            # (n^2 Rp FP32) and the mode-1 partials, the pass over T gives the mode-2 partials
-            ms_x, ms_t = tdx.bench_mttkrp_dtree(Xs, dt, scales, dims, U32, R, args.mttkrp_iters)
+            ms_x, ms_t = tdx.bench_mttkrp_dtree(Xs, dt, scales, dims, U1, R, args.mttkrp_iters)
             b_x, b_t = shared_pass_bytes(s_bytes, N, n, R)
             if name == "bf16":
                 mt[name]["traffic_per_launch"] = MTTKRP_BF16_TRAFFIC
@@ -301,58 +390,41 @@ def main():
                 "hbm_frac_pass_x": round(b_x / (ms_x * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "bytes_pass_x": b_x,
                 "ms_pass_t": round(ms_t, 4), "GBps_pass_t": round(b_t / (ms_t * 1e-3) / 1e9, 1), "bytes_pass_t": b_t}
     del Xb, Xq
+    torch.cuda.empty_cache()
 
-    # --- CPU baseline: oracle FP64 ALS sweep on the same tensor ---
     cpu = None
-    if not args.no_cpu:
-        log("CPU baseline (oracle FP64 ALS, one sweep)")
+    if not args.no_cpu and same:
         import oracle as O
-        Xh = X.cpu().numpy()
-        U0 = []
-        for m, d in enumerate(dims):
-            U0.append(O.normal(d * R, 1, m).reshape((d, R), order="F").astype(np.float64))
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        t0 = time.perf_counter()
-        O.cp_als(Xh, dims, U0, 1)
-        cpu_sweep = time.perf_counter() - t0
-        sweeps_needed = int(round(s32_med))
-        model = None
-        try:
-            with open("/proc/cpuinfo") as f:
-                model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), None)
-        except OSError:
-            pass
-        cpu = {"value": round(cpu_sweep * sweeps_needed, 3), "unit": "s", "cores": threads, "kind": "port",
-               "sample": f"1 FP64 ALS sweep (3 MTTKRPs + Cholesky solves) of the {n}^3 rank-{R} tensor on "
-                         f"{threads} OpenMP threads = {cpu_sweep:.2f} s, x {sweeps_needed} sweeps to target",
-               "cpu_model": model, "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "nproc": os.cpu_count()}
-        # C1 in full on the oracle beside the GPU (same tensor, same U(0,1) init from seed 1):
-        # 128^3 rank 16, 50 plain sweeps, per-sweep relative errors compared
-        import workloads as W
+        seed = seeds[0]
+        init_host = [u.cpu().numpy().reshape((d, R), order="F").astype(np.float64)
+                     for u, d in zip(W.normal_init(dims, R, seed), dims)]
+        cpu = cpu_leg(X, dims, R, init_host, anchor["seeds"][str(seed)]["relerr"], seed, log)
+        # C1 in full on the loop oracle beside the GPU (same tensor, same U(0,1) init from seed 1)
         X1, dims1, R1 = W.c1()
-        U1 = [O.uniform(d * R1, 1, m).reshape((d, R1), order="F") for m, d in enumerate(dims1)]
+        U1h = [O.uniform(d * R1, 1, m).reshape((d, R1), order="F") for m, d in enumerate(dims1)]
         t0 = time.perf_counter()
-        _, fit_c, _, _ = O.cp_als(X1.cpu().numpy(), dims1, [u.astype(np.float64) for u in U1], 50)
+        _, fit_c, _, _ = O.cp_als(X1.cpu().numpy(), dims1, [u.astype(np.float64) for u in U1h], 50)
         c1_cpu = time.perf_counter() - t0
         o1 = tdx.default_cp_options()
         o1.max_sweeps, o1.init, o1.record_fit = 50, 1, 1
-        Ud1 = [torch.from_numpy(np.ascontiguousarray(u.ravel(order="F"))).to(dev) for u in U1]
+        Ud1 = [torch.from_numpy(np.ascontiguousarray(u.ravel(order="F"))).cuda() for u in U1h]
         _, _, fit_g, ms_g = tdx.cp_als_ex(X1, dims1, R1, Ud1, o1, want_ms=True)
-        fit_g = np.asarray(fit_g, np.float64)
         cpu["c1"] = {"cpu_s": round(c1_cpu, 3), "gpu_s": round(float(np.sum(ms_g)) / 1e3, 5),
                      "relerr_final_cpu_fp64": float(fit_c[-1]), "relerr_final_gpu_fp32": float(fit_g[-1]),
-                     "max_abs_relerr_diff_over_sweeps": float(np.max(np.abs(fit_g - fit_c))),
+                     "max_abs_relerr_diff_over_sweeps": float(np.max(np.abs(np.asarray(fit_g) - fit_c))),
                      "workload": "128^3 rank 16, U(0,1) factors, 1% noise, U(0,1) init (seed 1), 50 sweeps"}
         del X1
 
+    del X
+    torch.cuda.empty_cache()
     extra = None if args.no_extra else extras(args, log)
 
-    # The BF16 sweep is: mode 0 by the fused MTTKRP, modes 1 and 2 by the shared pass; its
-    # longest kernel is the pass over X, the roofline kernel below.
     bf = mt["bf16"]
     sp = bf["shared_pass"]
     alg_m0 = 2 * N + 4 * R * 2 * n + 4 * n * R
     sweep_bytes = alg_m0 + sp["bytes_pass_x"] + sp["bytes_pass_t"]
+    msb = ms_sweep["bf16"]
+    tbm = medians["bf16"]
     out = {
         "metric": "CP-ALS sec to rel-err 1e-4 (mixed/learned vs plain FP32); MTTKRP HBM GB/s",
         "value": round(tbm, 5) if tbm is not None else None,
@@ -360,24 +432,27 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(msb_all, 4),
+        "ms_per_step": round(float(np.mean(step_ms)), 4),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "bf16",
         "data": "synthetic (Philox low-rank + 0.1% noise, generated in HBM)",
-        "config": {"workload": f"CP-ALS {n}x{n}x{n} rank {R}, BF16 storage {args.steps} sweeps + 2 FP32 refinement "
-                               f"sweeps; target = plain FP32 27-sweep rel-err + 1e-4; median over init seeds {seeds}",
-                   "dims": list(dims), "rank": R, "noise": 1e-3, "init_seeds": seeds},
-        "plain_fp32": {"median_sec_to_target": t32m, "ms_per_sweep": round(ms32_all, 4)},
-        "mixed_bf16": {"median_sec_to_target": tbm, "ms_per_sweep": round(msb_all, 4)},
-        "mixed_fp8": {"median_sec_to_target": t8m, "ms_per_sweep": round(ms8_all, 4)},
-        "speedup_vs_plain_median": round(float(np.median(speedups)), 3),
-        "seeds_reaching_target": {k: sum(1 for r in per_seed if r[k]["s"] is not None) for k in ("fp32", "bf16", "fp8")},
+        "config": {"workload": f"CP-ALS {n}x{n}x{n} rank {R}: {args.opening} FP32 + {MAIN_SWEEPS - args.opening} "
+                               f"BF16-storage sweeps + {REFINE_SWEEPS} FP32 refinement sweeps vs plain FP32 "
+                               f"{PLAIN_SWEEPS} sweeps; target = FP64-oracle relerr after {PLAIN_SWEEPS} sweeps from "
+                               f"the same init + 1e-4; timed from the first device work (norm, BF16 conversion, panels); "
+                               f"median over init seeds {seeds}",
+                   "dims": list(dims), "rank": R, "noise": 1e-3, "init_seeds": seeds, "opening_sweeps": args.opening},
+        "median_sec_to_target": medians,
+        "seeds_reaching_target": reached,
+        "speedup_bf16_vs_plain_median": round(float(np.median(speedups)), 3),
+        "speedup_of_medians": round(medians["fp32"] / tbm, 3) if (tbm and medians["fp32"]) else None,
+        "ms_per_sweep": ms_sweep,
         "per_seed": per_seed,
         "mttkrp": mt,
         "sweep_hbm": {"bytes_per_bf16_sweep": sweep_bytes,
-                      "GBps": round(sweep_bytes / (msb_all * 1e-3) / 1e9, 1),
+                      "GBps": round(sweep_bytes / (msb * 1e-3) / 1e9, 1),
                       "note": "algorithmic bytes of one BF16 sweep (fused mode-0 MTTKRP, shared pass over X, "
                               "pass over T) / measured ms per sweep (includes solves, Grams, fit)"},
         "roofline": {"bound": "hbm", "achieved": sp["GBps_pass_x"], "peak": HBM_PEAK_GBS,

@@ -968,9 +968,11 @@ static int capture_sweep(CpPlan &P, const Storage &S, hipStream_t s, bool fit, S
     return TDX_OK;
 }
 
-// Runs `sweeps` sweeps on storage S; records per-sweep ms if sweep_ms != NULL.
+// Runs `sweeps` sweeps on storage S; records per-sweep ms if sweep_ms != NULL, and the device
+// time from the event `origin` to the end of every sweep if elapsed_ms != NULL.
 static int run_sweeps(CpPlan &P, const Storage &S, hipStream_t s, int sweeps, bool fit, bool graph, float *sweep_ms,
-                      double tol, double target, int *done) {
+                      double tol, double target, int *done, hipEvent_t origin = nullptr,
+                      float *elapsed_ms = nullptr) {
     *done = 0;
     if (sweeps <= 0) return TDX_OK;
     SweepGraph G;
@@ -978,8 +980,9 @@ static int run_sweeps(CpPlan &P, const Storage &S, hipStream_t s, int sweeps, bo
         int st = capture_sweep(P, S, s, fit, G);
         if (st) return st;
     }
+    if (!origin) elapsed_ms = nullptr;
     std::vector<hipEvent_t> evs;
-    if (sweep_ms) {
+    if (sweep_ms || elapsed_ms) {
         evs.resize(sweeps + 1);
         for (auto &e : evs) TDX_HIP(hipEventCreate(&e));
         TDX_HIP(hipEventRecord(evs[0], s));
@@ -994,7 +997,7 @@ static int run_sweeps(CpPlan &P, const Storage &S, hipStream_t s, int sweeps, bo
             st = enqueue_sweep(P, S, s, fit);
             if (st) break;
         }
-        if (sweep_ms) {
+        if (!evs.empty()) {
             hipError_t e = hipEventRecord(evs[it + 1], s);
             if (e != hipSuccess) { st = record_hip(e, "hipEventRecord", __FILE__, __LINE__); break; }
         }
@@ -1011,9 +1014,12 @@ static int run_sweeps(CpPlan &P, const Storage &S, hipStream_t s, int sweeps, bo
             prev = f;
         }
     }
-    if (sweep_ms && st == TDX_OK) {
+    if (!evs.empty() && st == TDX_OK) {
         TDX_HIP(hipEventSynchronize(evs[*done]));
-        for (int it = 0; it < *done; ++it) TDX_HIP(hipEventElapsedTime(&sweep_ms[it], evs[it], evs[it + 1]));
+        for (int it = 0; it < *done; ++it) {
+            if (sweep_ms) TDX_HIP(hipEventElapsedTime(&sweep_ms[it], evs[it], evs[it + 1]));
+            if (elapsed_ms) TDX_HIP(hipEventElapsedTime(&elapsed_ms[it], origin, evs[it + 1]));
+        }
     }
     for (auto &e : evs) (void)hipEventDestroy(e);
     return st;
@@ -1048,7 +1054,7 @@ int tdx_cp_als_ex(const float *X, int order, const long *dims, long rank, float 
     tdx_cp_options opt;
     if (opt_in) opt = *opt_in;
     else tdx_cp_default_options(&opt);
-    if (opt.max_sweeps < 0 || opt.refine_sweeps < 0) return record(TDX_ERR_ARG);
+    if (opt.max_sweeps < 0 || opt.refine_sweeps < 0 || opt.opening_sweeps < 0) return record(TDX_ERR_ARG);
     if (opt.storage != TDX_F32 && opt.storage != TDX_BF16 && opt.storage != TDX_FP8_E4M3) return record(TDX_ERR_ARG);
     long N = 1;
     for (int m = 0; m < order; ++m) {
@@ -1135,18 +1141,30 @@ int tdx_cp_als_ex(const float *X, int order, const long *dims, long rank, float 
         if (!st) st = storage_setup(P, Smain, Xq, TDX_FP8_E4M3, scales);
     }
     if (st) return record(st);
-    int done_main = 0, done_ref = 0;
-    st = refresh_factors(P, Smain, s);
+    // BF16 / FP8 storage: the first `opening` of the max_sweeps main sweeps read the FP32 tensor
+    const int opening = (opt.storage == TDX_F32) ? 0 : std::min(opt.opening_sweeps, opt.max_sweeps);
+    float *elapsed = opt.trace ? opt.trace->elapsed_ms : nullptr;
+    int done_open = 0, done_main = 0, done_ref = 0;
+    st = refresh_factors(P, opening ? S32 : Smain, s);
     (void)hipEventRecord(evg.e[1], s);
+    if (!st && opening > 0) {
+        st = run_sweeps(P, S32, s, opening, opt.record_fit != 0, opt.use_graph != 0, sweep_ms, 0.0, 0.0, &done_open,
+                        evg.e[0], elapsed);
+        if (!st) st = refresh_factors(P, Smain, s);
+    }
     if (!st)
-        st = run_sweeps(P, Smain, s, opt.max_sweeps, opt.record_fit != 0, opt.use_graph != 0, sweep_ms, opt.tol,
-                        opt.storage == TDX_F32 ? opt.target_relerr : 0.0, &done_main);
+        st = run_sweeps(P, Smain, s, opt.max_sweeps - opening, opt.record_fit != 0, opt.use_graph != 0,
+                        sweep_ms ? sweep_ms + done_open : nullptr, opt.tol,
+                        opt.storage == TDX_F32 ? opt.target_relerr : 0.0, &done_main, evg.e[0],
+                        elapsed ? elapsed + done_open : nullptr);
+    done_main += done_open;
     if (!st && opt.storage != TDX_F32 && opt.refine_sweeps > 0) {
         // FP32 refinement: the mode-0 panel must be unscaled again (rebuilt by the first solve)
         st = refresh_factors(P, S32, s);
         if (!st)
             st = run_sweeps(P, S32, s, opt.refine_sweeps, opt.record_fit != 0, opt.use_graph != 0,
-                            sweep_ms ? sweep_ms + done_main : nullptr, 0.0, 0.0, &done_ref);
+                            sweep_ms ? sweep_ms + done_main : nullptr, 0.0, 0.0, &done_ref, evg.e[0],
+                            elapsed ? elapsed + done_main : nullptr);
     }
     (void)hipEventRecord(evg.e[2], s);
     hipError_t e = hipStreamSynchronize(s);

@@ -35,7 +35,8 @@ LS_NONE, LS_LEARNED, LS_LSB, LS_GRID = 0, 1, 2, 3
 
 
 class CpTrace(ctypes.Structure):
-    _fields_ = [("alpha", _P), ("accepted", _P), ("features", _P), ("grid_relerr", _P), ("plain_relerr", _P)]
+    _fields_ = [("alpha", _P), ("accepted", _P), ("features", _P), ("grid_relerr", _P), ("plain_relerr", _P),
+                ("elapsed_ms", _P)]
 
 
 class CpOptions(ctypes.Structure):
@@ -44,7 +45,7 @@ class CpOptions(ctypes.Structure):
                 ("record_fit", ctypes.c_int), ("line_search", ctypes.c_int), ("use_graph", ctypes.c_int),
                 ("mlp_params", _P), ("mlp_width", ctypes.c_int), ("ngrid", ctypes.c_int),
                 ("grid", ctypes.c_double * 8), ("trace", ctypes.POINTER(CpTrace)), ("target_relerr", ctypes.c_double),
-                ("shared_pass", ctypes.c_int)]
+                ("shared_pass", ctypes.c_int), ("opening_sweeps", ctypes.c_int)]
 
 
 class CpReport(ctypes.Structure):
@@ -307,33 +308,43 @@ def default_cp_options():
     return o
 
 
-def cp_als_ex(X, dims, rank, factors, opts=None, want_fit=True, want_ms=False, trace=None):
+def cp_als_ex(X, dims, rank, factors, opts=None, want_fit=True, want_ms=False, trace=None, want_elapsed=False):
     """Returns (status, report, fit_hist, sweep_ms).  With trace=dict() and a line search
     the dict receives per-sweep arrays alpha, accepted, features, plain_relerr (and
-    grid_relerr for the grid mode)."""
+    grid_relerr for the grid mode).  With want_elapsed, report.elapsed_ms holds the device ms
+    from the call's first device work to the end of every sweep."""
     import numpy as np
     o = opts if opts is not None else default_cp_options()
     total = o.max_sweeps + (o.refine_sweeps if o.storage != TDX_F32 else 0)
-    fit = np.zeros(max(total, 1), np.float64)
-    ms = np.zeros(max(total, 1), np.float32)
+    n = max(total, 1)
+    fit = np.zeros(n, np.float64)
+    ms = np.zeros(n, np.float32)
+    elapsed = np.zeros(n, np.float32) if want_elapsed else None
     rep = CpReport()
     tr = None
-    if trace is not None and o.line_search:
-        n = max(total, 1)
-        trace["alpha"] = np.zeros(n, np.float32)
-        trace["accepted"] = np.zeros(n, np.int32)
-        trace["features"] = np.zeros((n, LS_FEATURES), np.float32)
-        trace["plain_relerr"] = np.zeros(n, np.float64)
-        trace["grid_relerr"] = np.zeros((n, max(o.ngrid, 1)), np.float64)
-        tr = CpTrace(trace["alpha"].ctypes.data, trace["accepted"].ctypes.data, trace["features"].ctypes.data,
-                     trace["grid_relerr"].ctypes.data, trace["plain_relerr"].ctypes.data)
+    if (trace is not None and o.line_search) or want_elapsed:
+        tr = CpTrace()
+        if trace is not None and o.line_search:
+            trace["alpha"] = np.zeros(n, np.float32)
+            trace["accepted"] = np.zeros(n, np.int32)
+            trace["features"] = np.zeros((n, LS_FEATURES), np.float32)
+            trace["plain_relerr"] = np.zeros(n, np.float64)
+            trace["grid_relerr"] = np.zeros((n, max(o.ngrid, 1)), np.float64)
+            tr.alpha, tr.accepted = trace["alpha"].ctypes.data, trace["accepted"].ctypes.data
+            tr.features, tr.grid_relerr = trace["features"].ctypes.data, trace["grid_relerr"].ctypes.data
+            tr.plain_relerr = trace["plain_relerr"].ctypes.data
+        if want_elapsed:
+            tr.elapsed_ms = elapsed.ctypes.data
         o.trace = ctypes.pointer(tr)
     st = _call_status("tdx_cp_als_ex", ptr(X), len(dims), dims_arr(dims), rank, ptr_arr(factors), ctypes.byref(o),
                       ctypes.byref(rep), fit.ctypes.data if want_fit else None, ms.ctypes.data if want_ms else None)
     if tr is not None:
         o.trace = None
-        for k in trace:
-            trace[k] = trace[k][:rep.sweeps_done]
+        if trace is not None and o.line_search:
+            for k in trace:
+                trace[k] = trace[k][:rep.sweeps_done]
+    if want_elapsed:
+        rep.elapsed_ms = elapsed[:rep.sweeps_done]
     if st not in (TDX_OK, TDX_WARN_SINGULAR):
         raise TdxError(st, "tdx_cp_als_ex")
     return st, rep, fit[:rep.sweeps_done], ms[:rep.sweeps_done]

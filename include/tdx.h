@@ -184,6 +184,10 @@ typedef struct {
                                 pass over X that also writes T = X x_0 A^T (I2 I1 Rp floats), then
                                 a pass over T (default 1; no counterpart in the reference, which
                                 runs one MTTKRP per mode); 0: one fused MTTKRP per mode */
+    int opening_sweeps;      /* BF16/FP8 storage: the first opening_sweeps of the max_sweeps main
+                                sweeps read the FP32 tensor (default 0).  The early sweeps decide
+                                which components ALS resolves; rounding the Khatri-Rao operand to
+                                BF16 there can settle a run on another plateau */
 } tdx_cp_options;
 
 /* features of the extrapolation MLP, computed on the device after every sweep k:
@@ -199,6 +203,9 @@ typedef struct tdx_cp_trace {
     float *features;         /* [sweeps * TDX_LS_FEATURES] */
     double *grid_relerr;     /* [sweeps * ngrid] relative error at each grid alpha (line_search 3) */
     double *plain_relerr;    /* [sweeps] relative error of the plain ALS update of every sweep */
+    float *elapsed_ms;       /* [sweeps] device ms from the call's first device work (norm of X,
+                                conversion, panels) to the end of every sweep: time to an error
+                                level including every phase change (any line_search mode) */
 } tdx_cp_trace;
 
 typedef struct {
