@@ -44,6 +44,10 @@ enum WsSlot {
 };
 int workspace(int slot, size_t bytes, void **ptr);
 void release_workspace();
+// the fixed device scratch of tdx_mttkrp_ex_noalloc (64 MiB per GPU): its address on the
+// current device, taken once per device by hipGetSymbolAddress (mttkrp.hip)
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+int noalloc_scratch(void **ptr);
 
 __host__ __device__ inline long ceil_div(long a, long b) { return (a + b - 1) / b; }
 

@@ -26,6 +26,7 @@
 // Partials are summed over splits in FP64 by the consumer (mttkrp_reduce or the ALS solve).
 #include "mttkrp.hpp"
 #include <stdlib.h>
+#include <mutex>
 
 namespace tdx {
 
@@ -640,11 +641,77 @@ int mttkrp_reduce(const MttkrpPlan &p, const float *partial, float *out, hipStre
     return TDX_OK;
 }
 
+// ---- scratch of the no-allocation MTTKRP (tdx_mttkrp_ex_noalloc) ----
+// One statically allocated device array per GPU, held until the process ends, serves the factor
+// panels and the split-K partials.  Host code never passes the array's name as an address (in
+// host code it is a stand-in, not the device address): the address is taken once per device,
+// with that device current, by hipGetSymbolAddress, and kept per device.  Every launch writes
+// all of the scratch it later reads (panels including their zero padding, every partial row of
+// every split), so nothing needs clearing between calls.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+static constexpr int kScratchDevices = 64;
+static void *g_scratch_addr[kScratchDevices];
+static std::mutex g_scratch_mutex;
+
+int noalloc_scratch(void **ptr) {
+    int dev = 0;
+    TDX_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kScratchDevices) return record(TDX_ERR_UNSUPPORTED);
+    std::lock_guard<std::mutex> lk(g_scratch_mutex);
+    if (!g_scratch_addr[dev]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return record_hip(e, "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+        if (!p) return record(TDX_ERR_HIP);
+        g_scratch_addr[dev] = p;
+    }
+    if (ptr) *ptr = g_scratch_addr[dev];
+    return TDX_OK;
+}
+
 }  // namespace tdx
 
 using namespace tdx;
 
 extern "C" {
+
+int tdx_mttkrp_ex_noalloc(const void *X, int dtype, const float *scales, int order, const long *dims,
+                          const float *const *factors, long rank, int mode, float *out) {
+    if (!X || !dims || !factors || !out) return record(TDX_ERR_ARG);
+    MttkrpPlan p;
+    int st = mttkrp_plan(p, order, dims, mode, dtype, rank, X);
+    if (st) return st;
+    for (int m = 0; m < order; ++m)
+        if (m != mode && !factors[m]) return record(TDX_ERR_ARG);
+    if (dtype == TDX_FP8_E4M3 && !scales) return record(TDX_ERR_ARG);
+    // scratch layout: the panels of the non-output modes, then the partials (256-byte aligned)
+    size_t off[4] = {0, 0, 0, 0}, need = 0;
+    for (int m = 0; m < order; ++m) {
+        if (m == mode) continue;
+        off[m] = need;
+        need += (sizeof(float) * (size_t)panel_floats(dims[m], p.Rp) + 255) & ~size_t(255);
+    }
+    const size_t poff = need;
+    need += sizeof(float) * p.partial_floats();
+    if (need > kNoAllocScratchBytes) return record(TDX_ERR_TOO_LARGE);
+    void *base = nullptr;
+    st = noalloc_scratch(&base);
+    if (st) return st;
+    unsigned char *b = (unsigned char *)base;
+    hipStream_t s = stream();
+    const float *panels[4] = {nullptr, nullptr, nullptr, nullptr};
+    for (int m = 0; m < order; ++m) {
+        if (m == mode) continue;
+        const float *sc = (dtype == TDX_FP8_E4M3 && m == 0) ? scales : nullptr;
+        st = pack_panel(factors[m], dims[m], rank, p.Rp, sc, (float *)(b + off[m]), s);
+        if (st) return record(st);
+        panels[m] = (const float *)(b + off[m]);
+    }
+    float *part = (float *)(b + poff);
+    st = mttkrp_launch(p, X, panels, (dtype == TDX_FP8_E4M3 && mode == 0) ? scales : nullptr, part, s);
+    if (st) return record(st);
+    return record(mttkrp_reduce(p, part, out, s));
+}
 
 int tdx_mttkrp_ex(const void *X, int dtype, const float *scales, int order, const long *dims,
                   const float *const *factors, long rank, int mode, float *out) {

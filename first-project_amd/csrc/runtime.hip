@@ -84,6 +84,9 @@ int tdx_init(void) {
     int dev = 0;
     TDX_HIP(hipGetDevice(&dev));
     TDX_HIP(hipFree(nullptr));
+    // the no-allocation MTTKRP's scratch address on this device
+    int st = tdx::noalloc_scratch(nullptr);
+    if (st) return st;
     return tdx::record(TDX_OK);
 }
 
@@ -105,6 +108,7 @@ const char *tdx_status_string(int st) {
         case TDX_WARN_SINGULAR: return "singular normal equations: pseudo-inverse used";
         case TDX_ERR_NONFINITE: return "non-finite value detected";
         case TDX_ERR_UNSUPPORTED: return "unsupported shape or dtype";
+        case TDX_ERR_TOO_LARGE: return "problem larger than the fixed 64 MiB scratch of the no-allocation entry point";
         default: return "unknown status";
     }
 }

@@ -17,7 +17,8 @@ LIB_PATH = os.path.join(_HERE, "libtdx.so")
 _LIB = None
 
 TDX_F32, TDX_BF16, TDX_FP8_E4M3, TDX_F16 = 0, 1, 2, 3
-TDX_OK, TDX_ERR_ARG, TDX_ERR_ALLOC, TDX_ERR_HIP, TDX_WARN_SINGULAR, TDX_ERR_NONFINITE, TDX_ERR_UNSUPPORTED = range(7)
+(TDX_OK, TDX_ERR_ARG, TDX_ERR_ALLOC, TDX_ERR_HIP, TDX_WARN_SINGULAR, TDX_ERR_NONFINITE, TDX_ERR_UNSUPPORTED,
+ TDX_ERR_TOO_LARGE) = range(8)
 
 _L = ctypes.c_long
 _P = ctypes.c_void_p
@@ -71,6 +72,8 @@ _SIGS = {
     "tdx_dequantize_fp8_mode1": [_P, _P, _P, _L, _L],
     "tdx_mttkrp": [_P, _P, _P, _P, _L, _L, _L, _L, _L],
     "tdx_mttkrp_ex": [_P, ctypes.c_int, _P, ctypes.c_int, ctypes.POINTER(_L), ctypes.POINTER(_P), _L,
+                      ctypes.c_int, _P],
+    "tdx_mttkrp_ex_noalloc": [_P, ctypes.c_int, _P, ctypes.c_int, ctypes.POINTER(_L), ctypes.POINTER(_P), _L,
                       ctypes.c_int, _P],
     "tdx_bench_mttkrp": [_P, ctypes.c_int, _P, ctypes.c_int, ctypes.POINTER(_L), ctypes.POINTER(_P), _L,
                          ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)],
@@ -235,6 +238,13 @@ def mttkrp(X, left, right, XU, flag, m, n, k, r):
 def mttkrp_ex(X, dtype, scales, dims, factors, rank, mode, out):
     _call("tdx_mttkrp_ex", ptr(X), dtype, ptr(scales), len(dims), dims_arr(dims), ptr_arr(factors), rank, mode,
           ptr(out))
+
+
+def mttkrp_ex_noalloc(X, dtype, scales, dims, factors, rank, mode, out):
+    """tdx_mttkrp_ex with no device allocation (fixed 64 MiB scratch per GPU); returns the status
+    (TDX_ERR_TOO_LARGE when the panels and partials do not fit)."""
+    return _call_status("tdx_mttkrp_ex_noalloc", ptr(X), dtype, ptr(scales), len(dims), dims_arr(dims),
+                        ptr_arr(factors), rank, mode, ptr(out))
 
 
 def mttkrp_dtree(X, dtype, scales, dims, factors, rank, M1, M2):

@@ -28,7 +28,8 @@ enum {
     TDX_ERR_HIP = 3,          /* a HIP runtime call failed */
     TDX_WARN_SINGULAR = 4,    /* Gram-Hadamard not SPD: pseudo-inverse fallback was used */
     TDX_ERR_NONFINITE = 5,    /* NaN/Inf detected in the fit */
-    TDX_ERR_UNSUPPORTED = 6   /* shape/dtype combination not implemented */
+    TDX_ERR_UNSUPPORTED = 6,  /* shape/dtype combination not implemented */
+    TDX_ERR_TOO_LARGE = 7     /* tdx_mttkrp_ex_noalloc: panels + partials exceed its fixed scratch */
 };
 
 /* storage types of the tensor operand */
@@ -86,6 +87,16 @@ int tdx_mttkrp(const float *X, const float *left, const float *right, float *XU,
  * I_m x rank FP32 column-major (factors[mode] is not read).  out: dims[mode] x rank. */
 int tdx_mttkrp_ex(const void *X, int dtype, const float *scales, int order, const long *dims,
                   const float *const *factors, long rank, int mode, float *out);
+/* new, no counterpart in the reference: tdx_mttkrp_ex that allocates nothing, for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same arguments, kernels and results (bit for bit) as tdx_mttkrp_ex;
+ * the factor panels and split-K partials live in a fixed 64 MiB device array per GPU, held until
+ * the process ends (its address is taken by tdx_init or by the first call on a device).  Returns
+ * TDX_ERR_TOO_LARGE when they do not fit (e.g. a 1024^3 rank-64 MTTKRP needs 8 / 16 / 32 MiB
+ * for BF16 / FP32 / FP8 storage).  Calls that overlap on one GPU (other streams or threads)
+ * share that array and race; calls on the library stream are ordered. */
+int tdx_mttkrp_ex_noalloc(const void *X, int dtype, const float *scales, int order, const long *dims,
+                          const float *const *factors, long rank, int mode, float *out);
 /* new: times `iters` launches of the fused MTTKRP kernel alone with hipEvents on the
  * library stream (panels and workspace prepared once); *avg_ms = mean launch time. */
 int tdx_bench_mttkrp(const void *X, int dtype, const float *scales, int order, const long *dims,

@@ -57,5 +57,5 @@ def test_invalid_arguments_return_status(call):
 
 def test_status_strings():
     L = tdx.lib()
-    for st in range(7):
+    for st in range(8):
         assert len(L.tdx_status_string(st)) > 0
