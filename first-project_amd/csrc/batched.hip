@@ -35,88 +35,138 @@ struct BDesc {
 
 typedef float bf32x4 __attribute__((ext_vector_type(4)));
 
+// sum over the workgroup: wave shuffles, then one value per wave through LDS (sh >= BS/64
+// doubles); every thread returns the total
+template <int BS>
+__device__ inline double bsum(double v, double *sh) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int tid = threadIdx.x;
+    __syncthreads();
+    if ((tid & 63) == 0) sh[tid >> 6] = v;
+    __syncthreads();
+    double r = 0.0;
+#pragma unroll
+    for (int w = 0; w < BS / 64; ++w) r += sh[w];
+    return r;
+}
+
+__device__ inline int sel3(int m, int a, int b, int c) { return m == 0 ? a : (m == 1 ? b : c); }
+__device__ inline long sel3l(int m, long a, long b, long c) { return m == 0 ? a : (m == 1 ? b : c); }
+
+// The solve-phase arrays live in the slab (idle between the MTTKRP streams): V, L, W, V^{-1}
+// (R^2 doubles each), the Jacobi fallback's rotation table and the reduction scratch.
+template <int BS>
+struct SolveLds {
+    static constexpr int kDoubles = 4 * kBMaxR * kBMaxR + BS + 4 * (kBMaxR / 2 + 1);
+};
+
+// Workgroups per CU the variant is built for (the LDS of slabs up to 6K floats allows two)
 template <int BS, int SLAB>
-__global__ __launch_bounds__(BS) void k_cp_batched_sweep(const float *__restrict__ X, float *__restrict__ U,
+struct BatchedOcc {
+    static constexpr int kWgs = (SLAB <= 6144) ? 2 : 1;
+};
+
+template <int BS, int SLAB>
+__global__ __launch_bounds__(BS, (BatchedOcc<BS, SLAB>::kWgs)) void k_cp_batched_sweep(const float *__restrict__ X, float *__restrict__ U,
                                                          const BDesc *__restrict__ desc, const int *__restrict__ sched,
                                                          int ntensors, int *__restrict__ queue,
                                                          const double *__restrict__ normx2,
                                                          double *__restrict__ fit_hist, int max_sweeps,
                                                          const int *__restrict__ sweep_ctr, int *__restrict__ status) {
-    __shared__ float slab[SLAB];
+    static_assert(SLAB / 2 >= SolveLds<BS>::kDoubles, "the solve arrays fit in the slab");
+    __shared__ __attribute__((aligned(16))) float slab[SLAB];
     __shared__ float Us[3][kBMaxDim * kBMaxR];
     __shared__ float Ut[kBMaxDim * kBMaxR];  // A transposed, [i][r] with r < 16 (zero padded)
     __shared__ float Ms[kBMaxDim * kBMaxR];
     __shared__ float Ys[kSlabCols * kBMaxR > kBMaxDim * kBMaxR ? kSlabCols * kBMaxR : kBMaxDim * kBMaxR];
     __shared__ double Gs[3][kBMaxR * kBMaxR];
-    __shared__ double V[kBMaxR * kBMaxR], L[kBMaxR * kBMaxR], W[kBMaxR * kBMaxR], Vi[kBMaxR * kBMaxR];
-    __shared__ double sh[BS];
-    __shared__ double cs[4 * (kBMaxR / 2 + 1)];
+    __shared__ double red[BS / 64];
     __shared__ int fail, slot;
+    double *const V = reinterpret_cast<double *>(slab);
+    double *const L = V + kBMaxR * kBMaxR;
+    double *const W = L + kBMaxR * kBMaxR;
+    double *const Vi = W + kBMaxR * kBMaxR;
+    double *const sh = Vi + kBMaxR * kBMaxR;   // BS doubles (Jacobi fallback)
+    double *const cs = sh + BS;
     const int tid = threadIdx.x;
     const int sweep = *sweep_ctr;
+    constexpr int NW = BS / 64;
+    const int lane = tid & 63, wv = tid >> 6, lr = lane & 15, lk = lane >> 4;
     for (;;) {
         if (tid == 0) slot = atomicAdd(queue, 1);
         __syncthreads();
-        const int sl = slot;
+        const int sl = __builtin_amdgcn_readfirstlane(slot);
         if (sl >= ntensors) break;  // uniform: every wave exits once the queue is drained
-        const int t = sched[sl];
-        const BDesc d = desc[t];
-        const int dims[3] = {d.I, d.J, d.K};
-        const int R = d.R;
-        const float *x = X + d.xoff;
+        // the descriptor in scalar registers (no per-thread copy of it)
+        const int t = __builtin_amdgcn_readfirstlane(sched[sl]);
+        const BDesc *dp = desc + t;
+        const int I = dp->I, J = dp->J, K = dp->K, R = dp->R;
+        const long xoff = dp->xoff, uo0 = dp->uoff[0], uo1 = dp->uoff[1], uo2 = dp->uoff[2];
+        const float *x = X + xoff;
         // factors to LDS, Grams in FP64
-        for (int m = 0; m < 3; ++m)
-            for (int e = tid; e < dims[m] * R; e += BS) Us[m][e] = U[d.uoff[m] + e];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            const int Im = sel3(m, I, J, K);
+            const long uo = sel3l(m, uo0, uo1, uo2);
+            for (int e = tid; e < Im * R; e += BS) Us[m][e] = U[uo + e];
+        }
         __syncthreads();
-        for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            const int Im = sel3(m, I, J, K);
             for (int e = tid; e < R * R; e += BS) {
                 const int a = e % R, b = e / R;
                 double s = 0.0;
-                for (int i = 0; i < dims[m]; ++i) s += (double)Us[m][i + a * dims[m]] * (double)Us[m][i + b * dims[m]];
+                for (int i = 0; i < Im; ++i) s += (double)Us[m][i + a * Im] * (double)Us[m][i + b * Im];
                 Gs[m][e] = s;
             }
+        }
         __syncthreads();
         // slabs are runs of consecutive columns c = j + k*J of the mode-1 unfolding
         // (contiguous in memory), stored in LDS with column stride I + 1 (the MFMA operand
         // reads walk both indices); at most kSlabCols columns so the per-column scratch fits
-        const int ncol = d.J * d.K;
-        const int Ip = d.I + 1;
+        const int ncol = J * K;
+        const int Ip = I + 1;
         const int CC = min(SLAB / Ip, kSlabCols);
-        constexpr int NW = BS / 64;
-        const int lane = tid & 63, wv = tid >> 6, lr = lane & 15, lk = lane >> 4;
         double ip_last = 0.0;
         for (int n = 0; n < 3; ++n) {
-            const int In = dims[n];
+            const int In = sel3(n, I, J, K);
+            const long uon = sel3l(n, uo0, uo1, uo2);
             for (int e = tid; e < In * R; e += BS) Ms[e] = 0.0f;
             if (n == 1) {
                 // modes 1 and 2 contract i with the (just updated) A: A^T rows, zero padded
                 for (int e = tid; e < kBMaxDim * kBMaxR; e += BS) {
                     const int i = e >> 4, r = e & 15;
-                    Ut[e] = (i < d.I && r < R) ? Us[0][i + r * d.I] : 0.0f;
+                    Ut[e] = (i < I && r < R) ? Us[0][i + r * I] : 0.0f;
                 }
             }
             // mode 0 accumulators: row tiles wv + NW u of M (16 rows x 16 columns each)
             bf32x4 acc0[(8 + NW - 1) / NW];
 #pragma unroll
             for (int u = 0; u < (8 + NW - 1) / NW; ++u) acc0[u] = bf32x4{0.f, 0.f, 0.f, 0.f};
-            // -------- MTTKRP, slab by slab; the next slab's elements are loaded into
-            // registers while the current one is multiplied --------
+            // -------- MTTKRP, slab by slab, two slabs in flight: each slab's elements are
+            // loaded into one of two register buffers two slabs ahead of their use --------
             // element e = tid + BS u of a slab is (i, q) = (e mod I, e div I); (i0, q0) for
             // u = 0 and the per-u step (di, dq) are computed once
-            const int q0 = tid / d.I, i0 = tid - q0 * d.I, dq = BS / d.I, di = BS - dq * d.I;
+            const int q0 = tid / I, i0 = tid - q0 * I, dq = BS / I, di = BS - dq * I;
             constexpr int EPT = (SLAB + BS - 1) / BS;
-            float pre[EPT];
-            auto fetch = [&](int cs) {  // loads of the slab at column cs (clamped into it)
-                const int ccs = min(CC, ncol - cs);
-                const long base = (long)cs * d.I, last = (long)ccs * d.I - 1;
+            float preA[EPT], preB[EPT];
+            // Every fetch issues its loads unconditionally (a load issued on some paths only makes
+            // the compiler's wait counting drain the queue): past the last slab every lane reads
+            // the tensor's first element (one cached line).
+            auto fetch = [&](float (&pre)[EPT], int cs0) {  // loads of the slab at column cs0
+                const bool past = cs0 >= ncol;
+                const int ccs = min(CC, ncol - cs0);
+                const float *xs = x + (past ? 0L : (long)cs0 * I);  // uniform base, 32-bit offsets
+                const int last = past ? 0 : ccs * I - 1;
 #pragma unroll
                 for (int u = 0; u < EPT; ++u) {
-                    const long e = tid + (long)BS * u;
-                    pre[u] = x[base + (e < last ? e : last)];
+                    const int e = tid + BS * u;
+                    pre[u] = xs[e < last ? e : last];
                 }
             };
-            fetch(0);
-            for (int c0 = 0; c0 < ncol; c0 += CC) {
+            auto step = [&](float (&pre)[EPT], int c0) {
                 const int cc = min(CC, ncol - c0);
                 {
                     int i = i0, q = q0;
@@ -125,8 +175,8 @@ __global__ __launch_bounds__(BS) void k_cp_batched_sweep(const float *__restrict
                         if (q < cc) slab[i + q * Ip] = pre[u];
                         i += di;
                         q += dq;
-                        if (i >= d.I) {
-                            i -= d.I;
+                        if (i >= I) {
+                            i -= I;
                             ++q;
                         }
                     }
@@ -135,22 +185,22 @@ __global__ __launch_bounds__(BS) void k_cp_batched_sweep(const float *__restrict
                     // Khatri-Rao rows of the slab's columns: Ys[q*16 + r] = B[j_q,r] C[k_q,r]
                     for (int e = tid; e < cc * kBMaxR; e += BS) {
                         const int q = e >> 4, r = e & 15, c = c0 + q;
-                        Ys[e] = (r < R) ? Us[1][c % d.J + r * d.J] * Us[2][c / d.J + r * d.K] : 0.0f;
+                        Ys[e] = (r < R) ? Us[1][c % J + r * J] * Us[2][c / J + r * K] : 0.0f;
                     }
                 }
                 __syncthreads();
-                fetch((c0 + CC < ncol) ? c0 + CC : c0);  // the last slab re-reads itself (never stored)
+                fetch(pre, c0 + 2 * CC);
                 if (n == 0) {
                     // M += slab (I x cc) * KR (cc x 16): A[i][q] = slab, B[q][r] = Ys
                     const int nks = (cc + 3) >> 2;
 #pragma unroll
                     for (int u = 0; u < (8 + NW - 1) / NW; ++u) {
-                        const int t = wv + NW * u;
-                        if (16 * t < d.I) {
-                            const int i = 16 * t + lr;
+                        const int tt = wv + NW * u;
+                        if (16 * tt < I) {
+                            const int i = 16 * tt + lr;
                             for (int ks = 0; ks < nks; ++ks) {
                                 const int q = 4 * ks + lk;
-                                const float a = (i < d.I && q < cc) ? slab[i + q * Ip] : 0.0f;
+                                const float a = (i < I && q < cc) ? slab[i + q * Ip] : 0.0f;
                                 const float b = (q < cc) ? Ys[q * kBMaxR + lr] : 0.0f;
                                 acc0[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc0[u], 0, 0, 0);
                             }
@@ -158,61 +208,79 @@ __global__ __launch_bounds__(BS) void k_cp_batched_sweep(const float *__restrict
                     }
                 } else {
                     // T (cc x 16) = slab^T (cc x I) * A (I x 16) -> Ys
-                    const int nks = (d.I + 3) >> 2;
-                    for (int t = wv; 16 * t < cc; t += NW) {
+                    const int nks = (I + 3) >> 2;
+                    for (int tt = wv; 16 * tt < cc; tt += NW) {
                         bf32x4 tacc = {0.f, 0.f, 0.f, 0.f};
-                        const int q = 16 * t + lr;
+                        const int q = 16 * tt + lr;
                         for (int ks = 0; ks < nks; ++ks) {
                             const int i = 4 * ks + lk;
-                            const float a = (q < cc && i < d.I) ? slab[i + q * Ip] : 0.0f;
+                            const float a = (q < cc && i < I) ? slab[i + q * Ip] : 0.0f;
                             tacc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Ut[i * kBMaxR + lr], tacc, 0, 0, 0);
                         }
 #pragma unroll
                         for (int v = 0; v < 4; ++v) {
-                            const int qq = 16 * t + 4 * lk + v;
+                            const int qq = 16 * tt + 4 * lk + v;
                             if (qq < cc) Ys[qq * kBMaxR + lr] = tacc[v];
                         }
                     }
                     __syncthreads();
                     if (n == 1) {
                         // M[j,r] += sum over the slab's columns with this j of T[q,r] C[k_q,r]
-                        for (int e = tid; e < d.J * R; e += BS) {
-                            const int j = e % d.J, r = e / d.J;
+                        for (int e = tid; e < J * R; e += BS) {
+                            const int j = e % J, r = e / J;
                             float a2 = 0.0f;
-                            for (int q = (j - c0 % d.J + d.J) % d.J; q < cc; q += d.J)
-                                a2 = fmaf(Ys[q * kBMaxR + r], Us[2][(c0 + q) / d.J + r * d.K], a2);
+                            for (int q = (j - c0 % J + J) % J; q < cc; q += J)
+                                a2 = fmaf(Ys[q * kBMaxR + r], Us[2][(c0 + q) / J + r * K], a2);
                             Ms[e] += a2;
                         }
                     } else {
-                        // M[k,r] += sum over k's columns of T[q,r] B[j_q,r]
-                        const int kf = c0 / d.J, kl = (c0 + cc - 1) / d.J;
-                        for (int e = tid; e < (kl - kf + 1) * R; e += BS) {
-                            const int k = kf + e / R, r = e % R;
-                            const int qa = max(0, k * d.J - c0), qb = min(cc, (k + 1) * d.J - c0);
+                        // M[k,r] += sum over k's columns of T[q,r] B[j_q,r]: 16 lanes per (k, r),
+                        // strided over the columns, then a shuffle reduction within the 16
+                        const int kf = c0 / J, kl = (c0 + cc - 1) / J;
+                        const int npair = (kl - kf + 1) * R;
+                        const int g = tid & 15;
+                        for (int p0 = 0; p0 < npair; p0 += BS / 16) {
+                            const int pr = p0 + (tid >> 4);
                             float a2 = 0.0f;
-                            for (int q = qa; q < qb; ++q) a2 = fmaf(Ys[q * kBMaxR + r], Us[1][(c0 + q) % d.J + r * d.J], a2);
-                            Ms[k + r * d.K] += a2;
+                            int k = 0, r = 0;
+                            if (pr < npair) {
+                                k = kf + pr / R;
+                                r = pr % R;
+                                const int qa = max(0, k * J - c0), qb = min(cc, (k + 1) * J - c0);
+                                for (int q = qa + g; q < qb; q += 16)
+                                    a2 = fmaf(Ys[q * kBMaxR + r], Us[1][(c0 + q) % J + r * J], a2);
+                            }
+#pragma unroll
+                            for (int o = 8; o > 0; o >>= 1) a2 += __shfl_xor(a2, o, 16);
+                            if (pr < npair && g == 0) Ms[k + r * K] += a2;
                         }
                     }
                 }
                 __syncthreads();
+            };
+            fetch(preA, 0);
+            fetch(preB, CC);
+            for (int c0 = 0; c0 < ncol; c0 += 2 * CC) {
+                step(preA, c0);
+                step(preB, c0 + CC);  // past the last column: an empty step (no stores, no work)
             }
             if (n == 0) {
                 // mode-0 accumulators -> M (row 4 (lane / 16) + v of the tile, column lane % 16)
 #pragma unroll
                 for (int u = 0; u < (8 + NW - 1) / NW; ++u) {
-                    const int t = wv + NW * u;
+                    const int tt = wv + NW * u;
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
-                        const int i = 16 * t + 4 * lk + v;
-                        if (i < d.I && lr < R) Ms[i + lr * d.I] = acc0[u][v];
+                        const int i = 16 * tt + 4 * lk + v;
+                        if (i < I && lr < R) Ms[i + lr * I] = acc0[u][v];
                     }
                 }
                 __syncthreads();
             }
-            // -------- normal equations in FP64 --------
+            // -------- normal equations in FP64 (arrays in the idle slab) --------
             for (int e = tid; e < R * R; e += BS) {
                 double v = 1.0;
+#pragma unroll
                 for (int m = 0; m < 3; ++m)
                     if (m != n) v *= Gs[m][e];
                 V[e] = v;
@@ -232,9 +300,9 @@ __global__ __launch_bounds__(BS) void k_cp_batched_sweep(const float *__restrict
             __syncthreads();
             for (int e = tid; e < In * R; e += BS) {
                 Us[n][e] = Ys[e];
-                U[d.uoff[n] + e] = Ys[e];
+                U[uon + e] = Ys[e];
             }
-            if (n == 2) ip_last = block_sum(ip, sh);
+            if (n == 2) ip_last = bsum<BS>(ip, red);
             __syncthreads();
             for (int e = tid; e < R * R; e += BS) {
                 const int a = e % R, b = e / R;
@@ -247,7 +315,7 @@ __global__ __launch_bounds__(BS) void k_cp_batched_sweep(const float *__restrict
         // fit: ||X - Xhat||^2 = ||X||^2 - 2 <M_2, U_2> + sum(G0 o G1 o G2)
         double hx = 0.0;
         for (int e = tid; e < R * R; e += BS) hx += Gs[0][e] * Gs[1][e] * Gs[2][e];
-        hx = block_sum(hx, sh);
+        hx = bsum<BS>(hx, red);
         if (tid == 0 && sweep < max_sweeps) {
             const double nx2 = normx2[t];
             fit_hist[(long)t * max_sweeps + sweep] = sqrt(fabs(nx2 - 2.0 * ip_last + hx) / nx2);
@@ -298,14 +366,16 @@ struct Variant {
     const char *name;
 };
 
+// LDS per workgroup: the slab plus ~54 KB of factors, Grams and per-mode scratch; the 4K-6K
+// slab variants fit two workgroups per CU (160 KB of LDS), the larger ones one.
 static const Variant kVariants[] = {
-    {128, 4096, k_cp_batched_sweep<128, 4096>, "bs128_slab4k"},
     {256, 4096, k_cp_batched_sweep<256, 4096>, "bs256_slab4k"},
+    {256, 6144, k_cp_batched_sweep<256, 6144>, "bs256_slab6k"},
+    {512, 4096, k_cp_batched_sweep<512, 4096>, "bs512_slab4k"},
+    {512, 6144, k_cp_batched_sweep<512, 6144>, "bs512_slab6k"},
     {256, 8192, k_cp_batched_sweep<256, 8192>, "bs256_slab8k"},
     {512, 8192, k_cp_batched_sweep<512, 8192>, "bs512_slab8k"},
-    {256, 12288, k_cp_batched_sweep<256, 12288>, "bs256_slab12k"},
     {512, 12288, k_cp_batched_sweep<512, 12288>, "bs512_slab12k"},
-    {512, 16384, k_cp_batched_sweep<512, 16384>, "bs512_slab16k"},
 };
 static constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
@@ -344,7 +414,7 @@ const char *tdx_cp_batched_variant_name(int v) {
 // CP-ALS of `ntensors` 3rd-order tensors packed in X (tensor t at xoff[t], column-major
 // dims[3t..3t+2]) with ranks[t] <= 16 and extents <= 128; factors packed in U (mode m of
 // tensor t at uoff[3t+m], I_m x R column-major).  init = 0 draws U(0,1) from Philox stream
-// (seed + t, m); 1 uses the caller's U.  variant < 0 picks variant 1.  fit_hist (host, may
+// (seed + t, m); 1 uses the caller's U.  variant < 0 picks variant 3.  fit_hist (host, may
 // be NULL) receives ntensors x sweeps relative errors; sweep_ms (host, may be NULL) the
 // device time of every sweep.  The per-sweep launches are captured once in a hipGraph.
 int tdx_cp_als_batched(const float *X, float *U, const long *dims, const long *ranks, const long *xoff,
@@ -352,7 +422,7 @@ int tdx_cp_als_batched(const float *X, float *U, const long *dims, const long *r
                        double *fit_hist, float *sweep_ms) {
     if (!X || !U || !dims || !ranks || !xoff || !uoff || ntensors <= 0 || sweeps < 0) return record(TDX_ERR_ARG);
     if (variant >= kNumVariants) return record(TDX_ERR_ARG);
-    if (variant < 0) variant = 1;
+    if (variant < 0) variant = 3;
     std::vector<BDesc> h(ntensors);
     std::vector<std::pair<long, int>> order(ntensors);
     for (long t = 0; t < ntensors; ++t) {

@@ -109,6 +109,7 @@ const char *tdx_status_string(int st) {
         case TDX_ERR_NONFINITE: return "non-finite value detected";
         case TDX_ERR_UNSUPPORTED: return "unsupported shape or dtype";
         case TDX_ERR_TOO_LARGE: return "problem larger than the fixed 64 MiB scratch of the no-allocation entry point";
+        case TDX_ERR_IO: return "tensor file I/O failed or the file is not a valid .tdxq tensor";
         default: return "unknown status";
     }
 }

@@ -18,7 +18,7 @@ _LIB = None
 
 TDX_F32, TDX_BF16, TDX_FP8_E4M3, TDX_F16 = 0, 1, 2, 3
 (TDX_OK, TDX_ERR_ARG, TDX_ERR_ALLOC, TDX_ERR_HIP, TDX_WARN_SINGULAR, TDX_ERR_NONFINITE, TDX_ERR_UNSUPPORTED,
- TDX_ERR_TOO_LARGE) = range(8)
+ TDX_ERR_TOO_LARGE, TDX_ERR_IO) = range(9)
 
 _L = ctypes.c_long
 _P = ctypes.c_void_p
@@ -33,6 +33,11 @@ class TdxError(RuntimeError):
 
 LS_FEATURES = 12
 LS_NONE, LS_LEARNED, LS_LSB, LS_GRID = 0, 1, 2, 3
+
+
+class TensorInfo(ctypes.Structure):
+    _fields_ = [("order", ctypes.c_int), ("storage", ctypes.c_int), ("dims", ctypes.c_int64 * 4),
+                ("n_scales", ctypes.c_int64), ("data_bytes", ctypes.c_int64)]
 
 
 class CpTrace(ctypes.Structure):
@@ -73,6 +78,9 @@ _SIGS = {
     "tdx_mttkrp": [_P, _P, _P, _P, _L, _L, _L, _L, _L],
     "tdx_mttkrp_ex": [_P, ctypes.c_int, _P, ctypes.c_int, ctypes.POINTER(_L), ctypes.POINTER(_P), _L,
                       ctypes.c_int, _P],
+    "tdx_tensor_info_read": [ctypes.c_char_p, _P],
+    "tdx_tensor_save": [ctypes.c_char_p, _P, ctypes.c_int, _P, ctypes.c_int, ctypes.POINTER(_L), ctypes.c_int],
+    "tdx_tensor_load": [ctypes.c_char_p, _P, ctypes.c_size_t, _P, ctypes.c_size_t, ctypes.c_int, _P],
     "tdx_mttkrp_ex_noalloc": [_P, ctypes.c_int, _P, ctypes.c_int, ctypes.POINTER(_L), ctypes.POINTER(_P), _L,
                       ctypes.c_int, _P],
     "tdx_bench_mttkrp": [_P, ctypes.c_int, _P, ctypes.c_int, ctypes.POINTER(_L), ctypes.POINTER(_P), _L,
@@ -238,6 +246,41 @@ def mttkrp(X, left, right, XU, flag, m, n, k, r):
 def mttkrp_ex(X, dtype, scales, dims, factors, rank, mode, out):
     _call("tdx_mttkrp_ex", ptr(X), dtype, ptr(scales), len(dims), dims_arr(dims), ptr_arr(factors), rank, mode,
           ptr(out))
+
+
+def tensor_info(path):
+    info = TensorInfo()
+    _call("tdx_tensor_info_read", os.fsencode(path), ctypes.byref(info))
+    return info
+
+
+def _addr(a):
+    """device tensor -> (pointer, on_device=1); numpy array -> (pointer, 0)"""
+    if a is None:
+        return None, 0
+    if hasattr(a, "data_ptr"):
+        return ctypes.c_void_p(a.data_ptr()), 1
+    return ctypes.c_void_p(a.ctypes.data), 0
+
+
+def tensor_save(path, X, storage, dims, scales=None):
+    """Writes X (torch device tensor or numpy array, in `storage`) as a .tdxq file."""
+    px, dev_ = _addr(X)
+    ps, _ = _addr(scales)
+    _call("tdx_tensor_save", os.fsencode(path), px, storage, ps, len(dims), dims_arr(dims), dev_)
+
+
+def tensor_load(path, X, scales=None):
+    """Loads a .tdxq file into X (and scales for FP8); both torch device tensors or both numpy
+    arrays, sized in bytes at least data_bytes / 4 n_scales.  Returns the TensorInfo."""
+    px, dev_ = _addr(X)
+    ps, _ = _addr(scales)
+    nx = X.numel() * X.element_size() if hasattr(X, "numel") else X.nbytes
+    ns = 0 if scales is None else (scales.numel() * scales.element_size() if hasattr(scales, "numel")
+                                   else scales.nbytes)
+    info = TensorInfo()
+    _call("tdx_tensor_load", os.fsencode(path), px, nx, ps, ns, dev_, ctypes.byref(info))
+    return info
 
 
 def mttkrp_ex_noalloc(X, dtype, scales, dims, factors, rank, mode, out):

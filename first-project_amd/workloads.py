@@ -134,12 +134,24 @@ def run_c3(iters=10, bf16=False, history=False, data=None):
 if __name__ == "__main__":
     import argparse
     ap = argparse.ArgumentParser(description="run one benchmark recipe on cuda:0")
-    ap.add_argument("recipe", choices=["c3"])
-    ap.add_argument("--iters", type=int, default=10)
-    ap.add_argument("--bf16", action="store_true", help="BF16 Grams and first TTMs")
-    ap.add_argument("--history", action="store_true", help="relative error after every iteration")
+    ap.add_argument("recipe", choices=["c3", "c4"])
+    ap.add_argument("--iters", type=int, default=10, help="c3: HOOI iterations")
+    ap.add_argument("--bf16", action="store_true", help="c3: BF16 Grams and first TTMs")
+    ap.add_argument("--history", action="store_true", help="c3: relative error after every iteration")
+    ap.add_argument("--sweeps", type=int, default=3, help="c4: batched ALS sweeps")
+    ap.add_argument("--variant", type=int, default=-1, help="c4: batched kernel variant (-1: default)")
     a = ap.parse_args()
     tdx.lib()
-    sec, rel, hist = run_c3(a.iters, a.bf16, a.history)
-    print(f"c3 ({'bf16' if a.bf16 else 'fp32'}): {sec:.4f} s, final rel-err {rel:.6e}"
-          + (f", per iteration {list(map(float, hist))}" if hist is not None else ""))
+    if a.recipe == "c3":
+        sec, rel, hist = run_c3(a.iters, a.bf16, a.history)
+        print(f"c3 ({'bf16' if a.bf16 else 'fp32'}): {sec:.4f} s, final rel-err {rel:.6e}"
+              + (f", per iteration {list(map(float, hist))}" if hist is not None else ""))
+    else:
+        import torch
+        X, b = c4()
+        U = torch.empty(b.u_floats, device="cuda")
+        st, fit, ms = tdx.cp_als_batched(X, U, b, a.sweeps, init=0, seed=1, variant=a.variant)
+        gbs = 3 * 4 * b.x_floats / (float(np.mean(ms)) * 1e-3) / 1e9
+        print(f"c4: {b.n} tensors, {b.x_floats} elements, variant {a.variant}: ms per sweep "
+              f"{[round(float(v), 3) for v in ms]} ({gbs:.0f} GB/s of 3 passes over X), median final relerr "
+              f"{float(np.median(fit[:, -1])):.5f}")

@@ -29,7 +29,8 @@ enum {
     TDX_WARN_SINGULAR = 4,    /* Gram-Hadamard not SPD: pseudo-inverse fallback was used */
     TDX_ERR_NONFINITE = 5,    /* NaN/Inf detected in the fit */
     TDX_ERR_UNSUPPORTED = 6,  /* shape/dtype combination not implemented */
-    TDX_ERR_TOO_LARGE = 7     /* tdx_mttkrp_ex_noalloc: panels + partials exceed its fixed scratch */
+    TDX_ERR_TOO_LARGE = 7,    /* tdx_mttkrp_ex_noalloc: panels + partials exceed its fixed scratch */
+    TDX_ERR_IO = 8            /* tensor file: open/read/write failed or not a valid .tdxq file */
 };
 
 /* storage types of the tensor operand */
@@ -73,6 +74,28 @@ int tdx_dequantize_bf16(const uint16_t *Xq, float *X, long n);
  * n_rest = product of the other extents. */
 int tdx_quantize_fp8_mode1(const float *X, uint8_t *Xq, float *scales, long I, long n_rest);
 int tdx_dequantize_fp8_mode1(const uint8_t *Xq, const float *scales, float *X, long I, long n_rest);
+
+/* ---- the quantized tensor file format (.tdxq), the step before MTTKRP: no reference
+ *      counterpart beyond the in-memory conversion f2h (src/float2half.cu:14-18) ----
+ * A 128-byte header (magic "TDXQ", version 1, order, storage, dims[4], n_scales, data_bytes,
+ * FNV-1a 64 of the first 64 bytes), the FP8 slice scales (FP32, padded to 128 bytes), then
+ * the elements column-major in the storage type: FP32, BF16 bit patterns, or E4M3FN codes
+ * with one FP32 scale per mode-1 slice (the layout tdx_quantize_fp8_mode1 produces).
+ * on_device = 1: X and scales are device pointers (staged through pinned buffers on the
+ * library stream); 0: host pointers (no GPU needed). */
+typedef struct {
+    int order;
+    int storage;             /* TDX_F32, TDX_BF16 or TDX_FP8_E4M3 */
+    int64_t dims[4];         /* 1 past the order */
+    int64_t n_scales;        /* dims[0] for FP8 storage, else 0 */
+    int64_t data_bytes;
+} tdx_tensor_info;
+int tdx_tensor_info_read(const char *path, tdx_tensor_info *info);
+int tdx_tensor_save(const char *path, const void *X, int storage, const float *scales, int order, const long *dims,
+                    int on_device);
+/* capacities in bytes; TDX_ERR_ARG when the file's data or scales do not fit; info may be NULL */
+int tdx_tensor_load(const char *path, void *X, size_t x_capacity, float *scales, size_t scales_capacity,
+                    int on_device, tdx_tensor_info *info);
 
 /* ---- contractions ---- */
 /* mttkrp: src/mttkrp.cu:3-80 -- XU = X(flag) (left (.) right) for an m x n x k tensor.
