@@ -13,13 +13,15 @@ re-runs the oracle live on the first seed).  A path reaches its target at the fi
 whose relative error is <= anchor + 1e-4 (and its final direct error is too).
 
  * plain FP32 path: FP32 tensor, FP32-accumulating MFMA MTTKRP, FP64 solve, 27 sweeps.
- * mixed path (the value): 25 main sweeps + 2 FP32 refinement sweeps, the main sweeps
-   streaming the BF16 copy of X after --opening FP32 sweeps (default 2: rounding the
-   Khatri-Rao operand in the first sweeps settles some runs on another plateau; FP32 ALS on
-   the BF16-rounded tensor does not).  The schedule without opening sweeps and the FP8-E4M3
-   storage are reported beside it.  Time to target = device time from the call's first work
-   (norm of X, the BF16 conversion of X, factor panels) to the end of the sweep that reaches
-   the target.  value = median over the init seeds (a seed that misses counts as infinite).
+ * mixed path (the value): 25 main sweeps + 2 FP32 refinement sweeps, all streaming the BF16
+   copy of X; the first --opening (default 2) multiply it by a hi + lo BF16 Khatri-Rao
+   operand (near-FP32 factors), the rest by a BF16 one.  Rounding the Khatri-Rao operand in
+   the first sweeps settles some runs on another plateau, rounding the tensor does not (FP32
+   ALS on the BF16-rounded tensor reaches every seed's FP64 plateau).  Reported beside it: the
+   same schedule opening with FP32 sweeps (--opening-split 1 only), no opening sweeps, and the
+   FP8-E4M3 storage.  Time to target = device time from the call's first work (norm of X, the
+   BF16 conversion of X, factor panels) to the end of the sweep that reaches the target.
+   value = median over the init seeds (a seed that misses counts as infinite).
  * ms_per_step: --warmup untimed, then exactly --steps timed BF16 sweeps (hipEvents).
  * MTTKRP per mode for FP32 / BF16 / FP8-E4M3 storage, and the two kernels of the shared
    pass that the BF16/FP8 sweep uses for modes 1 and 2 (hipEvents on the library stream).
@@ -264,7 +266,10 @@ def main():
     ap.add_argument("--steps", type=int, default=25, help="timed BF16 sweeps of the ms_per_step loop")
     ap.add_argument("--warmup", type=int, default=2, help="untimed sweeps before every timed loop")
     ap.add_argument("--seeds", default="1,2,3,4,5,6,7,8,9", help="initialisation seeds of the C2 runs")
-    ap.add_argument("--opening", type=int, default=2, help="FP32 sweeps opening the mixed path's 25 main sweeps")
+    ap.add_argument("--opening", type=int, default=2, help="opening sweeps of the mixed path's 25 main sweeps")
+    ap.add_argument("--opening-split", type=int, default=1,
+                    help="1: the opening sweeps stream the BF16 copy with a hi + lo BF16 Khatri-Rao block; "
+                         "0: they stream the FP32 tensor")
     ap.add_argument("--mttkrp-iters", type=int, default=10)
     ap.add_argument("--c5-seeds", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
@@ -296,10 +301,11 @@ def main():
     if not same:
         log("the generated tensor differs from the anchor's: targets unavailable")
 
-    def run(storage, sweeps, refine, seed, opening=0):
+    def run(storage, sweeps, refine, seed, opening=0, split=0):
         U = W.normal_init(dims, R, seed)
         o = tdx.default_cp_options()
         o.max_sweeps, o.refine_sweeps, o.storage, o.opening_sweeps = sweeps, refine, storage, opening
+        o.opening_split = split
         o.init, o.record_fit, o.use_graph = 1, 1, 1
         st, rep, fit, ms = tdx.cp_als_ex(X, dims, R, U, o, want_ms=True, want_elapsed=True)
         return rep, np.asarray(fit), np.asarray(ms, np.float64), U
@@ -313,6 +319,7 @@ def main():
     w = max(args.warmup, 1)
     run(tdx.TDX_F32, w, 0, 1)
     run(tdx.TDX_BF16, w, 1, 1, opening=1)
+    run(tdx.TDX_BF16, w, 1, 1, opening=1, split=1)
     run(tdx.TDX_FP8_E4M3, w, 1, 1, opening=1)
     torch.cuda.synchronize()
 
@@ -329,19 +336,22 @@ def main():
         return float(rep.elapsed_ms[hit[0]]) / 1e3, int(hit[0] + 1)
 
     seeds = [int(v) for v in args.seeds.split(",")]
-    paths = {"fp32": (tdx.TDX_F32, PLAIN_SWEEPS, 0, 0),
-             "bf16": (tdx.TDX_BF16, MAIN_SWEEPS, REFINE_SWEEPS, args.opening),
-             "bf16_no_opening": (tdx.TDX_BF16, MAIN_SWEEPS, REFINE_SWEEPS, 0),
-             "fp8": (tdx.TDX_FP8_E4M3, MAIN_SWEEPS, REFINE_SWEEPS, args.opening),
-             "fp8_no_opening": (tdx.TDX_FP8_E4M3, MAIN_SWEEPS, REFINE_SWEEPS, 0)}
+    paths = {"fp32": (tdx.TDX_F32, PLAIN_SWEEPS, 0, 0, 0),
+             "bf16": (tdx.TDX_BF16, MAIN_SWEEPS, REFINE_SWEEPS, args.opening, args.opening_split),
+             "bf16_fp32_opening": (tdx.TDX_BF16, MAIN_SWEEPS, REFINE_SWEEPS, args.opening, 0),
+             "bf16_no_opening": (tdx.TDX_BF16, MAIN_SWEEPS, REFINE_SWEEPS, 0, 0),
+             "fp8": (tdx.TDX_FP8_E4M3, MAIN_SWEEPS, REFINE_SWEEPS, args.opening, 0),
+             "fp8_no_opening": (tdx.TDX_FP8_E4M3, MAIN_SWEEPS, REFINE_SWEEPS, 0, 0)}
+    if not args.opening_split:
+        del paths["bf16_fp32_opening"]
     per_seed = []
     for seed in seeds:
         a = anchor["seeds"].get(str(seed)) if same else None
         ref = a["relerr"][-1] if a else None
         target = ref + TARGET_DELTA if a else None
         row = {"seed": seed, "fp64_anchor": ref, "target": target}
-        for name, (storage, sweeps, refine, opening) in paths.items():
-            rep, fit, ms, U = run(storage, sweeps, refine, seed, opening)
+        for name, (storage, sweeps, refine, opening, split) in paths.items():
+            rep, fit, ms, U = run(storage, sweeps, refine, seed, opening, split)
             rel = tdx.cp_relerr(X, dims, U, R)
             t, k = time_to_target(rep, fit, target, rel) if a else (None, None)
             row[name] = {"s": t, "sweeps": k, "relerr": rel, "diff_vs_fp64": (rel - ref) if a else None,
@@ -438,12 +448,14 @@ def main():
         "vs_baseline": None,
         "dtype": "bf16",
         "data": "synthetic (Philox low-rank + 0.1% noise, generated in HBM)",
-        "config": {"workload": f"CP-ALS {n}x{n}x{n} rank {R}: {args.opening} FP32 + {MAIN_SWEEPS - args.opening} "
-                               f"BF16-storage sweeps + {REFINE_SWEEPS} FP32 refinement sweeps vs plain FP32 "
+        "config": {"workload": f"CP-ALS {n}x{n}x{n} rank {R}: {args.opening} opening sweeps ("
+                               f"{'BF16 tensor, hi+lo BF16 Khatri-Rao operand' if args.opening_split else 'FP32 tensor'}) + "
+                               f"{MAIN_SWEEPS - args.opening} BF16-storage sweeps + {REFINE_SWEEPS} FP32 refinement sweeps vs plain FP32 "
                                f"{PLAIN_SWEEPS} sweeps; target = FP64-oracle relerr after {PLAIN_SWEEPS} sweeps from "
                                f"the same init + 1e-4; timed from the first device work (norm, BF16 conversion, panels); "
                                f"median over init seeds {seeds}",
-                   "dims": list(dims), "rank": R, "noise": 1e-3, "init_seeds": seeds, "opening_sweeps": args.opening},
+                   "dims": list(dims), "rank": R, "noise": 1e-3, "init_seeds": seeds, "opening_sweeps": args.opening,
+                   "opening_split": args.opening_split},
         "median_sec_to_target": medians,
         "seeds_reaching_target": reached,
         "speedup_bf16_vs_plain_median": round(float(np.median(speedups)), 3),

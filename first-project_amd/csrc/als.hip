@@ -762,15 +762,26 @@ static int ensure_partial(CpPlan &P, const Storage &S) {
     return TDX_OK;
 }
 
-static int storage_setup(CpPlan &P, Storage &S, const void *X, int dtype, const float *scales) {
+// split: BF16 storage streamed with the hi + lo Khatri-Rao block (no shared pass); *ok = false
+// (and nothing set up) when some mode's plan is not the LDS kernel that implements it
+static int storage_setup(CpPlan &P, Storage &S, const void *X, int dtype, const float *scales, bool split = false,
+                         bool *ok = nullptr) {
     S.X = X;
     S.dtype = dtype;
     S.scales = scales;
     for (int m = 0; m < P.order; ++m) {
         int st = mttkrp_plan(S.mp[m], P.order, P.dims, m, dtype, P.R, X);
         if (st) return st;
+        if (split) {
+            if (dtype != TDX_BF16 || !S.mp[m].lds) {
+                if (ok) *ok = false;
+                return TDX_OK;
+            }
+            S.mp[m].split = true;
+        }
     }
-    int st = dtree_plan(S.dt, P.order, P.dims, dtype, P.R, P.shared_pass);
+    if (ok) *ok = true;
+    int st = dtree_plan(S.dt, P.order, P.dims, dtype, P.R, P.shared_pass && !split);
     if (st) return st;
     return ensure_partial(P, S);
 }
@@ -1141,14 +1152,23 @@ int tdx_cp_als_ex(const float *X, int order, const long *dims, long rank, float 
         if (!st) st = storage_setup(P, Smain, Xq, TDX_FP8_E4M3, scales);
     }
     if (st) return record(st);
-    // BF16 / FP8 storage: the first `opening` of the max_sweeps main sweeps read the FP32 tensor
+    // BF16 / FP8 storage: the first `opening` of the max_sweeps main sweeps read the FP32 tensor,
+    // or (opening_split, BF16) the BF16 copy with the split Khatri-Rao block
     const int opening = (opt.storage == TDX_F32) ? 0 : std::min(opt.opening_sweeps, opt.max_sweeps);
+    Storage Sopen = S32;
+    if (opening > 0 && opt.opening_split && opt.storage == TDX_BF16) {
+        Storage Ss;
+        bool ok = false;
+        st = storage_setup(P, Ss, Xq, TDX_BF16, nullptr, true, &ok);
+        if (st) return record(st);
+        if (ok) Sopen = Ss;
+    }
     float *elapsed = opt.trace ? opt.trace->elapsed_ms : nullptr;
     int done_open = 0, done_main = 0, done_ref = 0;
-    st = refresh_factors(P, opening ? S32 : Smain, s);
+    st = refresh_factors(P, opening ? Sopen : Smain, s);
     (void)hipEventRecord(evg.e[1], s);
     if (!st && opening > 0) {
-        st = run_sweeps(P, S32, s, opening, opt.record_fit != 0, opt.use_graph != 0, sweep_ms, 0.0, 0.0, &done_open,
+        st = run_sweeps(P, Sopen, s, opening, opt.record_fit != 0, opt.use_graph != 0, sweep_ms, 0.0, 0.0, &done_open,
                         evg.e[0], elapsed);
         if (!st) st = refresh_factors(P, Smain, s);
     }

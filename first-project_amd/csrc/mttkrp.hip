@@ -614,6 +614,7 @@ int mttkrp_launch(const MttkrpPlan &p, const void *X, const float *const *panels
     const int stt = storage_type(p.dtype);
     if (p.dtype == TDX_FP8_E4M3 && p.mode == 0 && !row_scale) return record(TDX_ERR_ARG);
     if (p.lds) return mttkrp_lds_launch(p, X, panels, row_scale, partial, s, skip);
+    if (p.split) return record(TDX_ERR_UNSUPPORTED);  // the split block exists in the LDS kernel only
     if (p.fast) {
         const int KS = (stt == 0) ? 16 : 32;
         a.FB = ceil_div(a.If, KS);

@@ -29,6 +29,7 @@ struct MttkrpPlan {
     long ksteps_per_split = 0;
     int lds_cps = 1;     // LDS kernel: i_0 chunks per split
     int lds_nss = 1;     // LDS kernel: slow-index ranges per chunk group
+    bool split = false;  // BF16 storage, LDS kernel: Khatri-Rao block as hi + lo BF16 parts
     size_t partial_floats() const { return (size_t)nsplit * rows_pad * Rp; }
 };
 

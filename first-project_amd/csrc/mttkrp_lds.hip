@@ -98,8 +98,13 @@ template <int ST> struct LTr {
     typedef typename std::conditional<ST == 2, uint2, uint4>::type V;
 };
 
-template <int ST, int NB, bool TR>
+// SPLIT (BF16 storage only): the Khatri-Rao block is kept as hi + lo BF16 parts
+// (hi = bf16(v), lo = bf16(v - hi)) and every tile is multiplied by both, so the block is
+// used to ~2^-17 relative instead of 2^-9: the MTTKRP of the BF16-rounded tensor with
+// near-FP32 factors (twice the MFMAs and block reads; the tensor traffic is unchanged).
+template <int ST, int NB, bool TR, bool SPLIT = false>
 __global__ __launch_bounds__(256, 2) void k_mttkrp_lds(LArgs a) {
+    static_assert(!SPLIT || ST == 1, "the split Khatri-Rao block is for BF16 storage");
     constexpr int KB = LTile<ST>::KB;
     constexpr int LPR = LTile<ST>::LPR;          // lanes per row in the global loads (n > 0)
     constexpr int RPP = 256 / LPR;                // rows per load pass
@@ -109,7 +114,7 @@ __global__ __launch_bounds__(256, 2) void k_mttkrp_lds(LArgs a) {
     typedef typename LTile<ST>::E LE;
     constexpr int Rp = 16 * NB;
     __shared__ __attribute__((aligned(16))) LE xs[128 * XLD];  // tensor tile [row][k]
-    __shared__ __attribute__((aligned(16))) LE bs[Rp * XLD];   // KR block [col][k]
+    __shared__ __attribute__((aligned(16))) LE bs[(SPLIT ? 2 : 1) * Rp * XLD];  // KR block [col][k] (+ lo part)
     if (a.skip && *a.skip) return;
 
     const int tid = threadIdx.x, lane = tid & 63;
@@ -293,6 +298,19 @@ __global__ __launch_bounds__(256, 2) void k_mttkrp_lds(LArgs a) {
                 for (int q = 0; q < KB / 16; ++q)
                     *reinterpret_cast<float4 *>(dst + 4 * q) =
                         float4{pc[4 * q] * w, pc[4 * q + 1] * w, pc[4 * q + 2] * w, pc[4 * q + 3] * w};
+            } else if constexpr (SPLIT) {
+#pragma unroll
+                for (int q = 0; q < KB / 32; ++q) {
+                    uint32_t hi[4], lo[4];
+#pragma unroll
+                    for (int h = 0; h < 4; ++h) {
+                        const float v0 = pc[8 * q + 2 * h] * w, v1 = pc[8 * q + 2 * h + 1] * w;
+                        hi[h] = lpack_bf16(v0, v1);
+                        lo[h] = lpack_bf16(v0 - __uint_as_float(hi[h] << 16), v1 - __uint_as_float(hi[h] & 0xffff0000u));
+                    }
+                    *reinterpret_cast<uint4 *>(dst + 8 * q) = uint4{hi[0], hi[1], hi[2], hi[3]};
+                    *reinterpret_cast<uint4 *>(dst + Rp * XLD + 8 * q) = uint4{lo[0], lo[1], lo[2], lo[3]};
+                }
             } else {
 #pragma unroll
                 for (int q = 0; q < KB / 32; ++q)
@@ -346,6 +364,17 @@ __global__ __launch_bounds__(256, 2) void k_mttkrp_lds(LArgs a) {
 #pragma unroll
                     for (int nb = 0; nb < NB; ++nb)
                         acc[rb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[rb], bv[nb], acc[rb][nb], 0, 0, 0);
+                if constexpr (SPLIT) {
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb)
+                        bv[nb] = __builtin_bit_cast(lbf16x8, *reinterpret_cast<const uint4 *>(
+                                                                 bs + Rp * XLD + (16 * nb + lm) * XLD + 32 * q + 8 * lg));
+#pragma unroll
+                    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+                        for (int nb = 0; nb < NB; ++nb)
+                            acc[rb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[rb], bv[nb], acc[rb][nb], 0, 0, 0);
+                }
             }
         }
     };
@@ -434,21 +463,24 @@ int mttkrp_lds_launch(const MttkrpPlan &p, const void *X, const float *const *pa
         if (m != p.mode && m != fm) a.slow[a.nslow++] = m;
     const int grid = p.rowtiles * p.nsplit;
     const int stt = (p.dtype == TDX_F32) ? 0 : (p.dtype == TDX_BF16) ? 1 : 2;
-#define TDX_LDS_LAUNCH(ST_, TR_)                                                  \
-    switch (p.Rp / 16) {                                                          \
-        case 1: k_mttkrp_lds<ST_, 1, TR_><<<grid, 256, 0, s>>>(a); break;         \
-        case 2: k_mttkrp_lds<ST_, 2, TR_><<<grid, 256, 0, s>>>(a); break;         \
-        case 3: k_mttkrp_lds<ST_, 3, TR_><<<grid, 256, 0, s>>>(a); break;         \
-        default: k_mttkrp_lds<ST_, 4, TR_><<<grid, 256, 0, s>>>(a); break;        \
+#define TDX_LDS_LAUNCH(ST_, TR_, SP_)                                                  \
+    switch (p.Rp / 16) {                                                               \
+        case 1: k_mttkrp_lds<ST_, 1, TR_, SP_><<<grid, 256, 0, s>>>(a); break;         \
+        case 2: k_mttkrp_lds<ST_, 2, TR_, SP_><<<grid, 256, 0, s>>>(a); break;         \
+        case 3: k_mttkrp_lds<ST_, 3, TR_, SP_><<<grid, 256, 0, s>>>(a); break;         \
+        default: k_mttkrp_lds<ST_, 4, TR_, SP_><<<grid, 256, 0, s>>>(a); break;        \
     }
+    if (p.split && stt != 1) return record(TDX_ERR_UNSUPPORTED);
     if (p.mode == 0) {
-        if (stt == 0) { TDX_LDS_LAUNCH(0, true) }
-        else if (stt == 1) { TDX_LDS_LAUNCH(1, true) }
-        else { TDX_LDS_LAUNCH(2, true) }
+        if (stt == 0) { TDX_LDS_LAUNCH(0, true, false) }
+        else if (stt == 1 && p.split) { TDX_LDS_LAUNCH(1, true, true) }
+        else if (stt == 1) { TDX_LDS_LAUNCH(1, true, false) }
+        else { TDX_LDS_LAUNCH(2, true, false) }
     } else {
-        if (stt == 0) { TDX_LDS_LAUNCH(0, false) }
-        else if (stt == 1) { TDX_LDS_LAUNCH(1, false) }
-        else { TDX_LDS_LAUNCH(2, false) }
+        if (stt == 0) { TDX_LDS_LAUNCH(0, false, false) }
+        else if (stt == 1 && p.split) { TDX_LDS_LAUNCH(1, false, true) }
+        else if (stt == 1) { TDX_LDS_LAUNCH(1, false, false) }
+        else { TDX_LDS_LAUNCH(2, false, false) }
     }
 #undef TDX_LDS_LAUNCH
     TDX_LAUNCH_CHECK();

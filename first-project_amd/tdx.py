@@ -51,7 +51,8 @@ class CpOptions(ctypes.Structure):
                 ("record_fit", ctypes.c_int), ("line_search", ctypes.c_int), ("use_graph", ctypes.c_int),
                 ("mlp_params", _P), ("mlp_width", ctypes.c_int), ("ngrid", ctypes.c_int),
                 ("grid", ctypes.c_double * 8), ("trace", ctypes.POINTER(CpTrace)), ("target_relerr", ctypes.c_double),
-                ("shared_pass", ctypes.c_int), ("opening_sweeps", ctypes.c_int)]
+                ("shared_pass", ctypes.c_int), ("opening_sweeps", ctypes.c_int),
+                ("opening_split", ctypes.c_int)]
 
 
 class CpReport(ctypes.Structure):

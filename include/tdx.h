@@ -222,6 +222,11 @@ typedef struct {
                                 sweeps read the FP32 tensor (default 0).  The early sweeps decide
                                 which components ALS resolves; rounding the Khatri-Rao operand to
                                 BF16 there can settle a run on another plateau */
+    int opening_split;       /* BF16 storage: the opening sweeps stream the BF16 copy with the
+                                Khatri-Rao operand split into hi + lo BF16 parts (near-FP32
+                                factors, BF16-rounded tensor) instead of the FP32 tensor (default
+                                0; falls back to the FP32 tensor where a mode's MTTKRP is not the
+                                LDS-staged kernel) */
 } tdx_cp_options;
 
 /* features of the extrapolation MLP, computed on the device after every sweep k:

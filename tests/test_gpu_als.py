@@ -252,3 +252,36 @@ def test_nonfinite_tensor_reports_status():
             tdx.cp_als_ex(dev(Xh), dims, R, Ud, o)
         assert e.value.status == tdx.TDX_ERR_NONFINITE
         assert tdx.lib().tdx_last_status() == tdx.TDX_ERR_NONFINITE
+
+
+def test_opening_split_is_als_of_the_bf16_tensor_with_fp32_factors():
+    """BF16 storage with the split (hi + lo BF16) Khatri-Rao block, the schedule's cheaper
+    opening sweeps: it is the ALS of the BF16-rounded tensor with near-FP32 factors, so its
+    factors after 6 sweeps match the plain FP32 path run on the dequantised tensor (relative
+    5e-5: the block is exact to 2^-17) and their direct errors against that tensor agree to 2e-7 -- where the plain BF16
+    sweeps (factor operand rounded to 2^-9) miss both by orders of magnitude more.  Orders 3
+    and 4, rank 16 and 8 (the transposed mode-0 kernel and the n > 0 kernel)."""
+    import torch
+    for dims, R in (((256, 96, 80), 16), ((64, 48, 40, 24), 8)):
+        X, _ = make_cp_tensor(dims, R, eta=1e-2)
+        Xb = torch.empty(X.numel(), dtype=torch.int16, device="cuda")
+        tdx.quantize_bf16(X, Xb)
+        Xd = empty(X.numel())
+        tdx.lib().tdx_dequantize_bf16(tdx.ptr(Xb), tdx.ptr(Xd), X.numel())
+        U0 = init_factors(dims, R, seed=3)
+        runs = {}
+        for name, (Xs, storage, opening, split) in {"split": (X, tdx.TDX_BF16, 6, 1),
+                                                     "ref": (Xd, tdx.TDX_F32, 0, 0),
+                                                     "bf16": (X, tdx.TDX_BF16, 0, 0)}.items():
+            Ud = [dev(u.ravel(order="F")) for u in U0]
+            o = tdx.default_cp_options()
+            o.max_sweeps, o.refine_sweeps, o.storage, o.init = 6, 0, storage, 1
+            o.opening_sweeps, o.opening_split, o.shared_pass = opening, split, 0
+            tdx.cp_als_ex(Xs, dims, R, Ud, o)
+            runs[name] = (Ud, tdx.cp_relerr(Xd, dims, Ud, R))
+        ds = max(relnorm(host(a), host(b)) for a, b in zip(runs["split"][0], runs["ref"][0]))
+        db = max(relnorm(host(a), host(b)) for a, b in zip(runs["bf16"][0], runs["ref"][0]))
+        es, eb = abs(runs["split"][1] - runs["ref"][1]), abs(runs["bf16"][1] - runs["ref"][1])
+        print(dims, R, "factor diff split", ds, "bf16", db, "relerr diff split", es, "bf16", eb)
+        assert ds <= 5e-5 and es <= 2e-7
+        assert db > 10 * ds
