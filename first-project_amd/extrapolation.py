@@ -187,6 +187,8 @@ def main():
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--max-sweeps", type=int, default=400)
     ap.add_argument("--out", default="extrapolation_mlp")
+    ap.add_argument("--eval-only", action="store_true",
+                    help="evaluate the saved weights (--out) on the held-out seeds without training")
     args = ap.parse_args()
     dims = (args.dim,) * 4
 
@@ -194,6 +196,13 @@ def main():
         print(m, file=sys.stderr, flush=True)
 
     t0 = time.time()
+    if args.eval_only:
+        P, meta = load(args.out)
+        seeds = range(2000, 2000 + args.eval_seeds)
+        rows, target = evaluate(seeds, P, dims, args.rank, max_sweeps=args.max_sweeps, log=log)
+        print(json.dumps({"target_relerr": target, "weights": meta, "eval_seeds": [2000, 1999 + args.eval_seeds],
+                          "summary": summarize(rows), "per_seed": rows, "wall_s": time.time() - t0}))
+        return
     feats, labels = collect(range(args.train_seeds), dims, args.rank, args.sweeps, log)
     log(f"collected {feats.shape[0]} samples in {time.time() - t0:.1f}s; mean label {labels.mean():.3f}")
     P, hist = train(feats, labels, epochs=args.epochs)
