@@ -200,8 +200,12 @@ __device__ inline bool sub_orth(double *X, int n, int r, double *S, double *M, d
 // (column-major) or null for the built-in one.  On success writes U (n x r FP32, may alias U0)
 // and w (optional) like sym_top_eigvecs_block and returns true; returns false having written
 // nothing.  Launch with kEigThreads threads and eig_sub_doubles(n) * 8 bytes of dynamic LDS.
+// rotate = false (HOOI before its last iteration, where only the span of U matters: every
+// later product contracts U U^T): when the span is certified cheaply (below) U is the
+// orthonormal basis V itself, in no particular order and without the Rayleigh-Ritz rotation;
+// otherwise the full path runs.
 __device__ inline bool sym_top_eigvecs_subspace(const double *Ag, int n, int r, const float *U0, float *U,
-                                                double *w, double *smem) {
+                                                double *w, double *smem, bool rotate = true) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, lr = lane & 15, lk = lane >> 4;
     double *V = smem;
     double *Z = V + kSubRows * kSubLd;
@@ -315,6 +319,51 @@ __device__ inline bool sym_top_eigvecs_subspace(const double *Ag, int n, int r, 
         Z = t;
     }
     if (!conv || !(hf > 0.0)) return false;
+
+    if (!rotate) {
+        // Span certificate without the eigenvalues of H: with orthonormal V and residual rho,
+        // r eigenvalues lambda_k of A lie within rho of the eigenvalues theta_k of H (Kahan).  If
+        // every theta_k > t = rho + b with b^2 = ||A||_F^2 - sum (theta_k - rho)^2 (+ rounding),
+        // then lambda_k > b, and every other eigenvalue has lambda^2 <= ||A||_F^2 -
+        // sum lambda_k^2 <= b^2: the span of V is the top-r invariant subspace.  sum (theta_k -
+        // rho)^2 = ||H||_F^2 - 2 rho tr(H) + r rho^2, and theta_min > t holds exactly when
+        // H - t I has a Cholesky factorisation with positive pivots.
+        double tr = 0.0;
+        for (int c = 0; c < r; ++c) tr += H[c * kSubHld + c];
+        const double ss = hf * hf - 2.0 * rho * tr + (double)r * rho * rho;
+        const double b2 = fmax(f2 - ss, 0.0) + 1e-12 * f2;
+        const double t = rho + sqrt(b2);
+        for (int e = tid; e < r * r; e += kEigThreads) {
+            const int a = e / r, bb = e % r;
+            S[a * kSubHld + bb] = H[a * kSubHld + bb] - ((a == bb) ? t : 0.0);
+        }
+        if (tid == 0) *flag = 1.0;
+        __syncthreads();
+        // right-looking Cholesky of the r x r block, one barrier per pivot
+        for (int k = 0; k < r; ++k) {
+            const double d = S[k * kSubHld + k];
+            if (!(d > 0.0)) {
+                if (tid == 0) *flag = 0.0;
+                break;  // uniform: every thread read the same pivot
+            }
+            const double dinv = 1.0 / d;
+            const int m2 = r - k - 1;
+            for (int e = tid; e < m2 * m2; e += kEigThreads) {
+                const int i = k + 1 + e / m2, j = k + 1 + e % m2;
+                S[i * kSubHld + j] = fma(-S[i * kSubHld + k] * dinv, S[k * kSubHld + j], S[i * kSubHld + j]);
+            }
+            __syncthreads();
+        }
+        __syncthreads();
+        if (*flag > 0.0) {
+            for (int e = tid; e < n * r; e += kEigThreads) {
+                const int i = e % n, c = e / n;
+                U[(long)c * n + i] = (float)V[i * kSubLd + c];
+            }
+            return true;
+        }
+        __syncthreads();
+    }
 
     // ---- Rayleigh-Ritz: H = W diag(theta) W^T by cyclic Jacobi, round-robin pairs ----
     // Thread (group g = tid / 32, index j = tid % 32) handles pairs g and g + 8: it computes

@@ -104,10 +104,10 @@ __global__ void k_gram_sum(const float *__restrict__ part, int nblk, long In, do
 // device) receives 1 for the subspace path, 0 for the tridiagonal one.
 __global__ __launch_bounds__(kEigThreads) void k_top_eigvecs(const double *__restrict__ G, int n, int r,
                                                              const float *U0, int try_sub, float *U, double *w,
-                                                             double *__restrict__ scratch, int *method) {
+                                                             double *__restrict__ scratch, int *method, int rotate) {
     extern __shared__ double smem[];
     if (try_sub && r <= kSubMaxR) {
-        if (sym_top_eigvecs_subspace(G, n, r, U0, U, w, smem)) {
+        if (sym_top_eigvecs_subspace(G, n, r, U0, U, w, smem, rotate != 0)) {
             if (method && threadIdx.x == 0) *method = 1;
             return;
         }
@@ -211,9 +211,10 @@ static long eig_kernel_lds_doubles(int n) {
 }
 
 // scratch: 2 n^2 doubles.  try_sub: try the subspace path first, starting from U0 (or the
-// built-in block when U0 is null).
+// built-in block when U0 is null).  rotate = 0: an orthonormal basis of the top-r subspace is
+// enough (the subspace path may then skip its Rayleigh-Ritz rotation; w is not written then).
 static int top_eigvecs(const double *G, long n, long r, double *scratch, const float *U0, int try_sub, float *U,
-                       double *w, int *method, hipStream_t s) {
+                       double *w, int *method, hipStream_t s, int rotate = 1) {
     if (n < 1 || n > kEigMaxN || r < 1 || r > n) return record(TDX_ERR_ARG);
     static bool attr_set = false;
     if (!attr_set) {
@@ -223,7 +224,7 @@ static int top_eigvecs(const double *G, long n, long r, double *scratch, const f
     }
     const int sub = (try_sub && subspace_enabled()) ? 1 : 0;
     k_top_eigvecs<<<1, kEigThreads, sizeof(double) * eig_kernel_lds_doubles((int)n), s>>>(
-        G, (int)n, (int)r, U0, sub, U, w, scratch, method);
+        G, (int)n, (int)r, U0, sub, U, w, scratch, method, rotate);
     TDX_LAUNCH_CHECK();
     return TDX_OK;
 }
@@ -308,10 +309,11 @@ static int tucker_run(const float *X, int order, const long *dims, const long *r
         Xs = Xb;
         xtype = TDX_BF16;
     }
-    // HOSVD init
+    // HOSVD init (only the spans matter when HOOI iterations follow: every later product
+    // contracts U U^T, so the eigenvector rotation is left to the last iteration)
     for (int m = 0; m < order; ++m) {
         st = unfold_gram(Xs, xtype, order, dims, m, W, W.G, s);
-        if (!st) st = top_eigvecs(W.G, dims[m], ranks[m], W.Q, nullptr, 1, U[m], nullptr, nullptr, s);
+        if (!st) st = top_eigvecs(W.G, dims[m], ranks[m], W.Q, nullptr, 1, U[m], nullptr, nullptr, s, iters == 0);
         if (st) return st;
     }
     k_norm2_blocks<float><<<1024, 256, 0, s>>>(X, N, W.scal);
@@ -346,8 +348,10 @@ static int tucker_run(const float *X, int order, const long *dims, const long *r
                 for (int m = 0; m < order; ++m) ydims[m] = tdims[m];
                 st = ttm_seq(T, TDX_F32, order, ydims, ranks, U, sm, ns, W.y2, fa, &Y, s);
                 if (!st) st = unfold_gram(Y, TDX_F32, order, ydims, n, W, W.G, s);
-                // warm start: the factor of the previous iteration spans nearly the same subspace
-                if (!st) st = top_eigvecs(W.G, dims[n], ranks[n], W.Q, U[n], 1, U[n], nullptr, nullptr, s);
+                // warm start: the factor of the previous iteration spans nearly the same subspace;
+                // eigenvectors (ascending, sign convention) only in the last iteration
+                if (!st)
+                    st = top_eigvecs(W.G, dims[n], ranks[n], W.Q, U[n], 1, U[n], nullptr, nullptr, s, it == iters - 1);
                 if (st) return st;
             }
         }
