@@ -185,11 +185,17 @@ def extras(args, log):
     best, times = tdx.cp_batched_autotune(X, U, b, trial_sweeps=2, cache_path=cache)
     st, fit, ms = tdx.cp_als_batched(X, U, b, 30, init=0, seed=1, variant=best)
     # per-tensor loop baseline on a sample: the single-tensor driver per tensor, eager launches
-    # (a graph per tensor costs more to capture than 30 small sweeps run) and with the graph
+    # and with a graph per call.  A first untimed call on the largest tensor leaves a plan
+    # whose buffers hold every tensor of the batch, so the timed calls allocate nothing.
     loop_n = 64
     idx = np.linspace(0, b.n - 1, loop_n).astype(int)
     loop = {}
     for use_graph in (0, 1):
+        dmax, rmax = int(b.shapes.max()), int(b.ranks.max())  # 96^3, rank 16: holds every tensor
+        oo = tdx.default_cp_options()
+        oo.max_sweeps, oo.record_fit, oo.use_graph = 30, 1, use_graph
+        tdx.cp_als_ex(X[:dmax ** 3], (dmax,) * 3, rmax, [torch.empty(dmax * rmax, device="cuda") for _ in range(3)],
+                      oo)
         tot = 0.0
         for t in idx:
             dims = tuple(int(v) for v in b.shapes[t])

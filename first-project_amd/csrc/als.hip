@@ -18,6 +18,8 @@
 // A sweep is captured once into a hipGraph and replayed.
 #include "mttkrp.hpp"
 #include "linalg.hpp"
+#include <memory>
+#include <mutex>
 #include "mlp.hpp"
 #include <math.h>
 #include <string.h>
@@ -624,6 +626,11 @@ struct CpPlan {
     int *accepted_hist = nullptr;
     double *grid_hist = nullptr, *plain_hist = nullptr;
     std::vector<void *> owned;
+    // capacities of the shape-dependent buffers (a later call of another shape reuses the plan
+    // when they suffice)
+    long cap_panel[4] = {0, 0, 0, 0};  // floats per mode panel (and per mode of U1 / Ua)
+    long cap_R = 0;                    // R x R buffers hold cap_R^2 doubles
+    long cap_nblk = 0;                 // solve blocks of the longest mode
 
     ~CpPlan() { release(); }
     FacSet set_of(float *const *p) const {
@@ -674,20 +681,40 @@ struct Storage {
     DtreePlan dt;         // order 3, BF16/FP8: modes 1 and 2 from one shared pass over X
 };
 
-static int plan_setup(CpPlan &P, int order, const long *dims, long R, float *const *factors, int max_hist) {
+// sets the shape of a plan whose buffers are (or will be) large enough for it
+static void plan_bind(CpPlan &P, int order, const long *dims, long R, float *const *factors) {
     P.order = order;
-    for (int m = 0; m < order; ++m) P.dims[m] = dims[m];
+    for (int m = 0; m < 4; ++m) P.dims[m] = m < order ? dims[m] : 1;
     P.R = R;
     P.Rp = (int)(ceil_div(R, 16) * 16);
+    for (int m = 0; m < 4; ++m) P.U[m] = m < order ? factors[m] : nullptr;
+}
+
+// whether a cached plan's buffers hold this shape
+static bool plan_fits(const CpPlan &P, int order, const long *dims, long R, int max_hist) {
+    if (P.order != order || R > P.cap_R || max_hist > P.max_sweeps_hist) return false;
+    const int Rp = (int)(ceil_div(R, 16) * 16);
     long nblk_max = 0;
     for (int m = 0; m < order; ++m) {
-        P.U[m] = factors[m];
-        TDX_HIP(hipMalloc(&P.panel[m], sizeof(float) * panel_floats(dims[m], P.Rp)));
+        if (panel_floats(dims[m], Rp) > P.cap_panel[m] || dims[m] * R > P.cap_panel[m]) return false;
+        nblk_max = std::max(nblk_max, ceil_div(dims[m], kSolveRows));
+    }
+    return nblk_max <= P.cap_nblk;
+}
+
+static int plan_setup(CpPlan &P, int order, const long *dims, long R, float *const *factors, int max_hist) {
+    plan_bind(P, order, dims, R, factors);
+    long nblk_max = 0;
+    P.cap_R = R;
+    for (int m = 0; m < order; ++m) {
+        P.cap_panel[m] = std::max(panel_floats(dims[m], P.Rp), dims[m] * R);
+        TDX_HIP(hipMalloc(&P.panel[m], sizeof(float) * P.cap_panel[m]));
         TDX_HIP(hipMalloc(&P.G[m], sizeof(double) * R * R));
         TDX_HIP(hipMalloc(&P.Vinv[m], sizeof(double) * R * R));
         long nb = ceil_div(dims[m], kSolveRows);
         if (nb > nblk_max) nblk_max = nb;
     }
+    P.cap_nblk = nblk_max;
     TDX_HIP(hipMalloc(&P.ip_part, sizeof(double) * nblk_max));
     TDX_HIP(hipMalloc(&P.gram_part, sizeof(double) * nblk_max * R * R));
     P.max_sweeps_hist = max_hist < 1 ? 1 : max_hist;
@@ -708,9 +735,9 @@ static int plan_setup(CpPlan &P, int order, const long *dims, long R, float *con
         };
         int rc = 0;
         for (int m = 0; m < order && !rc; ++m) {
-            rc = alloc((void **)&P.U1[m], sizeof(float) * dims[m] * R);
-            if (!rc) rc = alloc((void **)&P.Ua[m], sizeof(float) * dims[m] * R);
-            if (!rc) rc = alloc((void **)&P.panel_a[m], sizeof(float) * panel_floats(dims[m], P.Rp));
+            rc = alloc((void **)&P.U1[m], sizeof(float) * P.cap_panel[m]);
+            if (!rc) rc = alloc((void **)&P.Ua[m], sizeof(float) * P.cap_panel[m]);
+            if (!rc) rc = alloc((void **)&P.panel_a[m], sizeof(float) * P.cap_panel[m]);
             if (!rc) rc = alloc((void **)&P.Ga[m], sizeof(double) * R * R);
         }
         if (!rc) rc = alloc((void **)&P.ls, sizeof(LsState));
@@ -1036,6 +1063,52 @@ static int run_sweeps(CpPlan &P, const Storage &S, hipStream_t s, int sweeps, bo
     return st;
 }
 
+// ---- plan cache: the device buffers of a CP-ALS call are kept for the next call of the same
+// shape (a loop over many tensors of one shape then allocates nothing after the first call).
+// A plan is taken out of the cache while a call uses it, so concurrent calls never share one.
+struct PlanKey {
+    int ls_mode = 0, ngrid = 0, mlp_width = 0;
+    bool operator==(const PlanKey &o) const {
+        return ls_mode == o.ls_mode && ngrid == o.ngrid && mlp_width == o.mlp_width;
+    }
+};
+static constexpr size_t kPlanCacheSize = 4;
+static std::mutex g_plan_mutex;
+static std::vector<std::pair<PlanKey, std::unique_ptr<CpPlan>>> g_plans;  // most recent first
+
+// a cached plan of the same line-search configuration whose buffers hold the shape
+static std::unique_ptr<CpPlan> take_plan(const PlanKey &k, int order, const long *dims, long R, int max_hist) {
+    std::lock_guard<std::mutex> lk(g_plan_mutex);
+    for (size_t i = 0; i < g_plans.size(); ++i)
+        if (g_plans[i].first == k && plan_fits(*g_plans[i].second, order, dims, R, max_hist)) {
+            std::unique_ptr<CpPlan> p = std::move(g_plans[i].second);
+            g_plans.erase(g_plans.begin() + (long)i);
+            return p;
+        }
+    return nullptr;
+}
+
+static void give_plan(const PlanKey &k, std::unique_ptr<CpPlan> p) {
+    std::unique_ptr<CpPlan> evicted;
+    {
+        std::lock_guard<std::mutex> lk(g_plan_mutex);
+        g_plans.insert(g_plans.begin(), std::make_pair(k, std::move(p)));
+        if (g_plans.size() > kPlanCacheSize) {
+            evicted = std::move(g_plans.back().second);
+            g_plans.pop_back();
+        }
+    }
+    // evicted (if any) is released here, outside the lock
+}
+
+void release_cp_plans() {
+    std::vector<std::pair<PlanKey, std::unique_ptr<CpPlan>>> old;
+    {
+        std::lock_guard<std::mutex> lk(g_plan_mutex);
+        old.swap(g_plans);
+    }
+}
+
 }  // namespace tdx
 
 using namespace tdx;
@@ -1090,24 +1163,39 @@ int tdx_cp_als_ex(const float *X, int order, const long *dims, long rank, float 
             }
         }
     } guard{owned};
-    CpPlan P;
     const int total = opt.max_sweeps + (opt.storage == TDX_F32 ? 0 : opt.refine_sweeps);
     if (opt.line_search < 0 || opt.line_search > 3) return record(TDX_ERR_ARG);
+    if (opt.line_search == 1 && (!opt.mlp_params || opt.mlp_width <= 0 || opt.mlp_width > 128))
+        return record(TDX_ERR_ARG);
+    if (opt.line_search == 3 && (opt.ngrid <= 0 || opt.ngrid > 8)) return record(TDX_ERR_ARG);
+    // the plan (device buffers, side stream, events) of an earlier call with the same shape is
+    // reused; it returns to the cache only when this call succeeds
+    PlanKey key{};
+    key.ls_mode = opt.line_search;
+    key.ngrid = opt.line_search == 3 ? opt.ngrid : 0;
+    key.mlp_width = opt.line_search == 1 ? opt.mlp_width : 64;
+    std::unique_ptr<CpPlan> Pp = take_plan(key, order, dims, rank, total);
+    const bool fresh = !Pp;
+    if (fresh) Pp.reset(new CpPlan);
+    CpPlan &P = *Pp;
     P.ls_mode = opt.line_search;
     P.shared_pass = opt.shared_pass != 0;
     if (P.ls_mode == 1) {
-        if (!opt.mlp_params || opt.mlp_width <= 0 || opt.mlp_width > 128) return record(TDX_ERR_ARG);
         P.mlp = opt.mlp_params;
         P.mlp_width = opt.mlp_width;
     }
     if (P.ls_mode == 3) {
-        if (opt.ngrid <= 0 || opt.ngrid > 8) return record(TDX_ERR_ARG);
         P.ngrid = opt.ngrid;
         for (int g = 0; g < opt.ngrid; ++g) P.grid[g] = opt.grid[g];
     }
     if (P.ls_mode) opt.record_fit = 1;
-    int st = plan_setup(P, order, dims, rank, factors, total);
-    if (st) return st;
+    int st = TDX_OK;
+    if (fresh) {
+        st = plan_setup(P, order, dims, rank, factors, total);
+        if (st) return st;
+    } else {
+        plan_bind(P, order, dims, rank, factors);
+    }
     if (P.ls_mode) k_ls_init<<<1, 1, 0, s>>>(P.ls);
     // initialisation: modes 1..d-1 from the seeded generator unless the caller supplied them
     if (opt.init == 0) {
@@ -1218,6 +1306,7 @@ int tdx_cp_als_ex(const float *X, int order, const long *dims, long rank, float 
     }
     int result = status ? TDX_WARN_SINGULAR : TDX_OK;
     if (opt.record_fit && count > 0 && !isfinite(last)) result = TDX_ERR_NONFINITE;
+    give_plan(key, std::move(Pp));
     if (report) {
         report->sweeps_done = done_main + done_ref;
         report->status = result;

@@ -44,6 +44,8 @@ enum WsSlot {
 };
 int workspace(int slot, size_t bytes, void **ptr);
 void release_workspace();
+// frees the cached CP-ALS plans (als.hip); tdx_release_workspace calls it
+void release_cp_plans();
 // the fixed device scratch of tdx_mttkrp_ex_noalloc (64 MiB per GPU): its address on the
 // current device, taken once per device by hipGetSymbolAddress (mttkrp.hip)
 constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;

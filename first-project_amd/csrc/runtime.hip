@@ -134,6 +134,7 @@ int tdx_copy_to_device(void *dst, const void *src, size_t bytes) {
 }
 
 int tdx_release_workspace(void) {
+    tdx::release_cp_plans();
     tdx::release_workspace();
     return tdx::record(TDX_OK);
 }

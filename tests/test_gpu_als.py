@@ -285,3 +285,31 @@ def test_opening_split_is_als_of_the_bf16_tensor_with_fp32_factors():
         print(dims, R, "factor diff split", ds, "bf16", db, "relerr diff split", es, "bf16", eb)
         assert ds <= 5e-5 and es <= 2e-7
         assert db > 10 * ds
+
+
+def test_plan_reuse_across_shapes_is_bit_identical():
+    """The device buffers of a CP-ALS call are kept for later calls whose shape fits them: a
+    smaller call after a larger one (other extents and rank, orders 3 and 4) gives the factors and
+    fits of the same call on fresh buffers, bit for bit."""
+    cases = [((40, 36, 32), 8), ((24, 20, 16), 4), ((20, 16, 12, 10), 5), ((12, 10, 8, 6), 3)]
+    data = []
+    for dims, R in cases:
+        X, _ = make_cp_tensor(dims, R)
+        data.append((X, dims, R, init_factors(dims, R, seed=11)))
+
+    def run(X, dims, R, U0):
+        Ud = [dev(u.ravel(order="F")) for u in U0]
+        o = tdx.default_cp_options()
+        o.max_sweeps, o.init, o.record_fit = 5, 1, 1
+        st, rep, fit, _ = tdx.cp_als_ex(X, dims, R, Ud, o)
+        return fit, [host(u) for u in Ud]
+
+    fresh = []
+    for case in data:
+        tdx.lib().tdx_release_workspace()
+        fresh.append(run(*case))
+    tdx.lib().tdx_release_workspace()
+    for (X, dims, R, U0), (f0, u0) in zip(data, fresh):  # large, then small: the small ones reuse
+        f1, u1 = run(X, dims, R, U0)
+        assert np.array_equal(f1, f0)
+        assert all(np.array_equal(a, b) for a, b in zip(u1, u0))
