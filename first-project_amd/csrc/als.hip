@@ -344,6 +344,47 @@ __global__ __launch_bounds__(256) void k_norm2_partial(const T *__restrict__ X, 
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
+// ||X||^2 in FP64 (per-thread Kahan over float4 groups, block partials) with 16-byte
+// nontemporal loads, four in flight per thread; with Q the same pass writes the BF16 copy of X
+// (round to nearest even, the bits tdx_quantize_bf16 writes).  X and Xq 16-byte aligned.
+template <bool Q>
+__global__ __launch_bounds__(256) void k_norm2_stream(const float *__restrict__ X, long n, uint16_t *__restrict__ Xq,
+                                                      double *__restrict__ part) {
+    __shared__ double sh[256];
+    double s = 0.0, c = 0.0;
+    auto add = [&](double y) {
+        y -= c;
+        const double t = s + y;
+        c = (t - s) - y;
+        s = t;
+    };
+    auto take = [&](const float4 &v, long i4) {
+        add((double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z + (double)v.w * v.w);
+        if constexpr (Q)
+            reinterpret_cast<uint2 *>(Xq)[i4] =
+                uint2{(uint32_t)f32_to_bf16_bits(v.x) | ((uint32_t)f32_to_bf16_bits(v.y) << 16),
+                      (uint32_t)f32_to_bf16_bits(v.z) | ((uint32_t)f32_to_bf16_bits(v.w) << 16)};
+    };
+    const float4 *X4 = reinterpret_cast<const float4 *>(X);
+    const long n4 = n >> 2, stride = (long)gridDim.x * 256;
+    long i = blockIdx.x * 256L + threadIdx.x;
+    for (; i + 3 * stride < n4; i += 4 * stride) {
+        float4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = ld_stream(X4 + i + u * stride);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) take(v[u], i + u * stride);
+    }
+    for (; i < n4; i += stride) take(ld_stream(X4 + i), i);
+    if (blockIdx.x == 0)
+        for (long e = 4 * n4 + threadIdx.x; e < n; e += 256) {
+            add((double)X[e] * X[e]);
+            if constexpr (Q) Xq[e] = f32_to_bf16_bits(X[e]);
+        }
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
 __global__ void k_sum_partials(const double *__restrict__ part, int n, double *__restrict__ out) {
     __shared__ double sh[256];
     double s = 0.0;
@@ -827,16 +868,27 @@ static int refresh_factors(CpPlan &P, const Storage &S, hipStream_t s) {
     return TDX_OK;
 }
 
+// ||X||^2 of the FP32 tensor into *out; with Xbf (non-null) the same pass writes its BF16 copy.
+// scratch holds 4096 doubles.
 static int norm2(const void *X, int dtype, const float *scales, long I, long n, double *scratch, double *out,
-                 hipStream_t s) {
+                 hipStream_t s, uint16_t *Xbf = nullptr) {
+    if (dtype != TDX_F32) return record(TDX_ERR_UNSUPPORTED);
+    (void)scales;
+    (void)I;
+    const bool aligned = ((((uintptr_t)X) | ((uintptr_t)Xbf)) & 15) == 0;
+    if (aligned) {
+        const int blocks = std::min(4096, std::max(1, device_cus() * 8));
+        if (Xbf) k_norm2_stream<true><<<blocks, 256, 0, s>>>((const float *)X, n, Xbf, scratch);
+        else k_norm2_stream<false><<<blocks, 256, 0, s>>>((const float *)X, n, nullptr, scratch);
+        k_sum_partials<<<1, 256, 0, s>>>(scratch, blocks, out);
+        TDX_LAUNCH_CHECK();
+        return TDX_OK;
+    }
     const int blocks = 1024;
-    if (dtype == TDX_F32)
-        k_norm2_partial<float><<<blocks, 256, 0, s>>>((const float *)X, n, nullptr, I, scratch);
-    else
-        return record(TDX_ERR_UNSUPPORTED);
+    k_norm2_partial<float><<<blocks, 256, 0, s>>>((const float *)X, n, nullptr, I, scratch);
     k_sum_partials<<<1, 256, 0, s>>>(scratch, blocks, out);
     TDX_LAUNCH_CHECK();
-    (void)scales;
+    if (Xbf) return tdx_quantize_bf16((const float *)X, Xbf, n);
     return TDX_OK;
 }
 
@@ -1217,20 +1269,23 @@ int tdx_cp_als_ex(const float *X, int order, const long *dims, long rank, float 
     (void)hipEventRecord(evg.e[0], s);
     k_set_int<<<1, 1, 0, s>>>(P.counter, 0);
     k_set_int<<<1, 1, 0, s>>>(P.status, 0);
-    st = norm2(X, TDX_F32, nullptr, dims[0], N, P.scratch, P.normx2, s);
-    if (st) return st;
-
-    // storage copies for the main phase (workspace pool: no allocation after the first call)
+    // storage copies for the main phase (workspace pool: no allocation after the first call);
+    // the BF16 copy is written by the pass that sums ||X||^2
     void *Xq = nullptr;
     float *scales = nullptr;
+    if (opt.storage == TDX_BF16) {
+        st = workspace(WS_STORE, sizeof(uint16_t) * N, &Xq);
+        if (st) return st;
+    }
+    st = norm2(X, TDX_F32, nullptr, dims[0], N, P.scratch, P.normx2, s,
+               opt.storage == TDX_BF16 ? (uint16_t *)Xq : nullptr);
+    if (st) return st;
     Storage S32;
     st = storage_setup(P, S32, X, TDX_F32, nullptr);
     if (st) return st;
     Storage Smain = S32;
     if (opt.storage == TDX_BF16) {
-        st = workspace(WS_STORE, sizeof(uint16_t) * N, &Xq);
-        if (!st) st = tdx_quantize_bf16(X, (uint16_t *)Xq, N);
-        if (!st) st = storage_setup(P, Smain, Xq, TDX_BF16, nullptr);
+        st = storage_setup(P, Smain, Xq, TDX_BF16, nullptr);
     } else if (opt.storage == TDX_FP8_E4M3) {
         void *sp = nullptr;
         st = workspace(WS_STORE, N, &Xq);
