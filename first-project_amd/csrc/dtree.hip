@@ -21,16 +21,17 @@
 //     A operand is A_new^T from LDS, the B operand X read straight from HBM into registers.
 //     The contraction order over i0 is free, so each lane owns 32 contiguous bytes of a
 //     column per "run" (two k-steps of BF16, four of FP8; FP8 widened exactly to BF16 with
-//     v_cvt_scalef32_pk_bf16_fp8) and a wave reads whole 128-byte lines of 16 columns, 64
-//     bytes of each per instruction (this operand-order load streams at ~4.3 TB/s against
-//     ~6.2-6.7 for 128-512 contiguous bytes per column per instruction);
+//     v_cvt_scalef32_pk_bf16_fp8).  Each 16-byte load instruction reads 8 whole 128-byte
+//     lines (8 columns), and one DPP lane exchange (lane ^ 8) turns the pair of loads into
+//     the operand order, where a load of 16 columns x 64 bytes each (half lines) streamed at
+//     ~4.3 TB/s against ~6.2-6.7 for whole lines (513 -> 460 us at 1024^3 BF16 R 64);
 //   * chunks of 4 (BF16) / 8 (FP8) k-steps (8 KB of X per wave) rotate through four
 //     register buffers across the i2 loop, three chunks in flight while one is multiplied
 //     (nontemporal loads; every load is issued unconditionally, the chunks past the end
 //     re-read a valid one), and after the last chunk of a column i2 the tile is multiplied
 //     into the mode-1 partial by C[i2][:] (from LDS) and stored to T (16-byte nontemporal
-//     stores, r fastest; 501 us against 527 with ordinary stores at 1024^3 BF16) after the
-//     next column's first chunk;
+//     stores, r fastest, whole lines per instruction by the same lane exchange; nontemporal
+//     501 us against 527 with ordinary stores) after the next column's first chunk;
 //   * the mode-1 partial of the workgroup's i2 range goes to partial[split][i1][Rp], the
 //     layout k_solve sums (in split order, FP64).
 // k_m2: lane (i2, 4 consecutive r) walks an i1 range of T rows (16-byte loads, a T row is
@@ -55,6 +56,23 @@ __device__ __forceinline__ uint32_t dt_pack(float a, float b) {
 __device__ __forceinline__ uint2 dt_fp8x4(uint32_t q) {
     return uint2{__builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_scalef32_pk_bf16_fp8(q, 1.0f, false)),
                  __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_scalef32_pk_bf16_fp8(q, 1.0f, true))};
+}
+
+// Lanes l and l ^ 8 (DPP row_ror:8 within 16 lanes; hi: bit 3 of l set) exchange halves:
+// p' = hi ? q(l ^ 8) : p,  q' = hi ? q : p(l ^ 8).  Written with update_dpp and an explicit
+// old value: mov_dpp applied to the elements of a vector was merged into one exchange for all
+// elements by this compiler.
+__device__ __forceinline__ void swap_halves(uint32_t &p, uint32_t &q, bool hi) {
+    const int s = (int)(hi ? p : q);
+    const uint32_t x = (uint32_t)__builtin_amdgcn_update_dpp(s, s, 0x128, 0xf, 0xf, false);
+    p = hi ? x : p;
+    q = hi ? q : x;
+}
+__device__ __forceinline__ void swap_halves(uint4 &p, uint4 &q, bool hi) {
+    swap_halves(p.x, q.x, hi);
+    swap_halves(p.y, q.y, hi);
+    swap_halves(p.z, q.z, hi);
+    swap_halves(p.w, q.w, hi);
 }
 
 // ES: bytes per element; KPL: k-steps (32 i0 each) per 32-byte lane run; CH: k-steps per chunk
@@ -135,9 +153,15 @@ __global__ __launch_bounds__(256, 1) void k_ttm_m1(DtArgs a) {
     const long colb = a.I0 * ES, slabb = a.I1 * colb;  // bytes per column i1, per slab i2
     constexpr long chb = (long)RUNS * 128;             // bytes of one chunk of a column
     const long total = (long)len * NCH;
+    // Loads in line order: the two 16-byte loads of a lane's 32-byte run read columns
+    // 16c + (lm & 7) (first) and 16c + 8 + (lm & 7) (second) at bytes 32g + 16 (lm >> 3) of the
+    // run, so each load instruction reads 8 whole 128-byte lines; one lane-bit-3 exchange
+    // (swap_halves) then gives each lane its operand-order 32 bytes of column 16c + lm.
+    const bool hi8 = (lm & 8) != 0;
     const unsigned char *xl[2];
 #pragma unroll
-    for (int c = 0; c < 2; ++c) xl[c] = X + (i2s * a.I1 + i1w + 16 * c + lm) * colb + 8L * KPL * g * ES;
+    for (int c = 0; c < 2; ++c)
+        xl[c] = X + (i2s * a.I1 + i1w + 16 * c + (lm & 7)) * colb + 8L * KPL * g * ES + 16 * (lm >> 3);
     const long last_off = (long)(len - 1) * slabb;  // re-read target past the end (valid)
 
     dt_f32x4 acc[NRB][2], m1[NRB][2];
@@ -155,13 +179,17 @@ __global__ __launch_bounds__(256, 1) void k_ttm_m1(DtArgs a) {
         for (int c = 0; c < 2; ++c)
 #pragma unroll
             for (int j = 0; j < RUNS; ++j) {
-                const uint4 *p = reinterpret_cast<const uint4 *>(xl[c] + off + j * 128);
-                b[c][j][0] = ld_stream(p);
-                b[c][j][1] = ld_stream(p + 1);
+                const unsigned char *p = xl[c] + off + j * 128;
+                b[c][j][0] = ld_stream(reinterpret_cast<const uint4 *>(p));
+                b[c][j][1] = ld_stream(reinterpret_cast<const uint4 *>(p + 8 * colb));
             }
     };
-    auto compute = [&](const Buf &b, int kc, int zoff) {
+    auto compute = [&](Buf &b, int kc, int zoff) {
         const uint4 *ab = aop + zoff + kc * (CH * NRB * 64) + lane;
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int j = 0; j < RUNS; ++j) swap_halves(b[c][j][0], b[c][j][1], hi8);
 #pragma unroll
         for (int j = 0; j < RUNS; ++j)
 #pragma unroll
@@ -205,13 +233,30 @@ __global__ __launch_bounds__(256, 1) void k_ttm_m1(DtArgs a) {
             }
         }
     };
+    // A tile register holds, per store, 16 T rows x 64 bytes (half of each 128-byte line).
+    // For Rp >= 32 the two tiles of a row-block pair (rb 2p, 2p+1: r in [32p, 32p+32), one
+    // line of each row) exchange lane bit 3 (DPP row_ror:8 is lane ^ 8 within 16 lanes), so
+    // that each store writes 8 rows x one whole line: rows lm & 7 from the first register,
+    // rows 8 + (lm & 7) from the second, r = 32p + 16 (lm >> 3) + 4g.
     auto store_t = [&](long i2l) {
-        float *tp = a.T + ((i2s + i2l) * a.I1 + i1w + lm) * a.Rp + 4 * g;
-#pragma unroll
-        for (int rb = 0; rb < NRB; ++rb)
+        if constexpr (NRB == 1) {
+            float *tp = a.T + ((i2s + i2l) * a.I1 + i1w + lm) * a.Rp + 4 * g;
 #pragma unroll
             for (int c = 0; c < 2; ++c)
-                __builtin_nontemporal_store(tsv[rb][c], reinterpret_cast<dt_f32x4 *>(tp + 16L * c * a.Rp + 16 * rb));
+                __builtin_nontemporal_store(tsv[0][c], reinterpret_cast<dt_f32x4 *>(tp + 16L * c * a.Rp));
+        } else {
+            float *tp = a.T + ((i2s + i2l) * a.I1 + i1w + (lm & 7)) * a.Rp + 16 * (lm >> 3) + 4 * g;
+#pragma unroll
+            for (int p = 0; p < NRB / 2; ++p)
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    uint4 lo = __builtin_bit_cast(uint4, tsv[2 * p][c]), hi = __builtin_bit_cast(uint4, tsv[2 * p + 1][c]);
+                    swap_halves(lo, hi, hi8);
+                    float *q = tp + 16L * c * a.Rp + 32 * p;
+                    __builtin_nontemporal_store(__builtin_bit_cast(dt_f32x4, lo), reinterpret_cast<dt_f32x4 *>(q));
+                    __builtin_nontemporal_store(__builtin_bit_cast(dt_f32x4, hi), reinterpret_cast<dt_f32x4 *>(q + 8L * a.Rp));
+                }
+        }
     };
 
     // The chunk sequence runs column after column (NCH chunks each); chunk q lives in buffer
