@@ -17,6 +17,11 @@
 //     Row RL g + t of the tile (row group g) sits in LDS row RG t + g, so the 16-byte stores
 //     of a wave go to consecutive LDS rows (no bank conflicts); the MFMA rows follow the LDS
 //     order and are mapped back to tile rows when the partial is written;
+//   * n = 0, FP8 storage: the tile stays FP8 in LDS in its global order, [i_1][128 rows]
+//     (16 KB instead of 32 KB widened; 16-byte loads, 8 lanes per whole 128-byte line, stored
+//     as they arrive), and the MFMA A operand comes from ds_read_b64_tr_b8 (8 rows x 8 i_1
+//     bytes per 16-lane group, transposed) widened to BF16 in registers -- no register
+//     transpose, half the tile's LDS bytes;
 //   * the KB x Rp Khatri-Rao block  P_0[chunk, :] * w_slow  is formed once per step in LDS
 //     for all four waves (each thread keeps its 32 panel values of the chunk in registers
 //     for the whole slow loop);
@@ -105,15 +110,17 @@ template <int ST> struct LTr {
 template <int ST, int NB, bool TR, bool SPLIT = false>
 __global__ __launch_bounds__(256, 2) void k_mttkrp_lds(LArgs a) {
     static_assert(!SPLIT || ST == 1, "the split Khatri-Rao block is for BF16 storage");
+    constexpr bool F8T = (ST == 2) && TR;       // FP8 mode 0: FP8 tile [k][row] in LDS
     constexpr int KB = LTile<ST>::KB;
     constexpr int LPR = LTile<ST>::LPR;          // lanes per row in the global loads (n > 0)
     constexpr int RPP = 256 / LPR;                // rows per load pass
-    constexpr int NPASS = TR ? 8 : 128 / RPP;     // loads per thread per step
+    constexpr int NPASS = F8T ? 4 : TR ? 8 : 128 / RPP;  // loads per thread per step
     static_assert(!TR || 8 * LTr<ST>::KG == KB, "TR tile covers KB fast indices");
     constexpr int XLD = KB + LTile<ST>::PAD;      // LDS row pitch (elements)
     typedef typename LTile<ST>::E LE;
     constexpr int Rp = 16 * NB;
-    __shared__ __attribute__((aligned(16))) LE xs[128 * XLD];  // tensor tile [row][k]
+    // tensor tile: [row][k] (F8T: 128 k-rows of 128 FP8 bytes, 16-byte granules swizzled)
+    __shared__ __attribute__((aligned(16))) LE xs[F8T ? 128 * 128 / 2 : 128 * XLD];
     __shared__ __attribute__((aligned(16))) LE bs[(SPLIT ? 2 : 1) * Rp * XLD];  // KR block [col][k] (+ lo part)
     if (a.skip && *a.skip) return;
 
@@ -150,8 +157,10 @@ __global__ __launch_bounds__(256, 2) void k_mttkrp_lds(LArgs a) {
     // global-load geometry.  n > 0: this thread loads 16 bytes of rows  tid / LPR + RPP * p
     // at fast offset lcol.  TR: rows RL * (tid % RG) .. + RL - 1 at fast offsets
     // 8 * (tid / RG) + p (the row group is whole: the planner requires In % RL == 0).
-    const int lrow = TR ? LTr<ST>::RL * (tid % LTr<ST>::RG) : tid / LPR;
-    const int lcol = TR ? 8 * (tid / LTr<ST>::RG) : (tid % LPR) * ((ST == 0) ? 4 : (ST == 1) ? 8 : 16);
+    // F8T: this thread loads rows 16 (tid % 8) .. + 15 (16 bytes) of fast index tid / 8 + 32 p
+    // (8 lanes read one whole 128-byte line per fast index; the planner requires In % 16 == 0).
+    const int lrow = F8T ? 16 * (tid % 8) : TR ? LTr<ST>::RL * (tid % LTr<ST>::RG) : tid / LPR;
+    const int lcol = F8T ? tid / 8 : TR ? 8 * (tid / LTr<ST>::RG) : (tid % LPR) * ((ST == 0) ? 4 : (ST == 1) ? 8 : 16);
     long roff[NPASS];
     bool rok[NPASS];
 #pragma unroll
@@ -187,7 +196,7 @@ __global__ __launch_bounds__(256, 2) void k_mttkrp_lds(LArgs a) {
     };
     const int wc = kr_on ? kr_c : 0;
 
-    typedef typename std::conditional<TR, typename LTr<ST>::V, uint4>::type VT;
+    typedef typename std::conditional<TR && !F8T, typename LTr<ST>::V, uint4>::type VT;
     // one slow step's tensor loads (masked lanes recorded in ok) and its slow-factor values
     struct TileBuf {
         VT x[NPASS];
@@ -215,7 +224,7 @@ __global__ __launch_bounds__(256, 2) void k_mttkrp_lds(LArgs a) {
 #pragma unroll
         for (int p = 0; p < NPASS; ++p) {
             // fast index of this load; out-of-range loads read the tile origin instead
-            const long f = fc * KB + lcol + (TR ? p : 0);
+            const long f = fc * KB + lcol + (F8T ? 32 * p : TR ? p : 0);
             const bool fok = f < a.If;
             const bool in = rok[p] && fok;
             const long e = off + roff[p] + (fok ? f : 0) * sf;
@@ -233,7 +242,19 @@ __global__ __launch_bounds__(256, 2) void k_mttkrp_lds(LArgs a) {
     // tile crosses the edge of the rows or of the fast extent)
     auto write_tile = [&](const TileBuf &b, auto masked_tag) {
         constexpr bool MASKED = decltype(masked_tag)::value;
-        if constexpr (TR) {
+        if constexpr (F8T) {
+            // fast index k = lcol + 32 p, 16-byte granule g = lrow / 16 of its 128-byte k-row,
+            // stored at granule g ^ ((k >> 1) & 7): the 16 lanes of a write cover two whole
+            // k-rows, and the 8 k-rows a transposing read takes (same g) hit 8 distinct
+            // granule positions x two row parities -- no bank conflicts either way
+            unsigned char *x8 = reinterpret_cast<unsigned char *>(xs);
+#pragma unroll
+            for (int p = 0; p < NPASS; ++p) {
+                const int k = lcol + 32 * p;
+                const uint4 v = (!MASKED || ((b.ok >> p) & 1u)) ? b.x[p] : uint4{0u, 0u, 0u, 0u};
+                *reinterpret_cast<uint4 *>(x8 + k * 128 + 16 * ((lrow >> 4) ^ ((k >> 1) & 7))) = v;
+            }
+        } else if constexpr (TR) {
             if constexpr (ST == 0) {
                 // 8 loads of 4 rows -> tile rows lrow + t, 8 fast indices each (two 16-byte stores)
                 uint4 v[8];
@@ -352,9 +373,26 @@ __global__ __launch_bounds__(256, 2) void k_mttkrp_lds(LArgs a) {
             for (int q = 0; q < KB / 32; ++q) {
                 lbf16x8 av[2], bv[NB];
 #pragma unroll
-                for (int rb = 0; rb < 2; ++rb)
-                    av[rb] = __builtin_bit_cast(
-                        lbf16x8, *reinterpret_cast<const uint4 *>(xs + (32 * wave + 16 * rb + lm) * XLD + 32 * q + 8 * lg));
+                for (int rb = 0; rb < 2; ++rb) {
+                    if constexpr (F8T) {
+                        // ds_read_b64_tr_b8: in each 16-lane group, lane l receives byte l & 7
+                        // of the 8-byte pieces addressed by lanes 2j + (l >> 3), j = 0..7.  Lane
+                        // 2j + h of group lg addresses fast index k = 32 q + 8 lg + j, rows
+                        // 32 wave + 16 rb + 8 h .. + 7, so lane (lm, lg) gets X(row lm, k =
+                        // 32 q + 8 lg + j) for j = 0..7: the A operand, widened here to BF16.
+                        const int k = 32 * q + 8 * lg + (lm >> 1);
+                        const int gr = 2 * wave + rb;  // 16-row granule
+                        const unsigned char *src = reinterpret_cast<const unsigned char *>(xs) + k * 128 +
+                                                   16 * (gr ^ ((k >> 1) & 7)) + 8 * (lm & 1);
+                        typedef int lv2i __attribute__((ext_vector_type(2)));
+                        const lv2i w = __builtin_amdgcn_ds_read_tr8_b64_v2i32(
+                            (__attribute__((address_space(3))) lv2i *)(src));
+                        av[rb] = __builtin_bit_cast(lbf16x8, lfp8x8_to_bf16(uint2{(uint32_t)w.x, (uint32_t)w.y}));
+                    } else {
+                        av[rb] = __builtin_bit_cast(
+                            lbf16x8, *reinterpret_cast<const uint4 *>(xs + (32 * wave + 16 * rb + lm) * XLD + 32 * q + 8 * lg));
+                    }
+                }
 #pragma unroll
                 for (int nb = 0; nb < NB; ++nb)
                     bv[nb] = __builtin_bit_cast(
@@ -404,14 +442,14 @@ __global__ __launch_bounds__(256, 2) void k_mttkrp_lds(LArgs a) {
         }
     }
     // this wave's 32 rows of the partial: lane holds rows 4 lg + v of each 16-row block
-    // (TR: LDS row j holds tile row RL (j % RG) + j / RG)
+    // (TR other than FP8: LDS row j holds tile row RL (j % RG) + j / RG)
     float *dst = a.partial + (split * a.rows_pad + row0) * Rp;
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int j = 32 * wave + 16 * rb + 4 * lg + v;
-            const int tr = TR ? LTr<ST>::RL * (j % LTr<ST>::RG) + j / LTr<ST>::RG : j;
+            const int tr = (TR && !F8T) ? LTr<ST>::RL * (j % LTr<ST>::RG) + j / LTr<ST>::RG : j;
             float sc = 1.0f;
             if constexpr (TR && ST == 2) {
                 const long r = row0 + tr;
@@ -423,10 +461,10 @@ __global__ __launch_bounds__(256, 2) void k_mttkrp_lds(LArgs a) {
 }
 
 // covered for rank <= 64 and orders 3 and 4 (at most two slow modes); mode 0 needs whole
-// row groups (In a multiple of 4 rows for FP32, 8 otherwise)
+// row groups (In a multiple of 4 rows for FP32, 8 for BF16, 16 for FP8)
 bool mttkrp_lds_covers(int order, int mode, long rank, long In, int dtype) {
     if (rank > 64 || order < 3 || order > 4) return false;
-    if (mode == 0) return In % ((dtype == TDX_F32) ? 4 : 8) == 0;
+    if (mode == 0) return In % ((dtype == TDX_F32) ? 4 : (dtype == TDX_FP8_E4M3) ? 16 : 8) == 0;
     return true;
 }
 

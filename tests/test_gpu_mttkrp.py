@@ -123,12 +123,14 @@ def test_fp8_storage(dims, R):
     assert relnorm(run_mttkrp(q, tdx.TDX_FP8_E4M3, s, dims, U, 0), ref32) < 0.05
 
 
-def test_fp8_all_codes_decode():
+@pytest.mark.parametrize("I", [8, 16])
+def test_fp8_all_codes_decode(I):
     """Every finite E4M3 code through the MTTKRP: X = codes (scale 1 per slice would need
-    amax 448), factors of ones, so mode-0 output row i = sum of its decoded codes."""
+    amax 448), factors of ones, so mode-0 output row i = sum of its decoded codes.  I = 8
+    runs the register-tile kernel, I = 16 the LDS kernel's FP8 tile (transposing LDS reads)."""
     import torch
     codes = np.array([c for c in range(256) if (c & 0x7f) != 0x7f], np.uint8)  # 254 finite codes
-    I, J = 8, 32
+    J = 256 // I
     X = np.zeros(I * J, np.uint8)
     X[:codes.size] = codes
     q = dev(X).to(torch.uint8)
@@ -141,6 +143,26 @@ def test_fp8_all_codes_decode():
         U[1] = U1
         got = run_mttkrp(q, tdx.TDX_FP8_E4M3, s, dims, U, 0)[:, 0]
         assert np.array_equal(got, O.e4m3_decode(X[j * I:(j + 1) * I])), j
+
+
+@pytest.mark.parametrize("dims,R", [((256, 96, 3), 16), ((128, 200, 2), 64), ((48, 130, 5), 24),
+                                    ((144, 72, 4, 3), 32)])
+def test_fp8_mode0_exact(dims, R):
+    """Mode 0 from FP8 storage (the LDS kernel's FP8 tile: full and partial row tiles, full
+    and partial i_1 chunks, order 4) on codes of {0, +-0.5, +-1, +-2} with slice scales 1 and
+    integer factors: every product and partial sum is exact in FP32, so the result equals the
+    oracle's bit for bit."""
+    import torch
+    dec_all = O.e4m3_decode(np.arange(256, dtype=np.uint8))
+    pick = [int(np.flatnonzero(dec_all == v)[0]) for v in (0.0, 0.5, -0.5, 1.0, -1.0, 2.0, -2.0)]
+    rng = np.random.default_rng(sum(dims) + R)
+    Xc = np.array(pick, np.uint8)[rng.integers(0, len(pick), size=int(np.prod(dims)))]
+    U = [rng.integers(-2, 3, size=(d, R)).astype(np.float32) for d in dims]
+    q = dev(Xc).to(torch.uint8)
+    s = dev(np.ones(dims[0], np.float32))
+    got = run_mttkrp(q, tdx.TDX_FP8_E4M3, s, dims, U, 0)
+    ref = O.mttkrp(O.e4m3_decode(Xc), dims, [u.astype(np.float64) for u in U], 0)
+    assert np.array_equal(got, ref.astype(np.float32))
 
 
 # ---------------------------------------------------------------------------
