@@ -13,8 +13,9 @@
 // Kernels per mode: k_vinv (one workgroup, symmetric sweep-operator inverse, Jacobi
 // pseudo-inverse fallback), the fused MTTKRP (or the shared pass, order 3 BF16/FP8), then
 // k_solve (one workgroup per 16 rows: FP64 sum of the split-K partials, U = M V^{-1}, panel,
-// fit and Gram partials) and k_gram_psum (Gram of the new factor); after the last mode
-// k_gram_psum_fit (its Gram and the sweep's fit in one kernel).
+// fit and Gram partials) and k_gram_psum (Gram of the new factor; at R <= 16 summed by the
+// next mode's k_vinv instead); after the last mode k_gram_psum_fit (its Gram and the
+// sweep's fit in one kernel).
 // A sweep is captured once into a hipGraph and replayed.
 #include "mttkrp.hpp"
 #include "linalg.hpp"
@@ -31,6 +32,21 @@ static constexpr int kMaxR = 64;     // rank limit of the LDS-resident solve
 static constexpr int kSolveRows = 16;  // rows per k_solve workgroup
 static constexpr int kGramRows = 64;   // rows per Gram-partial workgroup
 
+// p[0] + p[stride] + ... + p[(n - 1) stride], added in index order, 16 loads in flight (the
+// ones past n re-read a valid entry and are not added)
+__device__ __forceinline__ double ordered_sum(const double *p, long stride, int n) {
+    double s = 0.0;
+    for (int b = 0; b < n; b += 16) {
+        double v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = p[(long)(b + u < n ? b + u : n - 1) * stride];
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+            if (b + u < n) s += v[u];
+    }
+    return s;
+}
+
 // V = hadamard_{m != mode} G_m; Vinv = V^{-1} by the symmetric sweep operator (FP64, the
 // register-row form of sweep_inverse_rows); if a pivot fails, a pseudo-inverse from the
 // Jacobi eigendecomposition (eigenvalues below 1e-12 * max dropped) and status |= 1.
@@ -38,32 +54,105 @@ static constexpr int kGramRows = 64;   // rows per Gram-partial workgroup
 // About 4 KB of LDS, so that the workgroup fits on a CU beside an MTTKRP or TTM workgroup
 // (up to 136 KB): V is read from the Grams, and the rare fallback works in fb_ws (global,
 // 2 R^2 doubles: a copy of V and the eigenvectors).
+// R <= 16: the same sweep operator in one wave, without barriers.  Lane l holds A[i][c] for
+// row i = l % 16 and columns c = l / 16 + 4 s (s = 0..3).  A stays symmetric under the sweep,
+// so pivot k's row equals its column, which sits in slot k / 4 of the lanes with
+// l / 16 = k % 4 and reaches every lane by ds_bpermute.  Returns 1 (uniform) on a failed pivot.
+template <class VAt>
+__device__ inline int sweep_inverse_wave16(VAt vat, double *Vinv, int R) {
+    const int l = threadIdx.x & 63, i = l & 15, cg = l >> 4;
+    double a[4], a0[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int c = cg + 4 * s;
+        a[s] = (i < R && c < R) ? vat(i, c) : 0.0;
+        a0[s] = a[s];
+    }
+    int bad = 0;  // uniform: every lane reads the same pivot
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        if (k < R && !bad) {
+            const int src = 16 * (k & 3);
+            const double v = a[k >> 2], v0 = a0[k >> 2];
+            const double d = __shfl(v, src + k), dk = __shfl(v0, src + k);
+            const double ti = __shfl(v, src + i);  // A[i][k]
+            double wr[4];                           // A[k][cg + 4 s] = A[cg + 4 s][k]
+#pragma unroll
+            for (int s = 0; s < 4; ++s) wr[s] = __shfl(v, src + cg + 4 * s);
+            if (!(d > 1e-13 * dk) || !isfinite(d)) {
+                bad = 1;
+            } else {
+                const double dinv = 1.0 / d;
+                const double t = ti * dinv;
+                const bool rk = i == k;
+                const double alpha = rk ? 0.0 : 1.0, beta = rk ? dinv : -t;
+#pragma unroll
+                for (int s = 0; s < 4; ++s) a[s] = fma(beta, wr[s], alpha * a[s]);
+                if (cg == (k & 3)) a[k >> 2] = rk ? -dinv : t;
+            }
+        }
+    }
+    if (bad) return 1;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int c = cg + 4 * s;
+        if (i < R && c < R) Vinv[i + c * R] = -a[s];
+    }
+    return 0;
+}
+
+// The inverse for one workgroup of 256 threads (the body of k_vinv).  G: the current Grams
+// (G[mode] unused).
 template <int NS>
-__global__ __launch_bounds__(256) void k_vinv(const double *__restrict__ G0, const double *__restrict__ G1,
-                                              const double *__restrict__ G2, const double *__restrict__ G3,
-                                              int order, int mode, int R, double *__restrict__ Vinv,
-                                              int *__restrict__ status, double *__restrict__ fb_ws) {
+__device__ void vinv_block(const double *const *G, int order, int mode, int R, double *Vinv, int *status,
+                           double *fb_ws) {
     __shared__ double Wb[128];
     __shared__ double dg[kMaxR];
     __shared__ double sh[256];
     __shared__ double cs[4 * (kMaxR / 2 + 1)];
     __shared__ int fail;
-    const double *G[4] = {G0, G1, G2, G3};
     auto vat = [&](int i, int c) {
         double v = 1.0;
         for (int m = 0; m < order; ++m)
             if (m != mode) v *= G[m][i + c * R];
         return v;
     };
-    for (int k = threadIdx.x; k < R; k += blockDim.x) dg[k] = vat(k, k);
-    // (sweep_inverse_rows starts with a barrier)
-    if (sweep_inverse_rows<NS>(vat, dg, Wb, Vinv, R, &fail)) {
+    int bad;
+    if constexpr (NS == 4) {
+        if (threadIdx.x < 64) {
+            const int b = sweep_inverse_wave16(vat, Vinv, R);
+            if (threadIdx.x == 0) fail = b;
+        }
+        __syncthreads();
+        bad = fail;
+    } else {
+        for (int k = threadIdx.x; k < R; k += blockDim.x) dg[k] = vat(k, k);
+        // (sweep_inverse_rows starts with a barrier)
+        bad = sweep_inverse_rows<NS>(vat, dg, Wb, Vinv, R, &fail);
+    }
+    if (bad) {
         double *L = fb_ws, *Q = fb_ws + R * R;
         for (int e = threadIdx.x; e < R * R; e += blockDim.x) L[e] = vat(e % R, e / R);
         __syncthreads();
         pinv_jacobi(L, Q, Vinv, R, cs, sh);
         if (threadIdx.x == 0) atomicOr(status, 1);
     }
+}
+
+// gram_part != nullptr: first G_gmode (Gsum) = the ordered sum of the nblk Gram partials of
+// that mode (what k_gram_psum computes, same order), so a small-rank sweep, whose inverses
+// run in line, needs no separate Gram kernel between a solve and the next mode's inverse.
+template <int NS>
+__global__ __launch_bounds__(256) void k_vinv(const double *G0, const double *G1, const double *G2, const double *G3,
+                                              int order, int mode, int R, double *__restrict__ Vinv,
+                                              int *__restrict__ status, double *__restrict__ fb_ws,
+                                              const double *__restrict__ gram_part, int nblk, double *Gsum) {
+    const double *G[4] = {G0, G1, G2, G3};
+    if (gram_part) {
+        for (int e = threadIdx.x; e < R * R; e += blockDim.x) Gsum[e] = ordered_sum(gram_part + e, (long)R * R, nblk);
+        __syncthreads();
+    }
+    vinv_block<NS>(G, order, mode, R, Vinv, status, fb_ws);
 }
 
 // M = sum of the split-K partials of the MTTKRP, FP64, in split order (deterministic);
@@ -222,20 +311,6 @@ __global__ __launch_bounds__(256) void k_solve(const float *__restrict__ partial
     }
 }
 
-// p[0] + p[stride] + ... + p[(n - 1) stride], added in index order, 16 loads in flight (the
-// ones past n re-read a valid entry and are not added)
-__device__ __forceinline__ double ordered_sum(const double *p, long stride, int n) {
-    double s = 0.0;
-    for (int b = 0; b < n; b += 16) {
-        double v[16];
-#pragma unroll
-        for (int u = 0; u < 16; ++u) v[u] = p[(long)(b + u < n ? b + u : n - 1) * stride];
-#pragma unroll
-        for (int u = 0; u < 16; ++u)
-            if (b + u < n) s += v[u];
-    }
-    return s;
-}
 
 // G = sum of the block Gram partials, in block order; one thread per entry
 __global__ __launch_bounds__(256) void k_gram_psum(const double *__restrict__ gram_part, int nblk, int RR,
@@ -913,15 +988,20 @@ static int enqueue_sweep(CpPlan &P, const Storage &S, hipStream_t s, bool fit) {
         const bool dt = S.dt.on && n > 0;
         return R > 16 && (dt || S.mp[n].rowtiles * (long)S.mp[n].nsplit <= device_cus());
     };
-    auto launch_vinv = [&](int n, hipStream_t vs) {
+    // gmode >= 0: the inverse first sums mode gmode's Gram partials (gnblk blocks) into G_gmode
+    auto launch_vinv = [&](int n, hipStream_t vs, int gmode = -1, int gnblk = 0) {
         const double *g2 = P.order > 2 ? P.G[2] : P.G[0], *g3 = P.order > 3 ? P.G[3] : P.G[0];
+        const double *gp = gmode >= 0 ? P.gram_part : nullptr;
+        double *gs = gmode >= 0 ? P.G[gmode] : nullptr;
         if (R <= 16) k_vinv<4><<<1, 256, 0, vs>>>(P.G[0], P.G[1], g2, g3, P.order, n, R, P.Vinv[n], P.status,
-                                                P.vinv_ws);
+                                                P.vinv_ws, gp, gnblk, gs);
         else if (R <= 32) k_vinv<8><<<1, 256, 0, vs>>>(P.G[0], P.G[1], g2, g3, P.order, n, R, P.Vinv[n], P.status,
-                                                     P.vinv_ws);
+                                                     P.vinv_ws, gp, gnblk, gs);
         else k_vinv<16><<<1, 256, 0, vs>>>(P.G[0], P.G[1], g2, g3, P.order, n, R, P.Vinv[n], P.status,
-                                         P.vinv_ws);
+                                         P.vinv_ws, gp, gnblk, gs);
     };
+    // R <= 16 (every inverse in line): mode n's Gram is summed by mode n + 1's inverse
+    int pending_gram = -1, pending_nblk = 0;
     // forked[n]: mode n's inverse was already forked to the side stream (after the previous
     // mode's solve, behind that mode's Gram sum)
     bool forked[4] = {false, false, false, false};
@@ -933,8 +1013,9 @@ static int enqueue_sweep(CpPlan &P, const Storage &S, hipStream_t s, bool fit) {
                 TDX_HIP(hipEventRecord(P.ev_fork[n], s));
                 TDX_HIP(hipStreamWaitEvent(P.side, P.ev_fork[n], 0));
             }
-            launch_vinv(n, beside ? P.side : s);
+            launch_vinv(n, beside ? P.side : s, pending_gram, pending_nblk);
             TDX_LAUNCH_CHECK();
+            pending_gram = -1;
             if (beside) TDX_HIP(hipEventRecord(P.ev_join[n], P.side));
         }
         const float *panels[4] = {P.panel[0], P.panel[1], P.panel[2], P.panel[3]};
@@ -986,6 +1067,9 @@ static int enqueue_sweep(CpPlan &P, const Storage &S, hipStream_t s, bool fit) {
                 TDX_LAUNCH_CHECK();
                 TDX_HIP(hipEventRecord(P.ev_join[n + 1], P.side));
                 forked[n + 1] = true;
+            } else if (R <= 16 && n + 1 < P.order && !beside_of(n + 1)) {
+                pending_gram = n;  // the next inverse sums it (one launch fewer per mode)
+                pending_nblk = nblk;
             } else {
                 k_gram_psum<<<gb, 256, 0, s>>>(P.gram_part, nblk, R * R, P.G[n]);
             }
