@@ -18,8 +18,9 @@ whose relative error is <= anchor + 1e-4 (and its final direct error is too).
    operand (near-FP32 factors), the rest by a BF16 one.  Rounding the Khatri-Rao operand in
    the first sweeps settles some runs on another plateau, rounding the tensor does not (FP32
    ALS on the BF16-rounded tensor reaches every seed's FP64 plateau).  Reported beside it: the
-   same schedule opening with FP32 sweeps (--opening-split 1 only), no opening sweeps, and the
-   FP8-E4M3 storage.  Time to target = device time from the call's first work (norm of X, the
+   same schedule opening with FP32 sweeps (--opening-split 1 only), no opening sweeps, and
+   FP8-E4M3 storage with the same three openings (its split opening streams a BF16 copy made
+   in the pass that also takes ||X||^2 and the FP8 slice maxima).  Time to target = device time from the call's first work (norm of X, the
    BF16 conversion of X, factor panels) to the end of the sweep that reaches the target.
    value = median over the init seeds (a seed that misses counts as infinite).
  * ms_per_step: --warmup untimed, then exactly --steps timed BF16 sweeps (hipEvents).
@@ -327,6 +328,7 @@ def main():
     run(tdx.TDX_BF16, w, 1, 1, opening=1)
     run(tdx.TDX_BF16, w, 1, 1, opening=1, split=1)
     run(tdx.TDX_FP8_E4M3, w, 1, 1, opening=1)
+    run(tdx.TDX_FP8_E4M3, w, 1, 1, opening=1, split=1)
     torch.cuda.synchronize()
 
     # --- the timed step loop: exactly --steps BF16 sweeps after --warmup untimed ones ---
@@ -346,10 +348,12 @@ def main():
              "bf16": (tdx.TDX_BF16, MAIN_SWEEPS, REFINE_SWEEPS, args.opening, args.opening_split),
              "bf16_fp32_opening": (tdx.TDX_BF16, MAIN_SWEEPS, REFINE_SWEEPS, args.opening, 0),
              "bf16_no_opening": (tdx.TDX_BF16, MAIN_SWEEPS, REFINE_SWEEPS, 0, 0),
-             "fp8": (tdx.TDX_FP8_E4M3, MAIN_SWEEPS, REFINE_SWEEPS, args.opening, 0),
+             "fp8": (tdx.TDX_FP8_E4M3, MAIN_SWEEPS, REFINE_SWEEPS, args.opening, args.opening_split),
+             "fp8_fp32_opening": (tdx.TDX_FP8_E4M3, MAIN_SWEEPS, REFINE_SWEEPS, args.opening, 0),
              "fp8_no_opening": (tdx.TDX_FP8_E4M3, MAIN_SWEEPS, REFINE_SWEEPS, 0, 0)}
     if not args.opening_split:
         del paths["bf16_fp32_opening"]
+        del paths["fp8_fp32_opening"]
     per_seed = []
     for seed in seeds:
         a = anchor["seeds"].get(str(seed)) if same else None

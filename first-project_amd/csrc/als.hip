@@ -422,11 +422,17 @@ __global__ __launch_bounds__(256) void k_norm2_partial(const T *__restrict__ X, 
 // ||X||^2 in FP64 (per-thread Kahan over float4 groups, block partials) with 16-byte
 // nontemporal loads, four in flight per thread; with Q the same pass writes the BF16 copy of X
 // (round to nearest even, the bits tdx_quantize_bf16 writes).  X and Xq 16-byte aligned.
-template <bool Q>
+// A (FP8 storage): also the per-mode-1-slice absolute maxima (bit patterns, NaN excluded, as
+// k_slice_amax) into amax[I]; the launch makes the grid stride (in elements) a multiple of I,
+// with I % 4 == 0, so a thread's float4s all cover the same 4 slices and it issues one
+// atomicMax per slice at the end.
+template <bool Q, bool A = false>
 __global__ __launch_bounds__(256) void k_norm2_stream(const float *__restrict__ X, long n, uint16_t *__restrict__ Xq,
-                                                      double *__restrict__ part) {
+                                                      double *__restrict__ part, uint32_t *__restrict__ amax = nullptr,
+                                                      long I = 1) {
     __shared__ double sh[256];
     double s = 0.0, c = 0.0;
+    uint32_t mx[4] = {0u, 0u, 0u, 0u};
     auto add = [&](double y) {
         y -= c;
         const double t = s + y;
@@ -435,6 +441,15 @@ __global__ __launch_bounds__(256) void k_norm2_stream(const float *__restrict__ 
     };
     auto take = [&](const float4 &v, long i4) {
         add((double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z + (double)v.w * v.w);
+        if constexpr (A) {
+            const float w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                uint32_t u = __float_as_uint(w[q]) & 0x7fffffffu;
+                if (u > 0x7f800000u) u = 0u;
+                mx[q] = u > mx[q] ? u : mx[q];
+            }
+        }
         if constexpr (Q)
             reinterpret_cast<uint2 *>(Xq)[i4] =
                 uint2{(uint32_t)f32_to_bf16_bits(v.x) | ((uint32_t)f32_to_bf16_bits(v.y) << 16),
@@ -452,10 +467,18 @@ __global__ __launch_bounds__(256) void k_norm2_stream(const float *__restrict__ 
     }
     for (; i < n4; i += stride) take(ld_stream(X4 + i), i);
     if (blockIdx.x == 0)
-        for (long e = 4 * n4 + threadIdx.x; e < n; e += 256) {
+        for (long e = 4 * n4 + threadIdx.x; e < n; e += 256) {  // (none when A: I % 4 == 0)
             add((double)X[e] * X[e]);
             if constexpr (Q) Xq[e] = f32_to_bf16_bits(X[e]);
         }
+    if constexpr (A) {
+        const long i0 = blockIdx.x * 256L + threadIdx.x;
+        if (i0 < n4) {
+            const long sl = (4 * i0) % I;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) atomicMax(&amax[sl + q], mx[q]);
+        }
+    }
     s = block_sum(s, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
@@ -469,6 +492,9 @@ __global__ void k_sum_partials(const double *__restrict__ part, int n, double *_
 }
 
 __global__ void k_set_int(int *p, int v) { *p = v; }
+__global__ void k_fill_u32_dev(uint32_t *p, long n, uint32_t v) {
+    for (long i = threadIdx.x; i < n; i += blockDim.x) p[i] = v;
+}
 
 // ---------------------------------------------------------------------------
 // Extrapolation step (Tensorlab cpd_als.m:234-339 line search with the step length from
@@ -944,17 +970,42 @@ static int refresh_factors(CpPlan &P, const Storage &S, hipStream_t s) {
 }
 
 // ||X||^2 of the FP32 tensor into *out; with Xbf (non-null) the same pass writes its BF16 copy.
-// scratch holds 4096 doubles.
+// scratch holds 4096 doubles.  With amax (non-null, zeroed, I entries) the pass also takes the
+// mode-1 slice maxima when a grid of whole slice periods exists (returns 1 in *amax_done);
+// otherwise the caller computes them.
 static int norm2(const void *X, int dtype, const float *scales, long I, long n, double *scratch, double *out,
-                 hipStream_t s, uint16_t *Xbf = nullptr) {
+                 hipStream_t s, uint16_t *Xbf = nullptr, uint32_t *amax = nullptr, int *amax_done = nullptr) {
     if (dtype != TDX_F32) return record(TDX_ERR_UNSUPPORTED);
     (void)scales;
-    (void)I;
+    if (amax_done) *amax_done = 0;
     const bool aligned = ((((uintptr_t)X) | ((uintptr_t)Xbf)) & 15) == 0;
     if (aligned) {
-        const int blocks = std::min(4096, std::max(1, device_cus() * 8));
-        if (Xbf) k_norm2_stream<true><<<blocks, 256, 0, s>>>((const float *)X, n, Xbf, scratch);
-        else k_norm2_stream<false><<<blocks, 256, 0, s>>>((const float *)X, n, nullptr, scratch);
+        int blocks = std::min(4096, std::max(1, device_cus() * 8));
+        bool fuse = false;
+        if (amax && I % 4 == 0 && n % I == 0) {
+            // grid stride 1024 blocks elements: a multiple of I for blocks a multiple of
+            // I / gcd(I, 1024)
+            long g = I, h = 1024;
+            while (h) {
+                const long t = g % h;
+                g = h;
+                h = t;
+            }
+            const long per = I / g;
+            if (per <= 4096) {
+                blocks = (int)(std::max(1L, blocks / per) * per);
+                fuse = blocks <= 4096;
+            }
+        }
+        if (fuse) {
+            if (Xbf) k_norm2_stream<true, true><<<blocks, 256, 0, s>>>((const float *)X, n, Xbf, scratch, amax, I);
+            else k_norm2_stream<false, true><<<blocks, 256, 0, s>>>((const float *)X, n, nullptr, scratch, amax, I);
+            if (amax_done) *amax_done = 1;
+        } else if (Xbf) {
+            k_norm2_stream<true><<<blocks, 256, 0, s>>>((const float *)X, n, Xbf, scratch);
+        } else {
+            k_norm2_stream<false><<<blocks, 256, 0, s>>>((const float *)X, n, nullptr, scratch);
+        }
         k_sum_partials<<<1, 256, 0, s>>>(scratch, blocks, out);
         TDX_LAUNCH_CHECK();
         return TDX_OK;
@@ -1355,14 +1406,32 @@ int tdx_cp_als_ex(const float *X, int order, const long *dims, long rank, float 
     k_set_int<<<1, 1, 0, s>>>(P.status, 0);
     // storage copies for the main phase (workspace pool: no allocation after the first call);
     // the BF16 copy is written by the pass that sums ||X||^2
-    void *Xq = nullptr;
+    // BF16 / FP8 storage: the first `opening` of the max_sweeps main sweeps read the FP32 tensor,
+    // or (opening_split) a BF16 copy with the split Khatri-Rao block
+    const int opening = (opt.storage == TDX_F32) ? 0 : std::min(opt.opening_sweeps, opt.max_sweeps);
+    const bool split_open = opening > 0 && opt.opening_split && opt.storage != TDX_F32;
+    void *Xq = nullptr, *Xb = nullptr;  // main-phase copy; BF16 copy (= Xq for BF16 storage)
     float *scales = nullptr;
+    uint32_t *amax = nullptr;
     if (opt.storage == TDX_BF16) {
         st = workspace(WS_STORE, sizeof(uint16_t) * N, &Xq);
         if (st) return st;
+        Xb = Xq;
+    } else if (opt.storage == TDX_FP8_E4M3) {
+        void *sp = nullptr, *ap = nullptr;
+        st = workspace(WS_STORE, N, &Xq);
+        if (!st) st = workspace(WS_STORE_SCALES, sizeof(float) * dims[0], &sp);
+        if (!st) st = workspace(WS_MISC3, sizeof(uint32_t) * dims[0], &ap);
+        if (!st && split_open) st = workspace(WS_STORE_BF16, sizeof(uint16_t) * N, &Xb);
+        if (st) return st;
+        scales = (float *)sp;
+        amax = (uint32_t *)ap;
+        k_fill_u32_dev<<<1, 256, 0, s>>>(amax, dims[0], 0u);
     }
-    st = norm2(X, TDX_F32, nullptr, dims[0], N, P.scratch, P.normx2, s,
-               opt.storage == TDX_BF16 ? (uint16_t *)Xq : nullptr);
+    // one pass: ||X||^2, the BF16 copy (if any) and, for FP8 storage, the slice maxima when the
+    // tensor shape allows (otherwise a pass of their own)
+    int amax_done = 0;
+    st = norm2(X, TDX_F32, nullptr, dims[0], N, P.scratch, P.normx2, s, (uint16_t *)Xb, amax, &amax_done);
     if (st) return st;
     Storage S32;
     st = storage_setup(P, S32, X, TDX_F32, nullptr);
@@ -1371,22 +1440,16 @@ int tdx_cp_als_ex(const float *X, int order, const long *dims, long rank, float 
     if (opt.storage == TDX_BF16) {
         st = storage_setup(P, Smain, Xq, TDX_BF16, nullptr);
     } else if (opt.storage == TDX_FP8_E4M3) {
-        void *sp = nullptr;
-        st = workspace(WS_STORE, N, &Xq);
-        if (!st) st = workspace(WS_STORE_SCALES, sizeof(float) * dims[0], &sp);
-        scales = (float *)sp;
-        if (!st) st = tdx_quantize_fp8_mode1(X, (uint8_t *)Xq, scales, dims[0], N / dims[0]);
+        if (!amax_done) st = slice_amax(X, amax, dims[0], N / dims[0], s);
+        if (!st) st = fp8_from_amax(X, amax, (uint8_t *)Xq, scales, dims[0], N / dims[0], s);
         if (!st) st = storage_setup(P, Smain, Xq, TDX_FP8_E4M3, scales);
     }
     if (st) return record(st);
-    // BF16 / FP8 storage: the first `opening` of the max_sweeps main sweeps read the FP32 tensor,
-    // or (opening_split, BF16) the BF16 copy with the split Khatri-Rao block
-    const int opening = (opt.storage == TDX_F32) ? 0 : std::min(opt.opening_sweeps, opt.max_sweeps);
     Storage Sopen = S32;
-    if (opening > 0 && opt.opening_split && opt.storage == TDX_BF16) {
+    if (split_open) {
         Storage Ss;
         bool ok = false;
-        st = storage_setup(P, Ss, Xq, TDX_BF16, nullptr, true, &ok);
+        st = storage_setup(P, Ss, Xb, TDX_BF16, nullptr, true, &ok);
         if (st) return record(st);
         if (ok) Sopen = Ss;
     }

@@ -40,6 +40,7 @@ enum WsSlot {
     WS_MISC3,
     WS_STORE,        // CP-ALS: the BF16 / FP8 copy of the tensor for the main phase
     WS_STORE_SCALES, // CP-ALS: its FP8 slice scales
+    WS_STORE_BF16,   // CP-ALS, FP8 storage with split-operand opening sweeps: the BF16 copy
     WS_COUNT
 };
 int workspace(int slot, size_t bytes, void **ptr);
@@ -50,6 +51,11 @@ void release_cp_plans();
 // current device, taken once per device by hipGetSymbolAddress (mttkrp.hip)
 constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
 int noalloc_scratch(void **ptr);
+// FP8 mode-1 slice quantisation in steps (elementwise.hip): slice_amax zeroes and fills amax[I];
+// fp8_from_amax writes the scales (max / 448, 1 for an all-zero slice) and the E4M3 codes
+int slice_amax(const float *X, uint32_t *amax, long I, long n_rest, hipStream_t s);
+int fp8_from_amax(const float *X, const uint32_t *amax, uint8_t *Xq, float *scales, long I, long n_rest,
+                  hipStream_t s);
 
 __host__ __device__ inline long ceil_div(long a, long b) { return (a + b - 1) / b; }
 

@@ -171,6 +171,61 @@ __global__ void k_to_e4m3(const float *__restrict__ X, const float *__restrict__
         Q[e] = f32_to_e4m3_bits(X[e] / scales[e % I]);
 }
 
+// the same, 16 consecutive elements per thread (I % 16 == 0, so they are 16 consecutive
+// slices): four 16-byte nontemporal loads, one 16-byte store; X, Q 16-byte aligned
+__global__ __launch_bounds__(256) void k_to_e4m3_16(const float *__restrict__ X, const float *__restrict__ scales,
+                                                    uint8_t *__restrict__ Q, long I, long n16) {
+    const float4 *X4 = reinterpret_cast<const float4 *>(X);
+    for (long b = blockIdx.x * 256L + threadIdx.x; b < n16; b += (long)gridDim.x * 256) {
+        float4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = ld_stream(X4 + 4 * b + u);
+        const float4 *sc = reinterpret_cast<const float4 *>(scales + (16 * b) % I);
+        uint32_t w[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const float4 c = sc[u];
+            w[u] = (uint32_t)f32_to_e4m3_bits(v[u].x / c.x) | ((uint32_t)f32_to_e4m3_bits(v[u].y / c.y) << 8) |
+                   ((uint32_t)f32_to_e4m3_bits(v[u].z / c.z) << 16) | ((uint32_t)f32_to_e4m3_bits(v[u].w / c.w) << 24);
+        }
+        typedef unsigned v4u __attribute__((ext_vector_type(4)));
+        __builtin_nontemporal_store(v4u{w[0], w[1], w[2], w[3]}, reinterpret_cast<v4u *>(Q) + b);
+    }
+}
+
+// slice maxima, 4 consecutive slices per lane (I % 4 == 0, X 16-byte aligned): a wave reads
+// 1 KB of one i_rest step per load, 4 steps in flight
+__global__ __launch_bounds__(256) void k_slice_amax4(const float *__restrict__ X, uint32_t *__restrict__ amax, long I,
+                                                     long rest) {
+    const long I4 = I / 4;
+    const long q = blockIdx.x * 64L + (threadIdx.x & 63);
+    const int cg = threadIdx.x >> 6;
+    if (q >= I4) return;
+    const float4 *X4 = reinterpret_cast<const float4 *>(X);
+    uint32_t m[4] = {0u, 0u, 0u, 0u};
+    auto take = [&](const float4 &v) {
+        const float w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            uint32_t u = __float_as_uint(w[k]) & 0x7fffffffu;
+            if (u > 0x7f800000u) u = 0u;
+            m[k] = u > m[k] ? u : m[k];
+        }
+    };
+    const long step = 4L * gridDim.y;
+    long t = blockIdx.y * 4L + cg;
+    for (; t + 3 * step < rest; t += 4 * step) {
+        float4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = ld_stream(X4 + q + (t + u * step) * I4);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) take(v[u]);
+    }
+    for (; t < rest; t += step) take(ld_stream(X4 + q + t * I4));
+#pragma unroll
+    for (int k = 0; k < 4; ++k) atomicMax(&amax[4 * q + k], m[k]);
+}
+
 __device__ __forceinline__ float e4m3_to_f32(uint8_t q) {
     uint32_t e = (q >> 3) & 0xf, m = q & 7;
     float s = (q & 0x80) ? -1.0f : 1.0f;
@@ -317,20 +372,54 @@ int tdx_dequantize_bf16(const uint16_t *Xq, float *X, long n) {
     return record(TDX_OK);
 }
 
+}  // extern "C"
+
+namespace tdx {
+
+// scales from the slice maxima, then the E4M3 codes
+int fp8_from_amax(const float *X, const uint32_t *amax, uint8_t *Xq, float *scales, long I, long n_rest,
+                  hipStream_t s) {
+    const long n = I * n_rest;
+    k_scales<<<grid_for(I), kBlock, 0, s>>>(amax, scales, I);
+    const bool vec = I % 16 == 0 && ((((uintptr_t)X) | ((uintptr_t)Xq) | ((uintptr_t)scales)) & 15) == 0;
+    if (vec) {
+        const long n16 = n / 16;
+        const int blocks = (int)std::min<long>(std::max<long>(1, ceil_div(n16, 256L)), 8L * device_cus());
+        k_to_e4m3_16<<<blocks, 256, 0, s>>>(X, scales, Xq, I, n16);
+    } else {
+        k_to_e4m3<<<grid_for(n), kBlock, 0, s>>>(X, scales, Xq, I, n);
+    }
+    TDX_LAUNCH_CHECK();
+    return TDX_OK;
+}
+
+int slice_amax(const float *X, uint32_t *amax, long I, long n_rest, hipStream_t s) {
+    k_fill_u32<<<grid_for(I), kBlock, 0, s>>>(amax, I, 0u);
+    long gy = n_rest < 1024 ? n_rest : 1024;
+    if (I % 4 == 0 && (((uintptr_t)X) & 15) == 0) {
+        dim3 g((unsigned)ceil_div(I / 4, 64L), (unsigned)((gy + 3) / 4));
+        k_slice_amax4<<<g, 256, 0, s>>>(X, amax, I, n_rest);
+    } else {
+        dim3 g((unsigned)ceil_div(I, 64), (unsigned)((gy + 3) / 4));
+        k_slice_amax<<<g, 256, 0, s>>>(X, amax, I, n_rest);
+    }
+    TDX_LAUNCH_CHECK();
+    return TDX_OK;
+}
+
+}  // namespace tdx
+
+extern "C" {
+
 int tdx_quantize_fp8_mode1(const float *X, uint8_t *Xq, float *scales, long I, long n_rest) {
     TDX_CHECK_ARG(X && Xq && scales && I > 0 && n_rest > 0);
     void *ws = nullptr;
     int st = workspace(WS_MISC3, sizeof(uint32_t) * I, &ws);
     if (st) return st;
     uint32_t *amax = (uint32_t *)ws;
-    k_fill_u32<<<grid_for(I), kBlock, 0, stream()>>>(amax, I, 0u);
-    long gy = n_rest < 1024 ? n_rest : 1024;
-    dim3 g((unsigned)ceil_div(I, 64), (unsigned)((gy + 3) / 4));
-    k_slice_amax<<<g, 256, 0, stream()>>>(X, amax, I, n_rest);
-    k_scales<<<grid_for(I), kBlock, 0, stream()>>>(amax, scales, I);
-    k_to_e4m3<<<grid_for(I * n_rest), kBlock, 0, stream()>>>(X, scales, Xq, I, I * n_rest);
-    TDX_LAUNCH_CHECK();
-    return record(TDX_OK);
+    st = slice_amax(X, amax, I, n_rest, stream());
+    if (!st) st = fp8_from_amax(X, amax, Xq, scales, I, n_rest, stream());
+    return record(st);
 }
 
 int tdx_dequantize_fp8_mode1(const uint8_t *Xq, const float *scales, float *X, long I, long n_rest) {

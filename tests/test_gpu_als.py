@@ -287,6 +287,29 @@ def test_opening_split_is_als_of_the_bf16_tensor_with_fp32_factors():
         assert db > 10 * ds
 
 
+def test_fp8_storage_split_opening_is_the_bf16_split_opening():
+    """FP8 storage with split opening sweeps streams a BF16 copy made in the same pass as
+    ||X||^2 and the FP8 slice maxima; with every sweep an opening sweep its factors are those
+    of the BF16-storage split opening, bit for bit.  With 2 opening sweeps and 4 FP8 sweeps
+    after them the run stays within the FP8 rounding of the FP32 path's error."""
+    for dims, R in (((256, 96, 80), 16), ((128, 64, 40, 24), 8)):
+        X, _ = make_cp_tensor(dims, R, eta=1e-2)
+        U0 = init_factors(dims, R, seed=5)
+        res = {}
+        for name, (storage, opening) in {"fp8": (tdx.TDX_FP8_E4M3, 6), "bf16": (tdx.TDX_BF16, 6),
+                                         "fp8_main": (tdx.TDX_FP8_E4M3, 2), "fp32": (tdx.TDX_F32, 0)}.items():
+            Ud = [dev(u.ravel(order="F")) for u in U0]
+            o = tdx.default_cp_options()
+            o.max_sweeps, o.refine_sweeps, o.storage, o.init = 6, 0, storage, 1
+            o.opening_sweeps, o.opening_split = opening, 1
+            tdx.cp_als_ex(X, dims, R, Ud, o)
+            res[name] = ([host(u) for u in Ud], tdx.cp_relerr(X, dims, Ud, R))
+        for a, b in zip(res["fp8"][0], res["bf16"][0]):
+            assert np.array_equal(a, b)
+        assert np.isfinite(res["fp8_main"][1])
+        assert abs(res["fp8_main"][1] - res["fp32"][1]) < 2e-2, (res["fp8_main"][1], res["fp32"][1])
+
+
 def test_plan_reuse_across_shapes_is_bit_identical():
     """The device buffers of a CP-ALS call are kept for later calls whose shape fits them: a
     smaller call after a larger one (other extents and rank, orders 3 and 4) gives the factors and
