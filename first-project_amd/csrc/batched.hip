@@ -324,7 +324,281 @@ __global__ __launch_bounds__(BS, (BatchedOcc<BS, SLAB>::kWgs)) void k_cp_batched
     }
 }
 
+// ---------------------------------------------------------------------------
+// The split design: one MTTKRP kernel and one solve kernel per mode for the whole batch.
+// k_bm_mttkrp<MODE> carries only that mode's LDS (the slab, two factor blocks, the
+// per-column scratch and, for modes 1 and 2, the accumulated M), so three or four
+// workgroups fit on a CU where the persistent kernel's 78 KB allowed two; each workgroup
+// walks the tensors of a queue and writes M (laid out like the factors) to HBM.
+// k_bm_solve<MODE> (one workgroup per tensor) forms V from the per-tensor FP64 Grams,
+// inverts it, writes U_n and G_n, and after mode 2 the fit.  The arithmetic, including every
+// summation order, is the persistent kernel's with 256 threads: the results are the same bit
+// for bit.
+
+// G[t][m] = U_m^T U_m (FP64) for the first sweep (each solve keeps its G_n current)
+__global__ __launch_bounds__(256) void k_bm_gram_init(const float *__restrict__ U, const BDesc *__restrict__ desc,
+                                                      int n, double *__restrict__ G) {
+    __shared__ float Us[kBMaxDim * kBMaxR];
+    for (int t = blockIdx.x; t < n; t += gridDim.x) {
+        const BDesc d = desc[t];
+        const int R = d.R;
+        for (int m = 0; m < 3; ++m) {
+            const int Im = sel3(m, d.I, d.J, d.K);
+            for (int e = threadIdx.x; e < Im * R; e += 256) Us[e] = U[d.uoff[m] + e];
+            __syncthreads();
+            for (int e = threadIdx.x; e < R * R; e += 256) {
+                const int a = e % R, b = e / R;
+                double s = 0.0;
+                for (int i = 0; i < Im; ++i) s += (double)Us[i + a * Im] * (double)Us[i + b * Im];
+                G[((long)t * 3 + m) * (kBMaxR * kBMaxR) + e] = s;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+template <int MODE, int BS, int SLAB, int WGS>
+__global__ __launch_bounds__(BS, WGS) void k_bm_mttkrp(const float *__restrict__ X, const float *__restrict__ U,
+                                                       float *__restrict__ Mbuf, const BDesc *__restrict__ desc,
+                                                       const int *__restrict__ sched, int ntensors,
+                                                       int *__restrict__ queue) {
+    __shared__ __attribute__((aligned(16))) float slab[SLAB];
+    __shared__ float Fa[kBMaxDim * kBMaxR];  // mode 0: B (J x R); modes 1, 2: A^T rows [i][16]
+    __shared__ float Fb[kBMaxDim * kBMaxR];  // mode 0: C; mode 1: C; mode 2: B
+    __shared__ float Ms[MODE == 0 ? 1 : kBMaxDim * kBMaxR];
+    __shared__ float Ys[kSlabCols * kBMaxR];
+    __shared__ int slot;
+    const int tid = threadIdx.x;
+    constexpr int NW = BS / 64;
+    const int lane = tid & 63, wv = tid >> 6, lr = lane & 15, lk = lane >> 4;
+    for (;;) {
+        if (tid == 0) slot = atomicAdd(queue, 1);
+        __syncthreads();
+        const int sl = __builtin_amdgcn_readfirstlane(slot);
+        if (sl >= ntensors) break;  // uniform
+        const int t = __builtin_amdgcn_readfirstlane(sched[sl]);
+        const BDesc *dp = desc + t;
+        const int I = dp->I, J = dp->J, K = dp->K, R = dp->R;
+        const long xoff = dp->xoff, uo0 = dp->uoff[0], uo1 = dp->uoff[1], uo2 = dp->uoff[2];
+        const float *x = X + xoff;
+        if constexpr (MODE == 0) {
+            for (int e = tid; e < J * R; e += BS) Fa[e] = U[uo1 + e];
+            for (int e = tid; e < K * R; e += BS) Fb[e] = U[uo2 + e];
+        } else {
+            for (int e = tid; e < kBMaxDim * kBMaxR; e += BS) {
+                const int i = e >> 4, r = e & 15;
+                Fa[e] = (i < I && r < R) ? U[uo0 + i + r * I] : 0.0f;
+            }
+            if constexpr (MODE == 1) {
+                for (int e = tid; e < K * R; e += BS) Fb[e] = U[uo2 + e];
+                for (int e = tid; e < J * R; e += BS) Ms[e] = 0.0f;
+            } else {
+                for (int e = tid; e < J * R; e += BS) Fb[e] = U[uo1 + e];
+                for (int e = tid; e < K * R; e += BS) Ms[e] = 0.0f;
+            }
+        }
+        __syncthreads();  // the factor blocks are read by every thread (mode 0: before the first step's barrier)
+        const int ncol = J * K;
+        const int Ip = I + 1;
+        const int CC = min(SLAB / Ip, kSlabCols);
+        bf32x4 acc0[(8 + NW - 1) / NW];
+#pragma unroll
+        for (int u = 0; u < (8 + NW - 1) / NW; ++u) acc0[u] = bf32x4{0.f, 0.f, 0.f, 0.f};
+        const int q0 = tid / I, i0 = tid - q0 * I, dq = BS / I, di = BS - dq * I;
+        constexpr int EPT = (SLAB + BS - 1) / BS;
+        float preA[EPT], preB[EPT];
+        auto fetch = [&](float (&pre)[EPT], int cs0) {
+            const bool past = cs0 >= ncol;
+            const int ccs = min(CC, ncol - cs0);
+            const float *xs = x + (past ? 0L : (long)cs0 * I);
+            const int last = past ? 0 : ccs * I - 1;
+#pragma unroll
+            for (int u = 0; u < EPT; ++u) {
+                const int e = tid + BS * u;
+                pre[u] = xs[e < last ? e : last];
+            }
+        };
+        auto step = [&](float (&pre)[EPT], int c0) {
+            const int cc = min(CC, ncol - c0);
+            {
+                int i = i0, q = q0;
+#pragma unroll
+                for (int u = 0; u < EPT; ++u) {
+                    if (q < cc) slab[i + q * Ip] = pre[u];
+                    i += di;
+                    q += dq;
+                    if (i >= I) {
+                        i -= I;
+                        ++q;
+                    }
+                }
+            }
+            if constexpr (MODE == 0) {
+                for (int e = tid; e < cc * kBMaxR; e += BS) {
+                    const int q = e >> 4, r = e & 15, c = c0 + q;
+                    Ys[e] = (r < R) ? Fa[c % J + r * J] * Fb[c / J + r * K] : 0.0f;
+                }
+            }
+            __syncthreads();
+            fetch(pre, c0 + 2 * CC);
+            if constexpr (MODE == 0) {
+                const int nks = (cc + 3) >> 2;
+#pragma unroll
+                for (int u = 0; u < (8 + NW - 1) / NW; ++u) {
+                    const int tt = wv + NW * u;
+                    if (16 * tt < I) {
+                        const int i = 16 * tt + lr;
+                        for (int ks = 0; ks < nks; ++ks) {
+                            const int q = 4 * ks + lk;
+                            const float a = (i < I && q < cc) ? slab[i + q * Ip] : 0.0f;
+                            const float b = (q < cc) ? Ys[q * kBMaxR + lr] : 0.0f;
+                            acc0[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc0[u], 0, 0, 0);
+                        }
+                    }
+                }
+            } else {
+                const int nks = (I + 3) >> 2;
+                for (int tt = wv; 16 * tt < cc; tt += NW) {
+                    bf32x4 tacc = {0.f, 0.f, 0.f, 0.f};
+                    const int q = 16 * tt + lr;
+                    for (int ks = 0; ks < nks; ++ks) {
+                        const int i = 4 * ks + lk;
+                        const float a = (q < cc && i < I) ? slab[i + q * Ip] : 0.0f;
+                        tacc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Fa[i * kBMaxR + lr], tacc, 0, 0, 0);
+                    }
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int qq = 16 * tt + 4 * lk + v;
+                        if (qq < cc) Ys[qq * kBMaxR + lr] = tacc[v];
+                    }
+                }
+                __syncthreads();
+                if constexpr (MODE == 1) {
+                    for (int e = tid; e < J * R; e += BS) {
+                        const int j = e % J, r = e / J;
+                        float a2 = 0.0f;
+                        for (int q = (j - c0 % J + J) % J; q < cc; q += J)
+                            a2 = fmaf(Ys[q * kBMaxR + r], Fb[(c0 + q) / J + r * K], a2);
+                        Ms[e] += a2;
+                    }
+                } else {
+                    const int kf = c0 / J, kl = (c0 + cc - 1) / J;
+                    const int npair = (kl - kf + 1) * R;
+                    const int g = tid & 15;
+                    for (int p0 = 0; p0 < npair; p0 += BS / 16) {
+                        const int pr = p0 + (tid >> 4);
+                        float a2 = 0.0f;
+                        int k = 0, r = 0;
+                        if (pr < npair) {
+                            k = kf + pr / R;
+                            r = pr % R;
+                            const int qa = max(0, k * J - c0), qb = min(cc, (k + 1) * J - c0);
+                            for (int q = qa + g; q < qb; q += 16)
+                                a2 = fmaf(Ys[q * kBMaxR + r], Fb[(c0 + q) % J + r * J], a2);
+                        }
+#pragma unroll
+                        for (int o = 8; o > 0; o >>= 1) a2 += __shfl_xor(a2, o, 16);
+                        if (pr < npair && g == 0) Ms[k + r * K] += a2;
+                    }
+                }
+            }
+            __syncthreads();
+        };
+        fetch(preA, 0);
+        fetch(preB, CC);
+        for (int c0 = 0; c0 < ncol; c0 += 2 * CC) {
+            step(preA, c0);
+            step(preB, c0 + CC);  // past the last column: an empty step
+        }
+        if constexpr (MODE == 0) {
+#pragma unroll
+            for (int u = 0; u < (8 + NW - 1) / NW; ++u) {
+                const int tt = wv + NW * u;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int i = 16 * tt + 4 * lk + v;
+                    if (i < I && lr < R) Mbuf[uo0 + i + lr * I] = acc0[u][v];
+                }
+            }
+        } else {
+            const int In = MODE == 1 ? J : K;
+            const long uon = MODE == 1 ? uo1 : uo2;
+            for (int e = tid; e < In * R; e += BS) Mbuf[uon + e] = Ms[e];
+        }
+        __syncthreads();  // LDS free for the next tensor
+    }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_bm_solve(float *__restrict__ U, const float *__restrict__ Mbuf,
+                                                  double *__restrict__ G, const BDesc *__restrict__ desc, int ntensors,
+                                                  const double *__restrict__ normx2, double *__restrict__ fit_hist,
+                                                  int max_sweeps, const int *__restrict__ sweep_ctr,
+                                                  int *__restrict__ status) {
+    constexpr int BS = 256;
+    __shared__ float Ms[kBMaxDim * kBMaxR];
+    __shared__ float Un[kBMaxDim * kBMaxR];
+    __shared__ double V[kBMaxR * kBMaxR], L[kBMaxR * kBMaxR], W[kBMaxR * kBMaxR], Vi[kBMaxR * kBMaxR];
+    __shared__ double sh[BS], cs[4 * (kBMaxR / 2 + 1)];
+    __shared__ double red[BS / 64];
+    __shared__ int fail;
+    const int tid = threadIdx.x;
+    const int sweep = *sweep_ctr;
+    for (int t = blockIdx.x; t < ntensors; t += gridDim.x) {
+        const BDesc d = desc[t];
+        const int R = d.R;
+        const int In = sel3(MODE, d.I, d.J, d.K);
+        const long uon = d.uoff[MODE];
+        double *Gt = G + (long)t * 3 * (kBMaxR * kBMaxR);
+        for (int e = tid; e < In * R; e += BS) Ms[e] = Mbuf[uon + e];
+        for (int e = tid; e < R * R; e += BS) {
+            double v = 1.0;
+#pragma unroll
+            for (int m = 0; m < 3; ++m)
+                if (m != MODE) v *= Gt[m * (kBMaxR * kBMaxR) + e];
+            V[e] = v;
+        }
+        __syncthreads();
+        if (block_spd_inverse<(kBMaxR * kBMaxR + BS - 1) / BS>(V, L, W, Vi, R, cs, sh, &fail) && tid == 0)
+            atomicOr(status, 1);
+        double ip = 0.0;
+        for (int e = tid; e < In * R; e += BS) {
+            const int i = e % In, b = e / In;
+            double s = 0.0;
+            for (int a = 0; a < R; ++a) s += (double)Ms[i + a * In] * Vi[a + b * R];
+            const float u = (float)s;
+            Un[e] = u;
+            ip += (double)Ms[e] * (double)u;
+        }
+        __syncthreads();
+        for (int e = tid; e < In * R; e += BS) U[uon + e] = Un[e];
+        double ip_last = 0.0;
+        if (MODE == 2) ip_last = bsum<BS>(ip, red);
+        double hx = 0.0;  // (mode 2) sum of G_0 o G_1 o G_2 over this thread's entries
+        for (int e = tid; e < R * R; e += BS) {
+            const int a = e % R, b = e / R;
+            double s = 0.0;
+            for (int i = 0; i < In; ++i) s += (double)Un[i + a * In] * (double)Un[i + b * In];
+            Gt[MODE * (kBMaxR * kBMaxR) + e] = s;
+            if (MODE == 2) hx += Gt[e] * Gt[kBMaxR * kBMaxR + e] * s;
+        }
+        if (MODE == 2) {
+            hx = bsum<BS>(hx, red);
+            if (tid == 0 && sweep < max_sweeps) {
+                const double nx2 = normx2[t];
+                fit_hist[(long)t * max_sweeps + sweep] = sqrt(fabs(nx2 - 2.0 * ip_last + hx) / nx2);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 __global__ void k_batched_prologue(int *queue) { *queue = 0; }
+__global__ void k_bm_prologue(int *queues) {
+    queues[0] = 0;
+    queues[1] = 0;
+    queues[2] = 0;
+}
 __global__ void k_batched_epilogue(int *sweep_ctr) { *sweep_ctr += 1; }
 
 __global__ void k_batched_norm2(const float *__restrict__ X, const BDesc *__restrict__ desc, int n,
@@ -359,11 +633,20 @@ __global__ void k_batched_init(float *__restrict__ U, const BDesc *__restrict__ 
 // ---------------------------------------------------------------------------
 typedef void (*BatchedKernel)(const float *, float *, const BDesc *, const int *, int, int *, const double *,
                               double *, int, const int *, int *);
+typedef void (*BmMttkrp)(const float *, const float *, float *, const BDesc *, const int *, int, int *);
 
 struct Variant {
     int bs, slab;
-    BatchedKernel fn;
+    BatchedKernel fn;  // persistent design; nullptr: the split design (m[])
     const char *name;
+    BmMttkrp m[3];
+};
+
+// buffers of the split design: M (laid out like the factors) and the per-tensor Grams
+struct BmBuffers {
+    float *M = nullptr;
+    double *G = nullptr;
+    int *queues = nullptr;  // 3 work-queue counters
 };
 
 // LDS per workgroup: the slab plus ~54 KB of factors, Grams and per-mode scratch; the 4K-6K
@@ -376,27 +659,45 @@ static const Variant kVariants[] = {
     {256, 8192, k_cp_batched_sweep<256, 8192>, "bs256_slab8k"},
     {512, 8192, k_cp_batched_sweep<512, 8192>, "bs512_slab8k"},
     {512, 12288, k_cp_batched_sweep<512, 12288>, "bs512_slab12k"},
+    // split design: LDS = slab + 32 KB (mode 0: 24 KB), three workgroups per CU at 4K floats
+    // (a 2K-float slab, four per CU, measured 4.2 ms on C4 against 3.3)
+    {256, 4096, nullptr, "split_bs256_slab4k",
+     {k_bm_mttkrp<0, 256, 4096, 3>, k_bm_mttkrp<1, 256, 4096, 3>, k_bm_mttkrp<2, 256, 4096, 3>}},
+    {256, 6144, nullptr, "split_bs256_slab6k",
+     {k_bm_mttkrp<0, 256, 6144, 2>, k_bm_mttkrp<1, 256, 6144, 2>, k_bm_mttkrp<2, 256, 6144, 2>}},
 };
 static constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
-static int launch_variant(int v, int grid, hipStream_t s, const float *X, float *U, const BDesc *desc,
-                          const int *sched, int n, int *queue, const double *normx2, double *fit, int max_sweeps,
-                          int *sweep_ctr, int *status) {
-    k_batched_prologue<<<1, 1, 0, s>>>(queue);
+static int grid_for_kernel(const void *fn, int bs) {
+    int per_cu = 2;
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, bs, 0);
+    if (per_cu < 1) per_cu = 1;
+    return device_cus() * per_cu;
+}
+
+// one sweep of variant v (captured into the sweep graph)
+static int launch_variant(int v, hipStream_t s, const float *X, float *U, const BDesc *desc, const int *sched, int n,
+                          int *queue, const double *normx2, double *fit, int max_sweeps, int *sweep_ctr, int *status,
+                          const BmBuffers &bm) {
     const Variant &V = kVariants[v];
-    hipLaunchKernelGGL(V.fn, dim3(grid), dim3(V.bs), 0, s, X, U, desc, sched, n, queue, normx2, fit, max_sweeps,
-                       sweep_ctr, status);
+    if (V.fn) {
+        k_batched_prologue<<<1, 1, 0, s>>>(queue);
+        hipLaunchKernelGGL(V.fn, dim3(grid_for_kernel((const void *)V.fn, V.bs)), dim3(V.bs), 0, s, X, U, desc, sched,
+                           n, queue, normx2, fit, max_sweeps, sweep_ctr, status);
+    } else {
+        k_bm_prologue<<<1, 1, 0, s>>>(bm.queues);
+        const int sgrid = n < 8192 ? n : 8192;
+        for (int m = 0; m < 3; ++m) {
+            hipLaunchKernelGGL(V.m[m], dim3(grid_for_kernel((const void *)V.m[m], V.bs)), dim3(V.bs), 0, s, X,
+                               (const float *)U, bm.M, desc, sched, n, bm.queues + m);
+            if (m == 0) k_bm_solve<0><<<sgrid, 256, 0, s>>>(U, bm.M, bm.G, desc, n, normx2, fit, max_sweeps, sweep_ctr, status);
+            else if (m == 1) k_bm_solve<1><<<sgrid, 256, 0, s>>>(U, bm.M, bm.G, desc, n, normx2, fit, max_sweeps, sweep_ctr, status);
+            else k_bm_solve<2><<<sgrid, 256, 0, s>>>(U, bm.M, bm.G, desc, n, normx2, fit, max_sweeps, sweep_ctr, status);
+        }
+    }
     k_batched_epilogue<<<1, 1, 0, s>>>(sweep_ctr);
     TDX_LAUNCH_CHECK();
     return TDX_OK;
-}
-
-static int grid_for_variant(int v) {
-    const int cus = device_cus();
-    int per_cu = 2;
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)kVariants[v].fn, kVariants[v].bs, 0);
-    if (per_cu < 1) per_cu = 1;
-    return cus * per_cu;
 }
 
 }  // namespace tdx
@@ -414,7 +715,8 @@ const char *tdx_cp_batched_variant_name(int v) {
 // CP-ALS of `ntensors` 3rd-order tensors packed in X (tensor t at xoff[t], column-major
 // dims[3t..3t+2]) with ranks[t] <= 16 and extents <= 128; factors packed in U (mode m of
 // tensor t at uoff[3t+m], I_m x R column-major).  init = 0 draws U(0,1) from Philox stream
-// (seed + t, m); 1 uses the caller's U.  variant < 0 picks variant 3.  fit_hist (host, may
+// (seed + t, m); 1 uses the caller's U.  variant < 0 picks variant 7 (split_bs256_slab4k,
+// the autotuner's choice on the 4096-tensor benchmark batch).  fit_hist (host, may
 // be NULL) receives ntensors x sweeps relative errors; sweep_ms (host, may be NULL) the
 // device time of every sweep.  The per-sweep launches are captured once in a hipGraph.
 int tdx_cp_als_batched(const float *X, float *U, const long *dims, const long *ranks, const long *xoff,
@@ -422,7 +724,7 @@ int tdx_cp_als_batched(const float *X, float *U, const long *dims, const long *r
                        double *fit_hist, float *sweep_ms) {
     if (!X || !U || !dims || !ranks || !xoff || !uoff || ntensors <= 0 || sweeps < 0) return record(TDX_ERR_ARG);
     if (variant >= kNumVariants) return record(TDX_ERR_ARG);
-    if (variant < 0) variant = 3;
+    if (variant < 0) variant = 7;
     std::vector<BDesc> h(ntensors);
     std::vector<std::pair<long, int>> order(ntensors);
     for (long t = 0; t < ntensors; ++t) {
@@ -453,19 +755,19 @@ int tdx_cp_als_batched(const float *X, float *U, const long *dims, const long *r
     double *d_nx = nullptr, *d_fit = nullptr;
     struct Guard {
         hipStream_t st;
-        void *p[5];
+        void *p[7];
         ~Guard() {
             if (st) (void)hipStreamSynchronize(st);
             for (void *q : p)
                 if (q) (void)hipFree(q);
             if (st) (void)hipStreamDestroy(st);
         }
-    } guard{owned, {nullptr, nullptr, nullptr, nullptr, nullptr}};
+    } guard{owned, {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr}};
     TDX_HIP(hipMalloc(&d_desc, sizeof(BDesc) * ntensors));
     guard.p[0] = d_desc;
     TDX_HIP(hipMalloc(&d_sched, sizeof(int) * ntensors));
     guard.p[1] = d_sched;
-    TDX_HIP(hipMalloc(&d_int, sizeof(int) * 4));
+    TDX_HIP(hipMalloc(&d_int, sizeof(int) * 8));
     guard.p[2] = d_int;
     TDX_HIP(hipMalloc(&d_nx, sizeof(double) * ntensors));
     guard.p[3] = d_nx;
@@ -480,13 +782,25 @@ int tdx_cp_als_batched(const float *X, float *U, const long *dims, const long *r
     if (init == 0) k_batched_init<<<1024, 256, 0, s>>>(U, d_desc, (int)ntensors, seed);
     k_batched_norm2<<<1024, 256, 0, s>>>(X, d_desc, (int)ntensors, d_nx);
     TDX_LAUNCH_CHECK();
-    const int grid = grid_for_variant(variant);
+    BmBuffers bm;
+    if (!kVariants[variant].fn) {
+        long mf = 0;  // M is laid out like the factors
+        for (long t = 0; t < ntensors; ++t)
+            for (int m = 0; m < 3; ++m) mf = std::max(mf, h[t].uoff[m] + (long)dims[3 * t + m] * h[t].R);
+        TDX_HIP(hipMalloc(&bm.M, sizeof(float) * mf));
+        guard.p[5] = bm.M;
+        TDX_HIP(hipMalloc(&bm.G, sizeof(double) * ntensors * 3 * kBMaxR * kBMaxR));
+        guard.p[6] = bm.G;
+        bm.queues = d_int + 3;
+        k_bm_gram_init<<<1024, 256, 0, s>>>(U, d_desc, (int)ntensors, bm.G);
+        TDX_LAUNCH_CHECK();
+    }
     // capture one sweep
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
     TDX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    int st = launch_variant(variant, grid, s, X, U, d_desc, d_sched, (int)ntensors, queue, d_nx, d_fit, S,
-                            sweep_ctr, status);
+    int st = launch_variant(variant, s, X, U, d_desc, d_sched, (int)ntensors, queue, d_nx, d_fit, S, sweep_ctr,
+                            status, bm);
     hipError_t e = hipStreamEndCapture(s, &g);
     if (st) {
         if (g) (void)hipGraphDestroy(g);

@@ -1,6 +1,7 @@
-"""Batched CP-ALS (one persistent kernel per sweep) against the FP64 oracle and against
-the single-tensor path, on a small batch of uneven tensors of the batched recipe
-(extents in [16, 96], ranks in {4, 8, 16}, U(0,1) factors, 1% noise)."""
+"""Batched CP-ALS (one persistent kernel per sweep, or the split design: one MTTKRP and one
+solve kernel per mode) against the FP64 oracle and against the single-tensor path, on small
+batches of uneven tensors of the batched recipe (extents in [16, 96], ranks in {4, 8, 16},
+U(0,1) factors, 1% noise)."""
 import numpy as np
 import pytest
 
@@ -64,6 +65,25 @@ def test_batched_equals_single_tensor_path_and_variants_agree():
         xt = X[b.xoff[t]:b.xoff[t] + int(np.prod(dims))]
         _, _, f1, _ = tdx.cp_als_ex(xt, dims, R, Ud, o)
         assert np.max(np.abs(f1 - fit[t])) < 1e-5
+
+
+def test_split_design_equals_persistent_bit_for_bit():
+    """The split design runs the persistent kernel's arithmetic in the same summation orders,
+    so at the same block size and slab the factors and fits agree bit for bit.  3000 tensors:
+    more than the workgroups resident at once, so every workgroup takes several tensors from
+    the queue (the case where a workgroup's factor blocks must be rewritten safely)."""
+    import torch
+    L = tdx.lib()
+    names = [L.tdx_cp_batched_variant_name(v).decode() for v in range(L.tdx_cp_batched_variants())]
+    b, X = make_batch(3000, seed=4, lo=16, hi=28)
+    for slab in ("4k", "6k"):
+        out = []
+        for name in (f"bs256_slab{slab}", f"split_bs256_slab{slab}"):
+            U = torch.empty(b.u_floats, device="cuda")
+            _, fit, _ = tdx.cp_als_batched(X, U, b, 2, init=0, seed=9, variant=names.index(name))
+            out.append((fit, host(U)))
+        assert np.array_equal(out[0][0], out[1][0]), slab
+        assert np.array_equal(out[0][1], out[1][1]), slab
 
 
 def test_autotuner_picks_and_caches(tmp_path):
