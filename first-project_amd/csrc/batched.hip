@@ -19,6 +19,7 @@
 #include <map>
 #include <stdio.h>
 #include <string>
+#include <string.h>
 #include <vector>
 
 namespace tdx {
@@ -31,6 +32,7 @@ struct BDesc {
     long xoff;
     long uoff[3];
     int I, J, K, R;
+    long toff;  // split design with T: X x_0 A^T of the tensor (J K x R floats) at toff
 };
 
 typedef float bf32x4 __attribute__((ext_vector_type(4)));
@@ -357,9 +359,12 @@ __global__ __launch_bounds__(256) void k_bm_gram_init(const float *__restrict__ 
     }
 }
 
+// Tbuf != nullptr (mode 1): T = X x_0 A^T, which modes 1 and 2 share, is also written to HBM,
+// one slab at a time, for k_bm_mode2_t
 template <int MODE, int BS, int SLAB, int WGS>
 __global__ __launch_bounds__(BS, WGS) void k_bm_mttkrp(const float *__restrict__ X, const float *__restrict__ U,
-                                                       float *__restrict__ Mbuf, const BDesc *__restrict__ desc,
+                                                       float *__restrict__ Mbuf, float *__restrict__ Tbuf,
+                                                       const BDesc *__restrict__ desc,
                                                        const int *__restrict__ sched, int ntensors,
                                                        int *__restrict__ queue) {
     __shared__ __attribute__((aligned(16))) float slab[SLAB];
@@ -474,6 +479,10 @@ __global__ __launch_bounds__(BS, WGS) void k_bm_mttkrp(const float *__restrict__
                 }
                 __syncthreads();
                 if constexpr (MODE == 1) {
+                    if (Tbuf) {
+                        float *tp = Tbuf + dp->toff + (long)c0 * R;
+                        for (int e = tid; e < cc * R; e += BS) tp[e] = Ys[(e / R) * kBMaxR + e % R];
+                    }
                     for (int e = tid; e < J * R; e += BS) {
                         const int j = e % J, r = e / J;
                         float a2 = 0.0f;
@@ -525,6 +534,81 @@ __global__ __launch_bounds__(BS, WGS) void k_bm_mttkrp(const float *__restrict__
             const long uon = MODE == 1 ? uo1 : uo2;
             for (int e = tid; e < In * R; e += BS) Mbuf[uon + e] = Ms[e];
         }
+        __syncthreads();  // LDS free for the next tensor
+    }
+}
+
+// Mode 2 from T (written by the mode-1 kernel): the slabs of T in the partition the mode-1
+// kernel used (CC columns), each loaded into the per-column scratch and reduced with the same
+// segmented sums, so M_2 equals the one streamed from X bit for bit.  The next slab's T is
+// loaded into registers while the current one is reduced.
+template <int BS, int SLAB, int WGS>
+__global__ __launch_bounds__(BS, WGS) void k_bm_mode2_t(const float *__restrict__ X, const float *__restrict__ U,
+                                                        float *__restrict__ Mbuf, float *__restrict__ Tbuf,
+                                                        const BDesc *__restrict__ desc,
+                                                        const int *__restrict__ sched, int ntensors,
+                                                        int *__restrict__ queue) {
+    (void)X;
+    __shared__ float Fb[kBMaxDim * kBMaxR];  // B
+    __shared__ float Ms[kBMaxDim * kBMaxR];
+    __shared__ float Ys[kSlabCols * kBMaxR];
+    __shared__ int slot;
+    const int tid = threadIdx.x;
+    constexpr int EPT = kSlabCols * kBMaxR / BS;
+    for (;;) {
+        if (tid == 0) slot = atomicAdd(queue, 1);
+        __syncthreads();
+        const int sl = __builtin_amdgcn_readfirstlane(slot);
+        if (sl >= ntensors) break;  // uniform
+        const int t = __builtin_amdgcn_readfirstlane(sched[sl]);
+        const BDesc *dp = desc + t;
+        const int I = dp->I, J = dp->J, K = dp->K, R = dp->R;
+        const long uo1 = dp->uoff[1], uo2 = dp->uoff[2];
+        const float *tb = Tbuf + dp->toff;
+        for (int e = tid; e < J * R; e += BS) Fb[e] = U[uo1 + e];
+        for (int e = tid; e < K * R; e += BS) Ms[e] = 0.0f;
+        const int ncol = J * K;
+        const int CC = min(SLAB / (I + 1), kSlabCols);
+        // entry e = tid + BS u of a slab is (q, r) = (e / 16, e % 16) of the scratch; its T value
+        // (0 past the slab or for r >= R) is loaded one slab ahead
+        float pre[EPT];
+        auto fetch = [&](int c0) {
+            const int cc = min(CC, ncol - c0);
+#pragma unroll
+            for (int u = 0; u < EPT; ++u) {
+                const int e = tid + BS * u, q = e >> 4, r = e & 15;
+                pre[u] = (c0 < ncol && q < cc && r < R) ? tb[(long)(c0 + q) * R + r] : 0.0f;
+            }
+        };
+        fetch(0);
+        __syncthreads();  // B and the zeroed M
+        for (int c0 = 0; c0 < ncol; c0 += CC) {
+            const int cc = min(CC, ncol - c0);
+#pragma unroll
+            for (int u = 0; u < EPT; ++u) Ys[tid + BS * u] = pre[u];
+            __syncthreads();
+            fetch(c0 + CC);
+            const int kf = c0 / J, kl = (c0 + cc - 1) / J;
+            const int npair = (kl - kf + 1) * R;
+            const int g = tid & 15;
+            for (int p0 = 0; p0 < npair; p0 += BS / 16) {
+                const int pr = p0 + (tid >> 4);
+                float a2 = 0.0f;
+                int k = 0, r = 0;
+                if (pr < npair) {
+                    k = kf + pr / R;
+                    r = pr % R;
+                    const int qa = max(0, k * J - c0), qb = min(cc, (k + 1) * J - c0);
+                    for (int q = qa + g; q < qb; q += 16)
+                        a2 = fmaf(Ys[q * kBMaxR + r], Fb[(c0 + q) % J + r * J], a2);
+                }
+#pragma unroll
+                for (int o = 8; o > 0; o >>= 1) a2 += __shfl_xor(a2, o, 16);
+                if (pr < npair && g == 0) Ms[k + r * K] += a2;
+            }
+            __syncthreads();
+        }
+        for (int e = tid; e < K * R; e += BS) Mbuf[uo2 + e] = Ms[e];
         __syncthreads();  // LDS free for the next tensor
     }
 }
@@ -633,7 +717,7 @@ __global__ void k_batched_init(float *__restrict__ U, const BDesc *__restrict__ 
 // ---------------------------------------------------------------------------
 typedef void (*BatchedKernel)(const float *, float *, const BDesc *, const int *, int, int *, const double *,
                               double *, int, const int *, int *);
-typedef void (*BmMttkrp)(const float *, const float *, float *, const BDesc *, const int *, int, int *);
+typedef void (*BmMttkrp)(const float *, const float *, float *, float *, const BDesc *, const int *, int, int *);
 
 struct Variant {
     int bs, slab;
@@ -646,6 +730,7 @@ struct Variant {
 struct BmBuffers {
     float *M = nullptr;
     double *G = nullptr;
+    float *T = nullptr;     // variants whose mode 2 reads T
     int *queues = nullptr;  // 3 work-queue counters
 };
 
@@ -665,7 +750,13 @@ static const Variant kVariants[] = {
      {k_bm_mttkrp<0, 256, 4096, 3>, k_bm_mttkrp<1, 256, 4096, 3>, k_bm_mttkrp<2, 256, 4096, 3>}},
     {256, 6144, nullptr, "split_bs256_slab6k",
      {k_bm_mttkrp<0, 256, 6144, 2>, k_bm_mttkrp<1, 256, 6144, 2>, k_bm_mttkrp<2, 256, 6144, 2>}},
+    // mode 2 from T = X x_0 A^T written by mode 1 (J K R floats per tensor, read instead of X)
+    {256, 4096, nullptr, "split_t_bs256_slab4k",
+     {k_bm_mttkrp<0, 256, 4096, 3>, k_bm_mttkrp<1, 256, 4096, 3>, k_bm_mode2_t<256, 4096, 3>}},
+    {256, 6144, nullptr, "split_t_bs256_slab6k",
+     {k_bm_mttkrp<0, 256, 6144, 2>, k_bm_mttkrp<1, 256, 6144, 2>, k_bm_mode2_t<256, 6144, 2>}},
 };
+static bool variant_uses_t(int v) { return v >= 0 && kVariants[v].m[2] != nullptr && strncmp(kVariants[v].name, "split_t_", 8) == 0; }
 static constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
 static int grid_for_kernel(const void *fn, int bs) {
@@ -689,7 +780,7 @@ static int launch_variant(int v, hipStream_t s, const float *X, float *U, const 
         const int sgrid = n < 8192 ? n : 8192;
         for (int m = 0; m < 3; ++m) {
             hipLaunchKernelGGL(V.m[m], dim3(grid_for_kernel((const void *)V.m[m], V.bs)), dim3(V.bs), 0, s, X,
-                               (const float *)U, bm.M, desc, sched, n, bm.queues + m);
+                               (const float *)U, bm.M, m >= 1 ? bm.T : nullptr, desc, sched, n, bm.queues + m);
             if (m == 0) k_bm_solve<0><<<sgrid, 256, 0, s>>>(U, bm.M, bm.G, desc, n, normx2, fit, max_sweeps, sweep_ctr, status);
             else if (m == 1) k_bm_solve<1><<<sgrid, 256, 0, s>>>(U, bm.M, bm.G, desc, n, normx2, fit, max_sweeps, sweep_ctr, status);
             else k_bm_solve<2><<<sgrid, 256, 0, s>>>(U, bm.M, bm.G, desc, n, normx2, fit, max_sweeps, sweep_ctr, status);
@@ -715,7 +806,7 @@ const char *tdx_cp_batched_variant_name(int v) {
 // CP-ALS of `ntensors` 3rd-order tensors packed in X (tensor t at xoff[t], column-major
 // dims[3t..3t+2]) with ranks[t] <= 16 and extents <= 128; factors packed in U (mode m of
 // tensor t at uoff[3t+m], I_m x R column-major).  init = 0 draws U(0,1) from Philox stream
-// (seed + t, m); 1 uses the caller's U.  variant < 0 picks variant 7 (split_bs256_slab4k,
+// (seed + t, m); 1 uses the caller's U.  variant < 0 picks variant 9 (split_t_bs256_slab4k,
 // the autotuner's choice on the 4096-tensor benchmark batch).  fit_hist (host, may
 // be NULL) receives ntensors x sweeps relative errors; sweep_ms (host, may be NULL) the
 // device time of every sweep.  The per-sweep launches are captured once in a hipGraph.
@@ -724,7 +815,7 @@ int tdx_cp_als_batched(const float *X, float *U, const long *dims, const long *r
                        double *fit_hist, float *sweep_ms) {
     if (!X || !U || !dims || !ranks || !xoff || !uoff || ntensors <= 0 || sweeps < 0) return record(TDX_ERR_ARG);
     if (variant >= kNumVariants) return record(TDX_ERR_ARG);
-    if (variant < 0) variant = 7;
+    if (variant < 0) variant = 9;
     std::vector<BDesc> h(ntensors);
     std::vector<std::pair<long, int>> order(ntensors);
     for (long t = 0; t < ntensors; ++t) {
@@ -737,6 +828,7 @@ int tdx_cp_als_batched(const float *X, float *U, const long *dims, const long *r
             d.R > kBMaxR)
             return record(TDX_ERR_UNSUPPORTED);
         d.xoff = xoff[t];
+        d.toff = 0;
         for (int m = 0; m < 3; ++m) d.uoff[m] = uoff[3 * t + m];
         order[t] = {-(long)d.I * d.J * d.K * d.R, (int)t};  // largest work first
     }
@@ -755,14 +847,14 @@ int tdx_cp_als_batched(const float *X, float *U, const long *dims, const long *r
     double *d_nx = nullptr, *d_fit = nullptr;
     struct Guard {
         hipStream_t st;
-        void *p[7];
+        void *p[8];
         ~Guard() {
             if (st) (void)hipStreamSynchronize(st);
             for (void *q : p)
                 if (q) (void)hipFree(q);
             if (st) (void)hipStreamDestroy(st);
         }
-    } guard{owned, {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr}};
+    } guard{owned, {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr}};
     TDX_HIP(hipMalloc(&d_desc, sizeof(BDesc) * ntensors));
     guard.p[0] = d_desc;
     TDX_HIP(hipMalloc(&d_sched, sizeof(int) * ntensors));
@@ -791,6 +883,16 @@ int tdx_cp_als_batched(const float *X, float *U, const long *dims, const long *r
         guard.p[5] = bm.M;
         TDX_HIP(hipMalloc(&bm.G, sizeof(double) * ntensors * 3 * kBMaxR * kBMaxR));
         guard.p[6] = bm.G;
+        if (variant_uses_t(variant)) {
+            long tf = 0;
+            for (long t = 0; t < ntensors; ++t) {
+                h[t].toff = tf;
+                tf += (long)h[t].J * h[t].K * h[t].R;
+            }
+            TDX_HIP(hipMalloc(&bm.T, sizeof(float) * tf));
+            guard.p[7] = bm.T;
+            TDX_HIP(hipMemcpyAsync(d_desc, h.data(), sizeof(BDesc) * ntensors, hipMemcpyHostToDevice, s));
+        }
         bm.queues = d_int + 3;
         k_bm_gram_init<<<1024, 256, 0, s>>>(U, d_desc, (int)ntensors, bm.G);
         TDX_LAUNCH_CHECK();

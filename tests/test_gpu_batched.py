@@ -68,8 +68,9 @@ def test_batched_equals_single_tensor_path_and_variants_agree():
 
 
 def test_split_design_equals_persistent_bit_for_bit():
-    """The split design runs the persistent kernel's arithmetic in the same summation orders,
-    so at the same block size and slab the factors and fits agree bit for bit.  3000 tensors:
+    """The split design (also with mode 2 from the stored T = X x_0 A^T) runs the persistent
+    kernel's arithmetic in the same summation orders, so at the same block size and slab the
+    factors and fits agree bit for bit.  3000 tensors:
     more than the workgroups resident at once, so every workgroup takes several tensors from
     the queue (the case where a workgroup's factor blocks must be rewritten safely)."""
     import torch
@@ -78,12 +79,13 @@ def test_split_design_equals_persistent_bit_for_bit():
     b, X = make_batch(3000, seed=4, lo=16, hi=28)
     for slab in ("4k", "6k"):
         out = []
-        for name in (f"bs256_slab{slab}", f"split_bs256_slab{slab}"):
+        for name in (f"bs256_slab{slab}", f"split_bs256_slab{slab}", f"split_t_bs256_slab{slab}"):
             U = torch.empty(b.u_floats, device="cuda")
             _, fit, _ = tdx.cp_als_batched(X, U, b, 2, init=0, seed=9, variant=names.index(name))
             out.append((fit, host(U)))
-        assert np.array_equal(out[0][0], out[1][0]), slab
-        assert np.array_equal(out[0][1], out[1][1]), slab
+        for o in out[1:]:
+            assert np.array_equal(out[0][0], o[0]), slab
+            assert np.array_equal(out[0][1], o[1]), slab
 
 
 def test_autotuner_picks_and_caches(tmp_path):
