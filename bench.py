@@ -211,11 +211,19 @@ def extras(args, log):
             torch.cuda.synchronize()
             tot += time.perf_counter() - t0
         loop["graph" if use_graph else "eager"] = round(tot * b.n / loop_n, 3)
+    # bytes per sweep: three MTTKRPs stream X three times; the variants whose mode 2 reads
+    # T = X x_0 A^T (J K R floats per tensor) stream X twice and write and read T once
     bytes_sweep = 3 * 4 * b.x_floats
+    vname = list(times)[best]
+    t_floats = int(np.sum(b.shapes[:, 1] * b.shapes[:, 2] * b.ranks))
+    moved = 2 * 4 * b.x_floats + 2 * 4 * t_floats if vname.startswith("split_t_") else bytes_sweep
+    sec = float(np.mean(ms)) * 1e-3
     out["c4"] = {"tensors": b.n, "elements": b.x_floats, "sweeps": 30, "batched_s": round(float(np.sum(ms)) / 1e3, 5),
-                 "batched_ms_per_sweep": round(float(np.mean(ms)), 4), "variant": best, "autotune_ms": times,
-                 "batched_GBps_per_sweep": round(bytes_sweep / (float(np.mean(ms)) * 1e-3) / 1e9, 1),
-                 "hbm_frac": round(bytes_sweep / (float(np.mean(ms)) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                 "batched_ms_per_sweep": round(float(np.mean(ms)), 4), "variant": best, "variant_name": vname,
+                 "autotune_ms": times,
+                 "batched_GBps_per_sweep": round(bytes_sweep / sec / 1e9, 1),
+                 "bytes_moved_per_sweep": moved, "hbm_GBps_per_sweep": round(moved / sec / 1e9, 1),
+                 "hbm_frac": round(moved / sec / 1e9 / HBM_PEAK_GBS, 4),
                  "per_tensor_loop_s_extrapolated": loop, "per_tensor_loop_sample": loop_n,
                  "median_relerr_final": float(np.median(fit[:, -1]))}
     del X, U

@@ -945,7 +945,7 @@ int tdx_cp_batched_autotune(const float *X, const float *U, const long *dims, co
                             int *best, float *ms_per_sweep) {
     if (!X || !U || !best || ntensors <= 0 || u_floats <= 0 || trial_sweeps <= 0) return record(TDX_ERR_ARG);
     // cache key: device name + tensor count + total elements + the full rank histogram (1..16)
-    // + an FNV-1a digest of the (extents, rank) list in order
+    // + an FNV-1a digest of the (extents, rank) list in order and of the variant names
     hipDeviceProp_t prop;
     int dev = 0;
     TDX_HIP(hipGetDevice(&dev));
@@ -966,6 +966,10 @@ int tdx_cp_batched_autotune(const float *X, const float *U, const long *dims, co
         mix(dims[3 * t + 2]);
         mix(ranks[t]);
     }
+    // (the variant set is part of the key: a cache written by a build with other variants
+    // never maps to a wrong index)
+    for (int v = 0; v < kNumVariants; ++v)
+        for (const char *c = kVariants[v].name; *c; ++c) mix((long)*c);
     char key[512];
     int kl = snprintf(key, sizeof(key), "%s|n%ld|e%ld|r", prop.gcnArchName, ntensors, tot);
     for (int r = 1; r <= 16 && kl > 0 && kl < (int)sizeof(key) - 24; ++r)
