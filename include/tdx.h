@@ -281,8 +281,9 @@ int tdx_cp_tensorcore(const float *X, float *A, float *B, float *C, long a, long
  *      call, src/cp_als.cu:3) ---- */
 /* CP-ALS of `ntensors` 3rd-order tensors packed in X (tensor t at element offset xoff[t],
  * column-major, extents dims[3t..3t+2] <= 128) with ranks[t] <= 16; factors packed in U (mode
- * m of tensor t at uoff[3t+m], I_m x R column-major).  One persistent kernel per sweep pulls
- * tensors from a size-sorted queue; the sweep is captured in a hipGraph.  init = 0 draws
+ * m of tensor t at uoff[3t+m], I_m x R column-major).  Workgroups pull tensors from a
+ * size-sorted queue: one persistent kernel per sweep, or (the split variants, the default)
+ * one MTTKRP and one solve kernel per mode; the sweep is captured in a hipGraph.  init = 0 draws
  * U(0,1) from Philox stream (seed + t, m), 1 uses the caller's U.  variant: kernel variant
  * (see tdx_cp_batched_autotune), < 0 for the default.  fit_hist (host, ntensors x sweeps,
  * may be NULL) receives the fast-fit relative errors; sweep_ms (host, may be NULL) the device
@@ -294,7 +295,8 @@ int tdx_cp_batched_variants(void);
 const char *tdx_cp_batched_variant_name(int variant);
 /* times trial_sweeps sweeps of every variant on a copy of U (U is not modified), returns the
  * fastest in *best and all timings in ms_per_sweep (host, may be NULL); the choice is cached
- * in cache_path ("key variant ms" lines, key = device + batch shape; NULL: no cache) */
+ * in cache_path ("key variant ms" lines, key = device + batch shape + variant set; NULL: no
+ * cache) */
 int tdx_cp_batched_autotune(const float *X, const float *U, const long *dims, const long *ranks, const long *xoff,
                             const long *uoff, long ntensors, long u_floats, int trial_sweeps, const char *cache_path,
                             int *best, float *ms_per_sweep);
