@@ -29,8 +29,10 @@ whose relative error is <= anchor + 1e-4 (and its final direct error is too).
    roofline: the longest kernel of the BF16 sweep, the shared pass over X.
  * cpu_baseline: the oracle's FP64 ALS in GEMM form (the reference CPU path,
    decomposition/CPU/python/mycp.py, on the host's BLAS threads) on the first seed for 27
-   sweeps: seconds to the same target; plus the FP32 GEMM ALS and the 128^3 rank-16
-   configuration in full on the loop oracle beside the GPU.
+   sweeps: seconds to the same target; plus the FP32 GEMM ALS, the 128^3 rank-16
+   configuration in full on the loop oracle beside the GPU, and the secondary configurations
+   in the oracle's GEMM forms: 128^4 Tucker HOSVD + 1 HOOI, 64 of the 4096 batched tensors
+   (30 sweeps each, extrapolated), and 10 plain sweeps on 2 held-out 96^4 seeds.
  * other_configs: C1, C3, C4, C5 (learned extrapolation on held-out seeds).
 
 Prints ONE JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
@@ -167,10 +169,12 @@ def extras(args, log):
     log("C3: Tucker 128^4, core 32^4, HOSVD + 10 HOOI (BF16 TTM chain, and FP32 for comparison)")
     data = W.c3()
     W.run_c3(1, True, data=data)  # warm-up: code objects, workspace
+    sec_1, rel_1, _ = W.run_c3(1, True, data=data)
     sec_b, rel_b, _ = W.run_c3(10, True, data=data)
     sec_f, rel_f, _ = W.run_c3(10, False, data=data)
     _, _, hist = W.run_c3(10, True, history=True, data=data)
-    out["c3"] = {"seconds_hosvd_plus_10_hooi_bf16": round(sec_b, 4), "relerr_final_bf16": rel_b,
+    out["c3"] = {"seconds_hosvd_plus_1_hooi_bf16": round(sec_1, 4), "relerr_after_1_hooi_bf16": rel_1,
+                 "seconds_hosvd_plus_10_hooi_bf16": round(sec_b, 4), "relerr_final_bf16": rel_b,
                  "seconds_hosvd_plus_10_hooi_fp32": round(sec_f, 4), "relerr_final_fp32": rel_f,
                  "relerr_per_iter_bf16": [float(h) for h in hist], "noise": 5e-3}
     del data
@@ -232,6 +236,66 @@ def extras(args, log):
         log(f"C5: learned extrapolation, {args.c5_seeds} held-out seeds")
         out["c5"] = c5_leg(args.c5_seeds, log)
     return out
+
+
+def cpu_secondary(log):
+    """The reference CPU paths on the secondary configurations, on the host's BLAS threads, in
+    the oracle's GEMM forms (pinned against its loop forms in tests/test_oracle.py): C3 HOSVD +
+    1 HOOI (decomposition/CPU/python/tucker.py's method) on the 128^4 tensor; C4 the plain FP64
+    ALS of 64 of the 4096 batched tensors (the GPU per-tensor loop's sample), 30 sweeps each
+    from the batched init, extrapolated to the batch; C5 10 FP64 plain sweeps on 2 held-out
+    seeds (seconds per sweep; plain ALS needs up to 400 there)."""
+    import numpy as np
+    import torch
+
+    import extrapolation as E
+    import oracle as O
+    import workloads as W
+    res = {}
+    log("CPU baseline: C3 HOSVD + 1 HOOI")
+    X, dims, ranks = W.c3()
+    Xh = X.cpu().numpy()
+    del X
+    torch.cuda.empty_cache()
+    t0 = time.perf_counter()
+    U, _, _ = O.tucker_hosvd_gemm(Xh, dims, ranks)
+    t_h = time.perf_counter() - t0
+    _, _, hist, secs = O.tucker_hooi_gemm(Xh, dims, ranks, 1, U)
+    res["c3"] = {"cpu_s_hosvd": round(t_h, 3), "cpu_s_hosvd_plus_1_hooi": round(t_h + float(secs[0]), 3),
+                 "relerr_after_1_hooi": float(hist[0]), "workload": "128^4, core 32^4, FP64"}
+    del Xh
+    log("CPU baseline: C4 sample, 64 tensors x 30 FP64 sweeps")
+    Xb, b = W.c4()
+    idx = np.linspace(0, b.n - 1, 64).astype(int)
+    tot = 0.0
+    for t in idx:
+        dims4 = tuple(int(v) for v in b.shapes[t])
+        R4 = int(b.ranks[t])
+        xt = Xb[int(b.xoff[t]):int(b.xoff[t]) + int(np.prod(dims4))].cpu().numpy()
+        U0 = [O.uniform(d * R4, 1 + int(t), m).reshape((d, R4), order="F").astype(np.float64)
+              for m, d in enumerate(dims4)]  # the batched init's Philox streams (seed 1 + t, m)
+        t0 = time.perf_counter()
+        O.cp_als3_gemm(xt, dims4, U0, 30)
+        tot += time.perf_counter() - t0
+    res["c4"] = {"cpu_s_sample": round(tot, 3), "sample": 64, "cpu_s_extrapolated": round(tot * b.n / 64, 2),
+                 "sweeps": 30}
+    del Xb
+    torch.cuda.empty_cache()
+    log("CPU baseline: C5, 2 held-out seeds x 10 FP64 plain sweeps")
+    dims5, R5 = (96, 96, 96, 96), 24
+    per = []
+    for sd in (2000, 2001):
+        X5 = E.swamp_tensor(sd, dims5, R5, eta=1e-3)
+        Xh5 = X5.cpu().numpy()
+        U0 = [u.cpu().numpy().reshape((96, R5), order="F").astype(np.float64) for u in E.init_factors(sd, dims5, R5)]
+        del X5
+        _, fit5, sec5 = O.cp_als_gemm(Xh5, dims5, U0, 10)
+        per.append({"seed": sd, "s_per_sweep": round(float(np.mean(sec5)), 4), "relerr_after_10": float(fit5[-1])})
+        log(f"  C5 seed {sd}: {per[-1]['s_per_sweep']} s per FP64 sweep")
+    res["c5"] = {"per_seed": per, "s_per_sweep": round(float(np.mean([p["s_per_sweep"] for p in per])), 4),
+                 "workload": "96^4 rank 24, congruence 0.9, 0.1% noise, FP64 plain ALS"}
+    torch.cuda.empty_cache()
+    return res
 
 
 def cpu_leg(X, dims, R, init_host, anchor, seed, log):
@@ -442,6 +506,7 @@ def main():
                      "max_abs_relerr_diff_over_sweeps": float(np.max(np.abs(np.asarray(fit_g) - fit_c))),
                      "workload": "128^3 rank 16, U(0,1) factors, 1% noise, U(0,1) init (seed 1), 50 sweeps"}
         del X1
+        cpu.update(cpu_secondary(log))
 
     del X
     torch.cuda.empty_cache()

@@ -419,3 +419,102 @@ def tucker_hosvd_gemm(X, dims, ranks):
         Y = _ttm_t_gemm(Y, cur, m, U[m])
         cur[m] = ranks[m]
     return U, Y, W
+
+def tucker_hooi_gemm(X, dims, ranks, iters, U0):
+    """FP64 HOOI in GEMM form (the iteration of orc_tucker_hooi,
+    decomposition/GPU/normal/tucker/tucker_hooi.cu:103-148: for each mode in order,
+    Y = X x_{m != n} U_m^T with the current factors, U_n = the top eigenvectors of
+    Y(n) Y(n)^T; the relative error sqrt(1 - ||core||^2 / ||X||^2) after every iteration),
+    with the TTM chains and Grams as GEMMs and the eigensolve by LAPACK; pinned against
+    orc_tucker_hooi in tests/test_oracle.py.  Returns (U list, core flat F-order, relerr
+    per iteration, seconds per iteration)."""
+    import time
+    Xd = np.asarray(X, np.float64).reshape(-1)
+    nx2 = float(np.dot(Xd, Xd))
+    U = [np.array(u, np.float64) for u in U0]
+    hist, secs = [], []
+
+    def chain(skip):
+        cur = list(dims)
+        Y = Xd
+        for m in range(len(dims)):
+            if m == skip:
+                continue
+            Y = _ttm_t_gemm(Y, cur, m, U[m])
+            cur[m] = ranks[m]
+        return Y, cur
+
+    for _ in range(iters):
+        t0 = time.perf_counter()
+        for n in range(len(dims)):
+            Y, cur = chain(n)
+            U[n], _ = _top_eigvecs_lapack(_unfold_gram_gemm(Y, cur, n), ranks[n])
+        core, _ = chain(-1)
+        hist.append(float(np.sqrt(max(nx2 - float(np.dot(core, core)), 0.0) / nx2)))
+        secs.append(time.perf_counter() - t0)
+    core, _ = chain(-1)
+    return U, core, np.asarray(hist), np.asarray(secs)
+
+
+def cp_als_gemm(X, dims, U0, sweeps, stop_at=None):
+    """FP64 CP-ALS of any order in GEMM form (the reference CPU path,
+    decomposition/CPU/python/mycp.py:11-22: for each mode in order, V = Hadamard of the other
+    Grams, M = X(n) (Khatri-Rao of the others), a Cholesky solve; the fast-fit relative error
+    of Tensorlab cpd_als.m:281-283 after every sweep), for order 4 where cp_als3_gemm does not
+    apply; pinned against orc_cp_als in tests/test_oracle.py.  M = X(n) KR is formed as a
+    tensordot of X with the Khatri-Rao rows of the longer side of mode n (the modes after it,
+    or before it), then a contraction with the other side's.
+    Returns (U list, relerr per sweep, seconds per sweep)."""
+    import time
+    d = len(dims)
+    R = U0[0].shape[1]
+    U = [np.array(u, np.float64) for u in U0]
+    Xd = np.asarray(X, np.float64).reshape(-1)
+    nx2 = float(np.dot(Xd, Xd))
+    G = [u.T @ u for u in U]
+
+    def mttkrp(n):
+        P = int(np.prod(dims[:n]))
+        Q = int(np.prod(dims[n + 1:]))
+        Y = Xd.reshape((P, dims[n], Q), order="F")
+        # Khatri-Rao rows of the later modes (index q = i_{n+1} + i_{n+2} I_{n+1} + ...)
+        Kq = np.ones((1, R))
+        for m in range(n + 1, d):
+            Kq = (U[m][:, None, :] * Kq[None, :, :]).reshape(-1, R)
+        Kp = np.ones((1, R))
+        for m in range(n):
+            Kp = (U[m][:, None, :] * Kp[None, :, :]).reshape(-1, R)
+        # contract the longer side first, so the intermediate is In x min(P, Q) x R
+        if Q >= P:
+            Z = np.tensordot(Y, Kq, axes=([2], [0]))    # (P, In, R)
+            return np.einsum("pir,pr->ir", Z, Kp, optimize=True)
+        Z = np.tensordot(Y, Kp, axes=([0], [0]))        # (In, Q, R)
+        return np.einsum("iqr,qr->ir", Z, Kq, optimize=True)
+
+    def solve(V, M):
+        try:
+            L = np.linalg.cholesky(V)
+            return np.linalg.solve(L.T, np.linalg.solve(L, M.T)).T
+        except np.linalg.LinAlgError:
+            return (np.linalg.pinv(V, rcond=1e-12) @ M.T).T
+
+    fit, secs = [], []
+    for _ in range(sweeps):
+        t0 = time.perf_counter()
+        for n in range(d):
+            V = np.ones((R, R))
+            for m in range(d):
+                if m != n:
+                    V = V * G[m]
+            M = mttkrp(n)
+            U[n] = solve(V, M)
+            G[n] = U[n].T @ U[n]
+        H = np.ones((R, R))
+        for m in range(d):
+            H = H * G[m]
+        r2 = nx2 - 2.0 * float(np.sum(M * U[d - 1])) + float(np.sum(H))
+        fit.append(float(np.sqrt(abs(r2) / nx2)))
+        secs.append(time.perf_counter() - t0)
+        if stop_at is not None and fit[-1] <= stop_at:
+            break
+    return U, np.asarray(fit), np.asarray(secs)

@@ -433,3 +433,42 @@ def test_tucker_hosvd_gemm_matches_loop_oracle(dims, ranks):
         assert np.max(np.abs(a - b)) < 1e-12
     assert np.max(np.abs(co - cg)) < 1e-12 * np.max(np.abs(co))
     assert all(np.all(np.diff(x) >= 0) for x in w)
+
+
+@pytest.mark.parametrize("dims,ranks", [((20, 16, 12), (4, 3, 5)), ((12, 10, 8, 6), (3, 4, 2, 3))])
+def test_tucker_hooi_gemm_matches_loop_oracle(dims, ranks):
+    """The GEMM/LAPACK HOOI of the CPU baseline gives the loop oracle's factors, core and
+    per-iteration errors (orders 3 and 4, 3 iterations from the HOSVD factors)."""
+    rng = np.random.default_rng(6)
+    core = rng.standard_normal(int(np.prod(ranks)))
+    Us = [np.linalg.qr(rng.standard_normal((d, r)))[0] for d, r in zip(dims, ranks)]
+    X = O.tucker_reconstruct(dims, ranks, core, Us)
+    E = rng.standard_normal(X.size)
+    X = (X + 5e-2 * np.linalg.norm(X) / np.linalg.norm(E) * E).astype(np.float32)
+    U0, _ = O.tucker_hosvd(X, dims, ranks)
+    Uo, co, ho = O.tucker_hooi(X, dims, ranks, 3, U0)
+    Ug, cg, hg, _ = O.tucker_hooi_gemm(X, dims, ranks, 3, U0)
+    for a, b in zip(Uo, Ug):
+        assert np.max(np.abs(a - b)) < 1e-10
+    assert np.max(np.abs(co - cg)) < 1e-10 * np.max(np.abs(co))
+    assert np.max(np.abs(ho - hg)) < 1e-12
+
+
+@pytest.mark.parametrize("dims,R", [((10, 9, 8, 7), 4), ((14, 12, 10), 5)])
+def test_cp_als_gemm_matches_loop_oracle(dims, R):
+    """The order-N GEMM ALS of the CPU baseline (C5 is order 4) gives the loop oracle's
+    per-sweep fast-fit errors and factors (orders 4 and 3)."""
+    rng = np.random.default_rng(7)
+    F = [rng.standard_normal((d, R)) for d in dims]
+    X = O.cp_reconstruct(dims, F)
+    E = rng.standard_normal(X.size)
+    X = (X + 1e-2 * np.linalg.norm(X) / np.linalg.norm(E) * E).astype(np.float32)
+    U0 = [rng.uniform(size=(d, R)) for d in dims]
+    Uo, fo, _, _ = O.cp_als(X, dims, U0, 12)
+    Ug, fg, _ = O.cp_als_gemm(X, dims, U0, 12)
+    assert np.max(np.abs(fo - fg)) < 1e-10
+    for a, b in zip(Uo, Ug):
+        assert np.max(np.abs(a - b)) < 1e-8 * np.max(np.abs(a))
+    _, fs, _ = O.cp_als_gemm(X, dims, U0, 12, stop_at=fg[3])
+    assert len(fs) == 4
+
