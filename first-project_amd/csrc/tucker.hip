@@ -86,14 +86,38 @@ __global__ __launch_bounds__(256) void k_gram_partial(const T *__restrict__ X, l
         for (int j = 0; j < 8; ++j) dst[(ty * 8 + i) * kGramMaxN + tx * 8 + j] = acc[i][j];
 }
 
-__global__ void k_gram_sum(const float *__restrict__ part, int nblk, long In, double *__restrict__ G) {
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < In * In; e += (long)gridDim.x * blockDim.x) {
-        // both triangles from the upper one, so G is symmetric bit for bit
-        const long i = e % In, j = e / In;
-        const long a = i < j ? i : j, b = i < j ? j : i;
-        double s = 0.0;
-        for (int k = 0; k < nblk; ++k) s += (double)part[(long)k * kGramMaxN * kGramMaxN + a * kGramMaxN + b];
-        G[e] = s;
+// G = the sum of the nblk FP32 partial Grams (pitch kGramMaxN), in FP64: a workgroup takes 64
+// consecutive entries (a, b) of the upper triangle (b fastest: contiguous reads) and its four
+// waves the four quarters of the block range (each summed in block order, 8 loads in
+// flight); the quarters are added in order and the sum written to both triangles, so G is
+// symmetric bit for bit.  Grid: ceil(In^2 / 64).
+__global__ __launch_bounds__(256) void k_gram_sum(const float *__restrict__ part, int nblk, long In,
+                                                  double *__restrict__ G) {
+    __shared__ double sh[4][64];
+    const int lane = threadIdx.x & 63, rg = threadIdx.x >> 6;
+    const long f = blockIdx.x * 64L + lane;
+    const long a = f / In, b = f - a * In;
+    const bool on = f < In * In && a <= b;
+    double s = 0.0;
+    if (on) {
+        const int k0 = (int)((long)rg * nblk / 4), k1 = (int)((long)(rg + 1) * nblk / 4);
+        const float *p = part + a * kGramMaxN + b;
+        constexpr long kStride = (long)kGramMaxN * kGramMaxN;
+        for (int k = k0; k < k1; k += 8) {
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = p[(long)(k + u < k1 ? k + u : k1 - 1) * kStride];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (k + u < k1) s += (double)v[u];
+        }
+    }
+    sh[rg][lane] = s;
+    __syncthreads();
+    if (rg == 0 && on) {
+        const double t = ((sh[0][lane] + sh[1][lane]) + sh[2][lane]) + sh[3][lane];
+        G[a + b * In] = t;
+        G[b + a * In] = t;
     }
 }
 
@@ -176,7 +200,7 @@ static int unfold_gram(const void *X, int xtype, int order, const long *dims, in
         const int nb = gram_mfma_launch(X, xtype, P, In, Q, W.gram_part, mb, s);
         if (nb < 0) return record(TDX_ERR_HIP);
         if (nb > 0) {
-            k_gram_sum<<<(int)ceil_div(In * In, 256), 256, 0, s>>>(W.gram_part, nb, In, G);
+            k_gram_sum<<<(int)ceil_div(In * In, 64L), 256, 0, s>>>(W.gram_part, nb, In, G);
             TDX_LAUNCH_CHECK();
             return TDX_OK;
         }
@@ -190,7 +214,7 @@ static int unfold_gram(const void *X, int xtype, int order, const long *dims, in
         k_gram_partial<uint16_t><<<(int)nblk, 256, 0, s>>>((const uint16_t *)X, P, In, Q, kper, W.gram_part);
     else
         k_gram_partial<float><<<(int)nblk, 256, 0, s>>>((const float *)X, P, In, Q, kper, W.gram_part);
-    k_gram_sum<<<(int)ceil_div(In * In, 256), 256, 0, s>>>(W.gram_part, (int)nblk, In, G);
+    k_gram_sum<<<(int)ceil_div(In * In, 64L), 256, 0, s>>>(W.gram_part, (int)nblk, In, G);
     TDX_LAUNCH_CHECK();
     return TDX_OK;
 }
