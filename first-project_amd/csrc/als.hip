@@ -1327,6 +1327,9 @@ int tdx_cp_als_ex(const float *X, int order, const long *dims, long rank, float 
     else tdx_cp_default_options(&opt);
     if (opt.max_sweeps < 0 || opt.refine_sweeps < 0 || opt.opening_sweeps < 0) return record(TDX_ERR_ARG);
     if (opt.storage != TDX_F32 && opt.storage != TDX_BF16 && opt.storage != TDX_FP8_E4M3) return record(TDX_ERR_ARG);
+    // the extrapolation step runs on FP32 storage only: with BF16 / FP8 storage it is switched
+    // off (a run of it ended in a host segmentation fault whose cause is not established)
+    if (opt.line_search != 0 && opt.storage != TDX_F32) return record(TDX_ERR_UNSUPPORTED);
     long N = 1;
     for (int m = 0; m < order; ++m) {
         if (dims[m] <= 0 || !factors[m]) return record(TDX_ERR_ARG);

@@ -206,7 +206,8 @@ typedef struct {
                                 accepted only if its exact relative error is lower (Tensorlab
                                 cpd_als.m:234-339): 0 none (plain ALS), 1 alpha from the MLP
                                 (mlp_params), 2 alpha = (sweep+1)^(1/3) (cpd_lsb.m:30),
-                                3 no step, record the relative error at every grid alpha */
+                                3 no step, record the relative error at every grid alpha.
+                                FP32 storage only: with BF16 / FP8 storage TDX_ERR_UNSUPPORTED */
     int use_graph;           /* capture each sweep in a hipGraph (default 1) */
     const float *mlp_params; /* device MLP parameters, nin = TDX_LS_FEATURES, nout = 1 */
     int mlp_width;

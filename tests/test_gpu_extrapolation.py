@@ -42,6 +42,20 @@ def test_lsb_accept_rule_and_exact_evaluation():
     assert abs(direct - fit[-1]) < 2e-6
 
 
+@pytest.mark.parametrize("storage", [tdx.TDX_BF16, tdx.TDX_FP8_E4M3])
+def test_line_search_with_reduced_storage_is_rejected(storage):
+    # switched off for BF16 / FP8 storage: the call returns before any device work
+    X = E.swamp_tensor(5, DIMS, R)
+    U = E.init_factors(5, DIMS, R)
+    before = [u.clone() for u in U]
+    o = tdx.default_cp_options()
+    o.max_sweeps, o.refine_sweeps, o.storage, o.init, o.line_search = 4, 1, storage, 1, tdx.LS_LSB
+    with pytest.raises(tdx.TdxError) as e:
+        tdx.cp_als_ex(X, DIMS, R, U, o)
+    assert e.value.status == tdx.TDX_ERR_UNSUPPORTED
+    assert all(bool((a == b).all()) for a, b in zip(U, before))
+
+
 def test_learned_step_gains_over_plain():
     feats, labels = E.collect(range(24), DIMS, R, sweeps=40)
     assert feats.shape == (24 * 40, tdx.LS_FEATURES)
