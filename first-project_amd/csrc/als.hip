@@ -54,20 +54,58 @@ __device__ __forceinline__ double ordered_sum(const double *p, long stride, int 
 // About 4 KB of LDS, so that the workgroup fits on a CU beside an MTTKRP or TTM workgroup
 // (up to 136 KB): V is read from the Grams, and the rare fallback works in fb_ws (global,
 // 2 R^2 doubles: a copy of V and the eigenvectors).
+// V[i][c] = product of the other modes' Grams in mode order, ((G_a G_b) G_c), the value of
+// multiplying them into 1.0 one mode after another.  A fill of a lane's register row issues
+// every load first: load(s, i, c, in) loads slot s's entries (from entry (0, 0) when !in, so
+// all addresses are valid), finish() waits once and combines (0 where !in).
+// (A loop over the modes with a branch per mode, or the scheduler's default interleaving,
+// waited for each load in turn: 32 round trips per lane at R = 64.)
+struct VGather {
+    const double *Ga, *Gb, *Gc;  // Gc is read and not used at order 3
+    int R;
+    bool four;
+    double x[3][16];
+    bool on[16];
+    __device__ double operator()(int i, int c) const {  // one entry, loads in order
+        const long e = i + (long)c * R;
+        const double v = Ga[e] * Gb[e];
+        return four ? v * Gc[e] : v;
+    }
+    // slot s (a compile-time index after unrolling) of the lane's row
+    __device__ void load(int s, int i, int c, bool in) {
+        const long e = in ? i + (long)c * R : 0;
+        x[0][s] = Ga[e];
+        x[1][s] = Gb[e];
+        x[2][s] = Gc[e];
+        on[s] = in;
+    }
+    template <int NS>
+    __device__ void finish(double (&a)[NS]) {
+        __builtin_amdgcn_sched_barrier(0);  // every load above is issued before the first use
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const double v = x[0][s] * x[1][s];
+            a[s] = on[s] ? (four ? v * x[2][s] : v) : 0.0;
+        }
+    }
+};
+
 // R <= 16: the same sweep operator in one wave, without barriers.  Lane l holds A[i][c] for
 // row i = l % 16 and columns c = l / 16 + 4 s (s = 0..3).  A stays symmetric under the sweep,
 // so pivot k's row equals its column, which sits in slot k / 4 of the lanes with
 // l / 16 = k % 4 and reaches every lane by ds_bpermute.  Returns 1 (uniform) on a failed pivot.
 template <class VAt>
-__device__ inline int sweep_inverse_wave16(VAt vat, double *Vinv, int R) {
+__device__ inline int sweep_inverse_wave16(VAt &vat, double *Vinv, int R) {
     const int l = threadIdx.x & 63, i = l & 15, cg = l >> 4;
     double a[4], a0[4];
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
         const int c = cg + 4 * s;
-        a[s] = (i < R && c < R) ? vat(i, c) : 0.0;
-        a0[s] = a[s];
+        vat.load(s, i, c, i < R && c < R);
     }
+    vat.finish(a);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) a0[s] = a[s];
     int bad = 0;  // uniform: every lane reads the same pivot
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
@@ -111,12 +149,15 @@ __device__ void vinv_block(const double *const *G, int order, int mode, int R, d
     __shared__ double sh[256];
     __shared__ double cs[4 * (kMaxR / 2 + 1)];
     __shared__ int fail;
-    auto vat = [&](int i, int c) {
-        double v = 1.0;
-        for (int m = 0; m < order; ++m)
-            if (m != mode) v *= G[m][i + c * R];
-        return v;
-    };
+    // V[i][c] = product of the other modes' Grams in mode order, ((G_a G_b) G_c), the value
+    // of multiplying them into 1.0 one mode after another.  Straight-line loads (the third is
+    // read and not used at order 3), so a caller's unrolled fill issues every load before its
+    // first wait (a loop over the modes with a branch per mode waited for each load in turn).
+    const double *Ga = (mode == 0) ? G[1] : G[0];
+    const double *Gb = (mode <= 1) ? G[2] : G[1];
+    const double *Gc = (order > 3) ? ((mode <= 2) ? G[3] : G[2]) : Ga;
+    const bool four = order > 3;
+    VGather vat{Ga, Gb, Gc, R, four};
     int bad;
     if constexpr (NS == 4) {
         if (threadIdx.x < 64) {
@@ -126,13 +167,12 @@ __device__ void vinv_block(const double *const *G, int order, int mode, int R, d
         __syncthreads();
         bad = fail;
     } else {
-        for (int k = threadIdx.x; k < R; k += blockDim.x) dg[k] = vat(k, k);
-        // (sweep_inverse_rows starts with a barrier)
+        // (sweep_inverse_rows writes the diagonal into dg and starts with a barrier)
         bad = sweep_inverse_rows<NS>(vat, dg, Wb, Vinv, R, &fail);
     }
     if (bad) {
         double *L = fb_ws, *Q = fb_ws + R * R;
-        for (int e = threadIdx.x; e < R * R; e += blockDim.x) L[e] = vat(e % R, e / R);
+        for (int e = threadIdx.x; e < R * R; e += blockDim.x) L[e] = vat(e % R, e / R);  // rare path
         __syncthreads();
         pinv_jacobi(L, Q, Vinv, R, cs, sh);
         if (threadIdx.x == 0) atomicOr(status, 1);
@@ -343,11 +383,16 @@ __global__ __launch_bounds__(256) void k_gram_psum_fit(const double *__restrict_
     const double *G[4] = {G0, G1, G2, G3};
     double h = 0.0;
     if (e < RR) {
+        // the other modes' Gram entries (straight-line loads, in flight with the partials'),
+        // multiplied into the sum in mode order
+        const double *Ga = (mode == 0) ? G[1] : G[0];
+        const double *Gb = (mode <= 1) ? G[2] : G[1];
+        const double *Gc = (order > 3) ? ((mode <= 2) ? G[3] : G[2]) : Ga;
+        const double g0 = Ga[e], g1 = Gb[e], g2 = Gc[e];
         const double s = ordered_sum(gram_part + e, RR, nblk);
         Gout[e] = s;
-        h = s;
-        for (int m = 0; m < order; ++m)
-            if (m != mode) h *= G[m][e];
+        h = s * g0 * g1;
+        if (order > 3) h *= g2;
     }
     h = block_sum(h, sh);
     if (tid == 0) {
@@ -1053,8 +1098,14 @@ static int enqueue_sweep(CpPlan &P, const Storage &S, hipStream_t s, bool fit) {
     };
     // R <= 16 (every inverse in line): mode n's Gram is summed by mode n + 1's inverse
     int pending_gram = -1, pending_nblk = 0;
-    // forked[n]: mode n's inverse was already forked to the side stream (after the previous
-    // mode's solve, behind that mode's Gram sum)
+    int prev_nblk = 0;  // Gram-partial blocks of the mode whose Gram a forked branch sums
+    // forked[n]: the fork point of mode n's inverse was recorded after the previous mode's
+    // solve (its branch also sums that mode's Gram).  A side branch is enqueued after the main
+    // kernel it runs beside, with the fork event recorded before that kernel: the dependencies
+    // are the same, and a captured graph then has the main kernel as the first successor of
+    // the fork point, which keeps the chain MTTKRP -> solve -> MTTKRP on one hardware queue
+    // (with the side branch first, each of those edges crossed queues: ~9-15 us each instead
+    // of ~5 at 1024^3, R 64).
     bool forked[4] = {false, false, false, false};
     for (int n = 0; n < P.order; ++n) {
         const bool dt = S.dt.on && n > 0;
@@ -1062,12 +1113,11 @@ static int enqueue_sweep(CpPlan &P, const Storage &S, hipStream_t s, bool fit) {
         if (!forked[n]) {
             if (beside) {
                 TDX_HIP(hipEventRecord(P.ev_fork[n], s));
-                TDX_HIP(hipStreamWaitEvent(P.side, P.ev_fork[n], 0));
+            } else {
+                launch_vinv(n, s, pending_gram, pending_nblk);
+                TDX_LAUNCH_CHECK();
+                pending_gram = -1;
             }
-            launch_vinv(n, beside ? P.side : s, pending_gram, pending_nblk);
-            TDX_LAUNCH_CHECK();
-            pending_gram = -1;
-            if (beside) TDX_HIP(hipEventRecord(P.ev_join[n], P.side));
         }
         const float *panels[4] = {P.panel[0], P.panel[1], P.panel[2], P.panel[3]};
         const float *row_scale = (S.dtype == TDX_FP8_E4M3 && n == 0) ? S.scales : nullptr;
@@ -1087,7 +1137,19 @@ static int enqueue_sweep(CpPlan &P, const Storage &S, hipStream_t s, bool fit) {
             st = mttkrp_launch(S.mp[n], S.X, panels, row_scale, P.partial, s, skip);
         }
         if (st) return st;
-        if (beside) TDX_HIP(hipStreamWaitEvent(s, P.ev_join[n], 0));
+        if (beside) {
+            // the side branch behind the fork point: (the previous mode's Gram sum,) the inverse
+            TDX_HIP(hipStreamWaitEvent(P.side, P.ev_fork[n], 0));
+            if (forked[n]) {
+                k_gram_psum<<<(int)ceil_div((long)R * R, 256L), 256, 0, P.side>>>(P.gram_part, prev_nblk, R * R,
+                                                                                 P.G[n - 1]);
+                TDX_LAUNCH_CHECK();
+            }
+            launch_vinv(n, P.side);
+            TDX_LAUNCH_CHECK();
+            TDX_HIP(hipEventRecord(P.ev_join[n], P.side));
+            TDX_HIP(hipStreamWaitEvent(s, P.ev_join[n], 0));
+        }
         const long In = P.dims[n];
         const int nblk = (int)ceil_div(In, kSolveRows);
         const float *pscale = (S.dtype == TDX_FP8_E4M3 && n == 0) ? S.scales : nullptr;
@@ -1110,14 +1172,9 @@ static int enqueue_sweep(CpPlan &P, const Storage &S, hipStream_t s, bool fit) {
                                                    P.ip_part, nblk, P.normx2, P.fit_hist, P.counter, P.hpart,
                                                    P.fit_done);
             } else if (early) {
-                TDX_HIP(hipEventRecord(P.ev_fork[n + 1], s));
-                TDX_HIP(hipStreamWaitEvent(P.side, P.ev_fork[n + 1], 0));
-                k_gram_psum<<<gb, 256, 0, P.side>>>(P.gram_part, nblk, R * R, P.G[n]);
-                TDX_LAUNCH_CHECK();
-                launch_vinv(n + 1, P.side);
-                TDX_LAUNCH_CHECK();
-                TDX_HIP(hipEventRecord(P.ev_join[n + 1], P.side));
+                TDX_HIP(hipEventRecord(P.ev_fork[n + 1], s));  // the branch follows mode n + 1's kernel
                 forked[n + 1] = true;
+                prev_nblk = nblk;
             } else if (R <= 16 && n + 1 < P.order && !beside_of(n + 1)) {
                 pending_gram = n;  // the next inverse sums it (one launch fewer per mode)
                 pending_nblk = nblk;

@@ -218,17 +218,22 @@ __device__ inline int block_spd_inverse(const double *V, double *L, double *W, d
 // w, w+4, ...) are contiguous: the entry of row c at (c % 4) * 16 + c / 4 (16-byte broadcast
 // reads).  One barrier per pivot (the halves alternate).  Returns 1 (every thread) when a
 // relative pivot is below 1e-13 or not finite; Vinv is then not written.  Wb: 128 doubles.
+// diag (R doubles, LDS) receives V[k][k] from the lanes that hold it.
 template <int NS, typename VAt>
-__device__ inline int sweep_inverse_rows(VAt vat, const double *diag, double *Wb, double *Vinv, int R, int *fail) {
+__device__ inline int sweep_inverse_rows(VAt &vat, double *diag, double *Wb, double *Vinv, int R, int *fail) {
     static_assert(NS == 4 || NS == 8 || NS == 16, "NS: slots per lane");
     const int tid = threadIdx.x, i = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     double a[NS];
 #pragma unroll
-    for (int s = 0; s < NS; ++s) {
+    for (int s = 0; s < NS; ++s) {  // every load issued, then one wait (see VGather)
         const int c = w + 4 * s;
-        a[s] = (i < R && c < R) ? vat(i, c) : 0.0;
+        vat.load(s, i, c, i < R && c < R);
     }
+    vat.finish(a);
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+        if (w + 4 * s == i && i < R) diag[i] = a[s];
     const int pi = (i & 3) * 16 + (i >> 2);  // this lane's row in the permuted column
     if (tid == 0) *fail = 0;
     __syncthreads();
