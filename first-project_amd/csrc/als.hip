@@ -19,6 +19,7 @@
 // A sweep is captured once into a hipGraph and replayed.
 #include "mttkrp.hpp"
 #include "linalg.hpp"
+#include "vinv.hpp"
 #include <memory>
 #include <mutex>
 #include "mlp.hpp"
@@ -28,156 +29,8 @@
 
 namespace tdx {
 
-static constexpr int kMaxR = 64;     // rank limit of the LDS-resident solve
 static constexpr int kSolveRows = 16;  // rows per k_solve workgroup
 static constexpr int kGramRows = 64;   // rows per Gram-partial workgroup
-
-// p[0] + p[stride] + ... + p[(n - 1) stride], added in index order, 16 loads in flight (the
-// ones past n re-read a valid entry and are not added)
-__device__ __forceinline__ double ordered_sum(const double *p, long stride, int n) {
-    double s = 0.0;
-    for (int b = 0; b < n; b += 16) {
-        double v[16];
-#pragma unroll
-        for (int u = 0; u < 16; ++u) v[u] = p[(long)(b + u < n ? b + u : n - 1) * stride];
-#pragma unroll
-        for (int u = 0; u < 16; ++u)
-            if (b + u < n) s += v[u];
-    }
-    return s;
-}
-
-// V = hadamard_{m != mode} G_m; Vinv = V^{-1} by the symmetric sweep operator (FP64, the
-// register-row form of sweep_inverse_rows); if a pivot fails, a pseudo-inverse from the
-// Jacobi eigendecomposition (eigenvalues below 1e-12 * max dropped) and status |= 1.
-// NS: register slots per lane, R <= 4 NS (4, 8 or 16 for R <= 16, 32, 64).
-// About 4 KB of LDS, so that the workgroup fits on a CU beside an MTTKRP or TTM workgroup
-// (up to 136 KB): V is read from the Grams, and the rare fallback works in fb_ws (global,
-// 2 R^2 doubles: a copy of V and the eigenvectors).
-// V[i][c] = product of the other modes' Grams in mode order, ((G_a G_b) G_c), the value of
-// multiplying them into 1.0 one mode after another.  A fill of a lane's register row issues
-// every load first: load(s, i, c, in) loads slot s's entries (from entry (0, 0) when !in, so
-// all addresses are valid), finish() waits once and combines (0 where !in).
-// (A loop over the modes with a branch per mode, or the scheduler's default interleaving,
-// waited for each load in turn: 32 round trips per lane at R = 64.)
-struct VGather {
-    const double *Ga, *Gb, *Gc;  // Gc is read and not used at order 3
-    int R;
-    bool four;
-    double x[3][16];
-    bool on[16];
-    __device__ double operator()(int i, int c) const {  // one entry, loads in order
-        const long e = i + (long)c * R;
-        const double v = Ga[e] * Gb[e];
-        return four ? v * Gc[e] : v;
-    }
-    // slot s (a compile-time index after unrolling) of the lane's row
-    __device__ void load(int s, int i, int c, bool in) {
-        const long e = in ? i + (long)c * R : 0;
-        x[0][s] = Ga[e];
-        x[1][s] = Gb[e];
-        x[2][s] = Gc[e];
-        on[s] = in;
-    }
-    template <int NS>
-    __device__ void finish(double (&a)[NS]) {
-        __builtin_amdgcn_sched_barrier(0);  // every load above is issued before the first use
-#pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            const double v = x[0][s] * x[1][s];
-            a[s] = on[s] ? (four ? v * x[2][s] : v) : 0.0;
-        }
-    }
-};
-
-// R <= 16: the same sweep operator in one wave, without barriers.  Lane l holds A[i][c] for
-// row i = l % 16 and columns c = l / 16 + 4 s (s = 0..3).  A stays symmetric under the sweep,
-// so pivot k's row equals its column, which sits in slot k / 4 of the lanes with
-// l / 16 = k % 4 and reaches every lane by ds_bpermute.  Returns 1 (uniform) on a failed pivot.
-template <class VAt>
-__device__ inline int sweep_inverse_wave16(VAt &vat, double *Vinv, int R) {
-    const int l = threadIdx.x & 63, i = l & 15, cg = l >> 4;
-    double a[4], a0[4];
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        const int c = cg + 4 * s;
-        vat.load(s, i, c, i < R && c < R);
-    }
-    vat.finish(a);
-#pragma unroll
-    for (int s = 0; s < 4; ++s) a0[s] = a[s];
-    int bad = 0;  // uniform: every lane reads the same pivot
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-        if (k < R && !bad) {
-            const int src = 16 * (k & 3);
-            const double v = a[k >> 2], v0 = a0[k >> 2];
-            const double d = __shfl(v, src + k), dk = __shfl(v0, src + k);
-            const double ti = __shfl(v, src + i);  // A[i][k]
-            double wr[4];                           // A[k][cg + 4 s] = A[cg + 4 s][k]
-#pragma unroll
-            for (int s = 0; s < 4; ++s) wr[s] = __shfl(v, src + cg + 4 * s);
-            if (!(d > 1e-13 * dk) || !isfinite(d)) {
-                bad = 1;
-            } else {
-                const double dinv = 1.0 / d;
-                const double t = ti * dinv;
-                const bool rk = i == k;
-                const double alpha = rk ? 0.0 : 1.0, beta = rk ? dinv : -t;
-#pragma unroll
-                for (int s = 0; s < 4; ++s) a[s] = fma(beta, wr[s], alpha * a[s]);
-                if (cg == (k & 3)) a[k >> 2] = rk ? -dinv : t;
-            }
-        }
-    }
-    if (bad) return 1;
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        const int c = cg + 4 * s;
-        if (i < R && c < R) Vinv[i + c * R] = -a[s];
-    }
-    return 0;
-}
-
-// The inverse for one workgroup of 256 threads (the body of k_vinv).  G: the current Grams
-// (G[mode] unused).
-template <int NS>
-__device__ void vinv_block(const double *const *G, int order, int mode, int R, double *Vinv, int *status,
-                           double *fb_ws) {
-    __shared__ double Wb[128];
-    __shared__ double dg[kMaxR];
-    __shared__ double sh[256];
-    __shared__ double cs[4 * (kMaxR / 2 + 1)];
-    __shared__ int fail;
-    // V[i][c] = product of the other modes' Grams in mode order, ((G_a G_b) G_c), the value
-    // of multiplying them into 1.0 one mode after another.  Straight-line loads (the third is
-    // read and not used at order 3), so a caller's unrolled fill issues every load before its
-    // first wait (a loop over the modes with a branch per mode waited for each load in turn).
-    const double *Ga = (mode == 0) ? G[1] : G[0];
-    const double *Gb = (mode <= 1) ? G[2] : G[1];
-    const double *Gc = (order > 3) ? ((mode <= 2) ? G[3] : G[2]) : Ga;
-    const bool four = order > 3;
-    VGather vat{Ga, Gb, Gc, R, four};
-    int bad;
-    if constexpr (NS == 4) {
-        if (threadIdx.x < 64) {
-            const int b = sweep_inverse_wave16(vat, Vinv, R);
-            if (threadIdx.x == 0) fail = b;
-        }
-        __syncthreads();
-        bad = fail;
-    } else {
-        // (sweep_inverse_rows writes the diagonal into dg and starts with a barrier)
-        bad = sweep_inverse_rows<NS>(vat, dg, Wb, Vinv, R, &fail);
-    }
-    if (bad) {
-        double *L = fb_ws, *Q = fb_ws + R * R;
-        for (int e = threadIdx.x; e < R * R; e += blockDim.x) L[e] = vat(e % R, e / R);  // rare path
-        __syncthreads();
-        pinv_jacobi(L, Q, Vinv, R, cs, sh);
-        if (threadIdx.x == 0) atomicOr(status, 1);
-    }
-}
 
 // gram_part != nullptr: first G_gmode (Gsum) = the ordered sum of the nblk Gram partials of
 // that mode (what k_gram_psum computes, same order), so a small-rank sweep, whose inverses
@@ -790,6 +643,7 @@ struct CpPlan {
     double *vinv_ws = nullptr;  // k_vinv fallback workspace, 2 R^2 doubles
     double *hpart = nullptr;    // k_gram_psum_fit: per-block partials (R^2 / 256 doubles)
     int *fit_done = nullptr;    // k_gram_psum_fit: ticket (0 between launches)
+    int *m2_done = nullptr;     // k_m2_vinv: ticket (0 between launches)
     bool shared_pass = true;    // modes 1 and 2 from one pass over X where eligible
     float *T = nullptr;         // shared-pass intermediate X x_0 A^T (I2 x I1 x Rp)
     size_t t_floats = 0;
@@ -841,12 +695,12 @@ struct CpPlan {
             panel[m] = nullptr; G[m] = nullptr; Vinv[m] = nullptr;
         }
         void *ps[] = {partial, msum, ip_part, gram_part, fit_hist, counter, status, normx2, scratch, vinv_ws, T,
-                      hpart, fit_done};
+                      hpart, fit_done, m2_done};
         for (void *p : ps)
             if (p) (void)hipFree(p);
         partial = nullptr; msum = nullptr; ip_part = nullptr; gram_part = nullptr; fit_hist = nullptr;
         counter = nullptr; status = nullptr; normx2 = nullptr; scratch = nullptr; vinv_ws = nullptr; T = nullptr;
-        hpart = nullptr; fit_done = nullptr;
+        hpart = nullptr; fit_done = nullptr; m2_done = nullptr;
         t_floats = 0;
         if (events) {
             for (int m = 0; m < 4; ++m) {
@@ -914,6 +768,8 @@ static int plan_setup(CpPlan &P, int order, const long *dims, long R, float *con
     TDX_HIP(hipMalloc(&P.hpart, sizeof(double) * ceil_div(R * R, 256L)));
     TDX_HIP(hipMalloc(&P.fit_done, sizeof(int)));
     TDX_HIP(hipMemset(P.fit_done, 0, sizeof(int)));
+    TDX_HIP(hipMalloc(&P.m2_done, sizeof(int)));
+    TDX_HIP(hipMemset(P.m2_done, 0, sizeof(int)));
     if (P.ls_mode) {
         auto alloc = [&](void **p, size_t bytes) -> int {
             TDX_HIP(hipMalloc(p, bytes));
@@ -1080,7 +936,11 @@ static int enqueue_sweep(CpPlan &P, const Storage &S, hipStream_t s, bool fit) {
     // mode 2 from T.  The inverse (4 KB of LDS) fits beside either kernel's workgroups.
     // The fork and join between the streams cost ~15 us of replay time per mode, more
     // than the inverse itself at R <= 16 (~7 us), so small ranks keep it in line.
+    // Order 3 with the shared pass, R > 16: mode 1's Gram sum and mode 2's inverse run inside
+    // the pass over T (k_m2_vinv), so mode 2 has no inverse launch of its own.
+    const bool fuse2 = S.dt.on && P.order == 3 && R > 16;
     auto beside_of = [&](int n) {
+        if (fuse2 && n == 2) return false;
         const bool dt = S.dt.on && n > 0;
         return R > 16 && (dt || S.mp[n].rowtiles * (long)S.mp[n].nsplit <= device_cus());
     };
@@ -1110,7 +970,7 @@ static int enqueue_sweep(CpPlan &P, const Storage &S, hipStream_t s, bool fit) {
     for (int n = 0; n < P.order; ++n) {
         const bool dt = S.dt.on && n > 0;
         const bool beside = beside_of(n);
-        if (!forked[n]) {
+        if (!forked[n] && !(fuse2 && n == 2)) {
             if (beside) {
                 TDX_HIP(hipEventRecord(P.ev_fork[n], s));
             } else {
@@ -1130,7 +990,12 @@ static int enqueue_sweep(CpPlan &P, const Storage &S, hipStream_t s, bool fit) {
             nsplit = S.dt.nsplit1;
             rows_pad = S.dt.I1;
         } else if (dt && n == 2) {
-            st = dtree_m2_launch(S.dt, P.T, P.panel[1], P.partial, s);
+            if (fuse2)
+                st = dtree_m2_vinv_launch(S.dt, P.T, P.panel[1], P.partial, P.gram_part,
+                                          (int)ceil_div(P.dims[1], kSolveRows), P.G[0], P.G[1], R, P.Vinv[2],
+                                          P.status, P.vinv_ws, P.m2_done, s);
+            else
+                st = dtree_m2_launch(S.dt, P.T, P.panel[1], P.partial, s);
             nsplit = S.dt.nsplit2;
             rows_pad = S.dt.I2;
         } else {
@@ -1175,6 +1040,8 @@ static int enqueue_sweep(CpPlan &P, const Storage &S, hipStream_t s, bool fit) {
                 TDX_HIP(hipEventRecord(P.ev_fork[n + 1], s));  // the branch follows mode n + 1's kernel
                 forked[n + 1] = true;
                 prev_nblk = nblk;
+            } else if (fuse2 && n == 1) {
+                // G_1 is summed by the pass over T (k_m2_vinv)
             } else if (R <= 16 && n + 1 < P.order && !beside_of(n + 1)) {
                 pending_gram = n;  // the next inverse sums it (one launch fewer per mode)
                 pending_nblk = nblk;

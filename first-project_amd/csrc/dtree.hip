@@ -38,6 +38,7 @@
 //   Rp*4 contiguous bytes) against B_new from its panel (8 rows per 32-byte load), FP32
 //   FMAs in i1 order; partial[split][i2][Rp].
 #include "mttkrp.hpp"
+#include "vinv.hpp"
 
 namespace tdx {
 
@@ -319,15 +320,16 @@ __global__ __launch_bounds__(256, 1) void k_ttm_m1(DtArgs a) {
                                           4 * g) = m1[rb][c];
 }
 
-// M2 partials: lane (i2 = GPW i2 per wave, r = 4 q .. 4 q + 3), i1 in [split i1per, ...)
+// M2 partials: lane (i2 = GPW i2 per wave, r = 4 q .. 4 q + 3), i1 in [split i1per, ...);
+// blk is the block's index among the pass's blocks
 template <int NQ>
-__global__ __launch_bounds__(256) void k_m2(const float *__restrict__ T, const float *__restrict__ panelB,
-                                            float *__restrict__ partial, long I1, long I2, int Rp, int i1per,
-                                            int nb2) {
+__device__ __forceinline__ void m2_block(const float *__restrict__ T, const float *__restrict__ panelB,
+                                         float *__restrict__ partial, long I1, long I2, int Rp, int i1per, int nb2,
+                                         int blk) {
     constexpr int GPW = 64 / NQ;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int q = lane % NQ, gi = lane / NQ;
-    const int split = blockIdx.x / nb2, b = blockIdx.x % nb2;
+    const int split = blk / nb2, b = blk % nb2;
     const long i2 = ((long)b * 4 + wave) * GPW + gi;
     if (i2 >= I2) return;  // no barriers in this kernel
     const long i1s = (long)split * i1per;
@@ -362,6 +364,45 @@ __global__ __launch_bounds__(256) void k_m2(const float *__restrict__ T, const f
     }
     *reinterpret_cast<float4 *>(partial + ((long)split * I2 + i2) * Rp + 4 * q) =
         float4{acc[0], acc[1], acc[2], acc[3]};
+}
+
+template <int NQ>
+__global__ __launch_bounds__(256) void k_m2(const float *__restrict__ T, const float *__restrict__ panelB,
+                                            float *__restrict__ partial, long I1, long I2, int Rp, int i1per,
+                                            int nb2) {
+    m2_block<NQ>(T, panelB, partial, I1, I2, Rp, i1per, nb2, blockIdx.x);
+}
+
+// The pass over T with the mode-2 side work of the sweep in the same launch: blocks
+// [0, ngb) sum mode 1's Gram partials into G1 (each entry in block order, as k_gram_psum),
+// and the last of them to finish (atomic ticket, reset by that block) computes the mode-2
+// inverse (G0 * G1)^{-1} (as k_vinv); blocks ngb.. are k_m2's.  The Gram blocks come first in
+// dispatch order, so the inverse starts at once instead of waiting for a separate Gram
+// launch, a fork and a join; the solve after this kernel needs nothing else.
+template <int NQ, int NS>
+__global__ __launch_bounds__(256) void k_m2_vinv(const float *__restrict__ T, const float *__restrict__ panelB,
+                                                 float *__restrict__ partial, long I1, long I2, int Rp, int i1per,
+                                                 int nb2, int ngb, const double *__restrict__ gram_part, int gnblk,
+                                                 const double *G0, double *G1, int R, double *__restrict__ Vinv,
+                                                 int *__restrict__ status, double *__restrict__ fb_ws,
+                                                 int *__restrict__ done) {
+    if ((int)blockIdx.x >= ngb) {
+        m2_block<NQ>(T, panelB, partial, I1, I2, Rp, i1per, nb2, (int)blockIdx.x - ngb);
+        return;
+    }
+    __shared__ int last;
+    const int RR = R * R;
+    const int e = (int)blockIdx.x * 256 + (int)threadIdx.x;
+    if (e < RR) G1[e] = ordered_sum(gram_part + e, RR, gnblk);
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) last = (atomicAdd(done, 1) == ngb - 1);
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    if (threadIdx.x == 0) *done = 0;
+    const double *G[4] = {G0, G1, G0, G0};
+    vinv_block<NS>(G, 3, 2, R, Vinv, status, fb_ws);
 }
 
 // ---------------------------------------------------------------------------
@@ -464,6 +505,25 @@ int dtree_m1_launch(const DtreePlan &d, const void *X, const float *panelA, cons
     a.blocks_i1 = d.blocks_i1;
     const int rc = m1_call(d, &a, s);
     if (rc) return rc;
+    TDX_LAUNCH_CHECK();
+    return TDX_OK;
+}
+
+int dtree_m2_vinv_launch(const DtreePlan &d, const float *T, const float *panelB, float *partial,
+                         const double *gram_part, int gnblk, const double *G0, double *G1, int R, double *Vinv,
+                         int *status, double *fb_ws, int *done, hipStream_t s) {
+    if (!d.on || R <= 16 || R > kMaxR) return record(TDX_ERR_UNSUPPORTED);
+    const int ngb = (int)ceil_div((long)R * R, 256L);
+    const int grid = ngb + d.nb2 * d.nsplit2;
+    const int nq = d.Rp / 4;
+#define TDX_M2V(NQ_, NS_)                                                                                         \
+    k_m2_vinv<NQ_, NS_><<<grid, 256, 0, s>>>(T, panelB, partial, d.I1, d.I2, d.Rp, d.i1per, d.nb2, ngb, gram_part, \
+                                             gnblk, G0, G1, R, Vinv, status, fb_ws, done)
+    // 16 < R <= 32: Rp 32; 32 < R <= 64 with the shared pass on: Rp 64 (Rp 48 turns it off)
+    if (nq == 8 && R <= 32) TDX_M2V(8, 8);
+    else if (nq == 16) TDX_M2V(16, 16);
+    else return record(TDX_ERR_UNSUPPORTED);
+#undef TDX_M2V
     TDX_LAUNCH_CHECK();
     return TDX_OK;
 }
