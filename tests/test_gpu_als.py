@@ -188,6 +188,35 @@ def test_shared_pass_fast_fit_is_the_iterates_error():
         assert np.array_equal(a, (O.bf16_bits(a).astype(np.uint32) << 16).view(np.float32))
 
 
+@pytest.mark.parametrize("R", [24, 56])
+def test_shared_pass_mode2_inverse_and_gram(R):
+    """Ranks 17..64 with the shared pass: mode 1's Gram is summed and mode 2's inverse formed
+    inside the pass over T.  The per-sweep fast fit (which uses that Gram) equals the direct
+    relative error of the returned factors (FP64 oracle) to 5e-5, and the sweeps follow the ones
+    without the shared pass (a fused MTTKRP per mode, separate inverse) to within BF16 rounding."""
+    dims = (256, 96, 64)
+    X, _ = make_cp_tensor(dims, R, eta=1e-3, kind="normal")
+    U0 = init_factors(dims, R, seed=7)
+    Xh = host(X)
+    res = {}
+    for shared in (1, 0):
+        Ud = [dev(u.ravel(order="F")) for u in U0]
+        o = tdx.default_cp_options()
+        o.max_sweeps, o.refine_sweeps, o.storage, o.init, o.record_fit = 4, 0, tdx.TDX_BF16, 1, 1
+        o.shared_pass = shared
+        st, rep, fit, ms = tdx.cp_als_ex(X, dims, R, Ud, o)
+        assert st == tdx.TDX_OK and rep.sweeps_done == 4
+        U = [host(u).reshape((d, R), order="F").astype(np.float64) for u, d in zip(Ud, dims)]
+        res[shared] = (np.asarray(fit), O.cp_relerr(Xh, dims, U))
+    fit1, true1 = res[1]
+    assert np.all(np.isfinite(fit1))
+    assert abs(fit1[-1] - true1) <= 5e-5, (fit1[-1], true1)
+    fit0, true0 = res[0]
+    assert not np.array_equal(fit1, fit0)  # the shared pass ran (another summation order)
+    assert np.max(np.abs(fit1 - fit0)) <= 1e-2, (fit1, fit0)
+    assert abs(true1 - true0) <= 1e-2, (true1, true0)
+
+
 def test_order4_cp_als():
     dims, R = (24, 16, 20, 8), 5
     X, _ = make_cp_tensor(dims, R, eta=1e-3)
