@@ -54,7 +54,7 @@ HBM_PEAK_GBS = 8000.0
 # plain 1 GiB copy); counters cannot be read in the timed run itself
 MTTKRP_BF16_TRAFFIC = 2171970286
 # the same for the shared pass over X (BF16 storage: X read, T written)
-SHARED_PASS_BF16_TRAFFIC = 2443205440
+SHARED_PASS_BF16_TRAFFIC = 2308532224
 TARGET_DELTA = 1e-4
 MAIN_SWEEPS, REFINE_SWEEPS, PLAIN_SWEEPS = 25, 2, 27
 ANCHOR_PATH = os.path.join(ROOT, "tests", "golden", "c2_fp64_anchor.json")
@@ -66,9 +66,9 @@ def log(msg):
 
 def shared_pass_bytes(s_bytes, N, n, R):
     """Algorithmic HBM bytes of the shared pass at n^3, rank R: the pass over X reads X (s N
-    bytes) and A, C and writes T (n^2 Rp FP32); the pass over T reads T and B."""
+    bytes) and A, C and writes T (n^2 Rp BF16); the pass over T reads T and B."""
     Rp = -(-R // 16) * 16
-    t_bytes = 4 * n * n * Rp
+    t_bytes = 2 * n * n * Rp
     return s_bytes * N + t_bytes + 2 * 4 * n * R, t_bytes + 4 * n * R
 
 

@@ -645,8 +645,8 @@ struct CpPlan {
     int *fit_done = nullptr;    // k_gram_psum_fit: ticket (0 between launches)
     int *m2_done = nullptr;     // k_m2_vinv: ticket (0 between launches)
     bool shared_pass = true;    // modes 1 and 2 from one pass over X where eligible
-    float *T = nullptr;         // shared-pass intermediate X x_0 A^T (I2 x I1 x Rp)
-    size_t t_floats = 0;
+    uint16_t *T = nullptr;      // shared-pass intermediate X x_0 A^T (BF16, I2 x I1 x Rp)
+    size_t t_bytes = 0;
     int max_sweeps_hist = 0;
     hipStream_t side = nullptr;
     hipEvent_t ev_fork[4], ev_join[4];
@@ -701,7 +701,7 @@ struct CpPlan {
         partial = nullptr; msum = nullptr; ip_part = nullptr; gram_part = nullptr; fit_hist = nullptr;
         counter = nullptr; status = nullptr; normx2 = nullptr; scratch = nullptr; vinv_ws = nullptr; T = nullptr;
         hpart = nullptr; fit_done = nullptr; m2_done = nullptr;
-        t_floats = 0;
+        t_bytes = 0;
         if (events) {
             for (int m = 0; m < 4; ++m) {
                 (void)hipEventDestroy(ev_fork[m]);
@@ -807,11 +807,11 @@ static int ensure_partial(CpPlan &P, const Storage &S) {
         if (S.mp[m].partial_floats() > need) need = S.mp[m].partial_floats();
     if (S.dt.on) {
         if (S.dt.partial_floats() > need) need = S.dt.partial_floats();
-        if (S.dt.t_floats() > P.t_floats) {
+        if (S.dt.t_bytes() > P.t_bytes) {
             if (P.T) TDX_HIP(hipFree(P.T));
             P.T = nullptr;
-            TDX_HIP(hipMalloc(&P.T, sizeof(float) * S.dt.t_floats()));
-            P.t_floats = S.dt.t_floats();
+            TDX_HIP(hipMalloc(&P.T, S.dt.t_bytes()));
+            P.t_bytes = S.dt.t_bytes();
         }
     }
     if (need > P.partial_floats) {

@@ -17,8 +17,8 @@
 //     panel, pre-scaled by the FP8 slice scales) is converted once into BF16 MFMA operands
 //     in LDS (I0 Rp 2 bytes, 128 KB at I0 = 1024, Rp = 64), in the exact per-lane order the
 //     MFMA reads (one conflict-free 16-byte read per lane per operand);
-//   * each wave computes T^T tiles (Rp x 32 columns i1) with v_mfma_f32_16x16x32_bf16: the
-//     A operand is A_new^T from LDS, the B operand X read straight from HBM into registers.
+//   * each wave computes T tiles (32 rows i1 x Rp) with v_mfma_f32_16x16x32_bf16: the A
+//     operand is X read straight from HBM into registers, the B operand A_new from LDS.
 //     The contraction order over i0 is free, so each lane owns 32 contiguous bytes of a
 //     column per "run" (two k-steps of BF16, four of FP8; FP8 widened exactly to BF16 with
 //     v_cvt_scalef32_pk_bf16_fp8).  Each 16-byte load instruction reads 8 whole 128-byte
@@ -29,20 +29,25 @@
 //     register buffers across the i2 loop, three chunks in flight while one is multiplied
 //     (nontemporal loads; every load is issued unconditionally, the chunks past the end
 //     re-read a valid one), and after the last chunk of a column i2 the tile is multiplied
-//     into the mode-1 partial by C[i2][:] (from LDS) and stored to T (16-byte nontemporal
-//     stores, r fastest, whole lines per instruction by the same lane exchange; nontemporal
-//     501 us against 527 with ordinary stores) after the next column's first chunk;
+//     into the mode-1 partial by C[i2][:] (from LDS, FP32) and stored to T after the next
+//     column's first chunk: T is BF16 (round to nearest even) in the MFMA output order, so a
+//     lane's 16 bytes per row block go out as they are and each store instruction writes 1 KB
+//     contiguous (NRB stores per tile; dt_t_index).  FP32 T written row-major with whole
+//     lines per store took ~90 us of a 461 us pass at 1024^3 R64, half of it the write
+//     traffic, half the issue of the stores;
 //   * the mode-1 partial of the workgroup's i2 range goes to partial[split][i1][Rp], the
 //     layout k_solve sums (in split order, FP64).
-// k_m2: lane (i2, 4 consecutive r) walks an i1 range of T rows (16-byte loads, a T row is
-//   Rp*4 contiguous bytes) against B_new from its panel (8 rows per 32-byte load), FP32
-//   FMAs in i1 order; partial[split][i2][Rp].
+// k_m2: a wave per (i2, range of tiles); per tile a lane reads one 16-byte piece of T per row
+//   block (8 BF16: four consecutive i1 of two 16-column halves, one r) and the matching rows
+//   of B_new (one 16-byte panel load per half), FP32 FMAs, the four lane groups added at the
+//   end; partial[split][i2][Rp].
 #include "mttkrp.hpp"
 #include "vinv.hpp"
 
 namespace tdx {
 
 typedef float dt_f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned dt_u32x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 dt_bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 dt_bf16x2 __attribute__((ext_vector_type(2)));
 
@@ -86,7 +91,7 @@ struct DtArgs {
     const void *X;
     const float *panelA;  // mode-0 panel (FP8: scaled by the slice scales)
     const float *C;       // mode-2 factor, column-major I2 x R
-    float *T;             // I2 x I1 x Rp, r fastest
+    uint16_t *T;          // BF16, tile layout (see dt_t_index)
     float *partial;       // nsplit1 x I1 x Rp
     long I0, I1, I2;
     int Rp, R;
@@ -214,50 +219,35 @@ __global__ __launch_bounds__(256, 1) void k_ttm_m1(DtArgs a) {
                 for (int rb = 0; rb < NRB; ++rb)
 #pragma unroll
                     for (int c = 0; c < 2; ++c)
-                        acc[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[rb], bv[c], acc[rb][c], 0, 0, 0);
+                        acc[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bv[c], av[rb], acc[rb][c], 0, 0, 0);
             }
     };
-    // finished tile (rows r = 16 rb + 4 g + v, columns i1 = i1w + 16 c + lm): into the mode-1
-    // partial now, saved for its T store
-    dt_f32x4 tsv[NRB][2];
+    // finished tile (rows i1 = i1w + 16 c + 4 g + v, columns r = 16 rb + lm): into the mode-1
+    // partial now, packed to BF16 (round to nearest even) for its T store
+    uint4 tpk[NRB];
     auto take = [&](long i2l) {
-        const float *cp = cs + i2l * a.Rp + 4 * g;
+        const float *cp = cs + i2l * a.Rp + lm;
 #pragma unroll
         for (int rb = 0; rb < NRB; ++rb) {
-            const dt_f32x4 cv = *reinterpret_cast<const dt_f32x4 *>(cp + 16 * rb);
+            const float cv = cp[16 * rb];
+            tpk[rb] = uint4{dt_pack(acc[rb][0][0], acc[rb][0][1]), dt_pack(acc[rb][0][2], acc[rb][0][3]),
+                            dt_pack(acc[rb][1][0], acc[rb][1][1]), dt_pack(acc[rb][1][2], acc[rb][1][3])};
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
-                tsv[rb][c] = acc[rb][c];
 #pragma unroll
-                for (int v = 0; v < 4; ++v) m1[rb][c][v] = fmaf(acc[rb][c][v], cv[v], m1[rb][c][v]);
+                for (int v = 0; v < 4; ++v) m1[rb][c][v] = fmaf(acc[rb][c][v], cv, m1[rb][c][v]);
                 acc[rb][c] = dt_f32x4{0.f, 0.f, 0.f, 0.f};
             }
         }
     };
-    // A tile register holds, per store, 16 T rows x 64 bytes (half of each 128-byte line).
-    // For Rp >= 32 the two tiles of a row-block pair (rb 2p, 2p+1: r in [32p, 32p+32), one
-    // line of each row) exchange lane bit 3 (DPP row_ror:8 is lane ^ 8 within 16 lanes), so
-    // that each store writes 8 rows x one whole line: rows lm & 7 from the first register,
-    // rows 8 + (lm & 7) from the second, r = 32p + 16 (lm >> 3) + 4g.
+    // T tile of this wave and column: NRB stores of 16 bytes per lane, each 1 KB contiguous
     auto store_t = [&](long i2l) {
-        if constexpr (NRB == 1) {
-            float *tp = a.T + ((i2s + i2l) * a.I1 + i1w + lm) * a.Rp + 4 * g;
+        unsigned char *tp = reinterpret_cast<unsigned char *>(a.T) +
+                            ((i2s + i2l) * (a.I1 / 32) + i1w / 32) * 64L * a.Rp + 16 * lane;
 #pragma unroll
-            for (int c = 0; c < 2; ++c)
-                __builtin_nontemporal_store(tsv[0][c], reinterpret_cast<dt_f32x4 *>(tp + 16L * c * a.Rp));
-        } else {
-            float *tp = a.T + ((i2s + i2l) * a.I1 + i1w + (lm & 7)) * a.Rp + 16 * (lm >> 3) + 4 * g;
-#pragma unroll
-            for (int p = 0; p < NRB / 2; ++p)
-#pragma unroll
-                for (int c = 0; c < 2; ++c) {
-                    uint4 lo = __builtin_bit_cast(uint4, tsv[2 * p][c]), hi = __builtin_bit_cast(uint4, tsv[2 * p + 1][c]);
-                    swap_halves(lo, hi, hi8);
-                    float *q = tp + 16L * c * a.Rp + 32 * p;
-                    __builtin_nontemporal_store(__builtin_bit_cast(dt_f32x4, lo), reinterpret_cast<dt_f32x4 *>(q));
-                    __builtin_nontemporal_store(__builtin_bit_cast(dt_f32x4, hi), reinterpret_cast<dt_f32x4 *>(q + 8L * a.Rp));
-                }
-        }
+        for (int rb = 0; rb < NRB; ++rb)
+            __builtin_nontemporal_store(dt_u32x4{tpk[rb].x, tpk[rb].y, tpk[rb].z, tpk[rb].w},
+                                        reinterpret_cast<dt_u32x4 *>(tp + 1024 * rb));
     };
 
     // The chunk sequence runs column after column (NCH chunks each); chunk q lives in buffer
@@ -316,61 +306,76 @@ __global__ __launch_bounds__(256, 1) void k_ttm_m1(DtArgs a) {
     for (int rb = 0; rb < NRB; ++rb)
 #pragma unroll
         for (int c = 0; c < 2; ++c)
-            *reinterpret_cast<dt_f32x4 *>(a.partial + ((long)split * a.I1 + i1w + 16 * c + lm) * a.Rp + 16 * rb +
-                                          4 * g) = m1[rb][c];
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                a.partial[((long)split * a.I1 + i1w + 16 * c + 4 * g + v) * a.Rp + 16 * rb + lm] = m1[rb][c][v];
 }
 
-// M2 partials: lane (i2 = GPW i2 per wave, r = 4 q .. 4 q + 3), i1 in [split i1per, ...);
-// blk is the block's index among the pass's blocks
-template <int NQ>
-__device__ __forceinline__ void m2_block(const float *__restrict__ T, const float *__restrict__ panelB,
-                                         float *__restrict__ partial, long I1, long I2, int Rp, int i1per, int nb2,
+// M2 partials: wave w of block b takes i2 = 4 b + w and the T tiles [split tper, ...) of that
+// slab; lane (lm, g) holds, per tile and row block rb, the BF16 values of i1 = 32 tile + 16 c +
+// 4 g + v (c = 0, 1; v = 0..3) and r = 16 rb + lm (one 16-byte load, the layout the pass over X
+// stores) and multiplies them by B_new[i1][r] (one 16-byte panel load per c: four consecutive
+// rows of one column); the four lane groups g are added by lane exchanges at the end.
+// blk is the block's index among the pass's blocks.
+template <int NRB>
+__device__ __forceinline__ void m2_block(const uint16_t *__restrict__ T, const float *__restrict__ panelB,
+                                         float *__restrict__ partial, long I1, long I2, int Rp, int tper, int nb2,
                                          int blk) {
-    constexpr int GPW = 64 / NQ;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int q = lane % NQ, gi = lane / NQ;
+    const int tid = threadIdx.x, lane = tid & 63, lm = lane & 15, g = lane >> 4, wave = tid >> 6;
     const int split = blk / nb2, b = blk % nb2;
-    const long i2 = ((long)b * 4 + wave) * GPW + gi;
-    if (i2 >= I2) return;  // no barriers in this kernel
-    const long i1s = (long)split * i1per;
-    const long i1e = (i1s + i1per < I1) ? i1s + i1per : I1;  // multiples of 8
-    const float *tr = T + (i2 * I1) * Rp + 4 * q;
-    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    const long i2 = 4L * b + wave;
+    if (i2 >= I2) return;  // no barriers in this function
+    const long ntile = I1 / 32;
+    const long t0 = (long)split * tper, t1 = (t0 + tper < ntile) ? t0 + tper : ntile;
+    const unsigned char *tb = reinterpret_cast<const unsigned char *>(T) + i2 * ntile * 64L * Rp + 16 * lane;
+    float acc[NRB];
+#pragma unroll
+    for (int rb = 0; rb < NRB; ++rb) acc[rb] = 0.0f;
 #pragma unroll 2
-    for (long i1 = i1s; i1 < i1e; i1 += 8) {
-        float4 t[8], bb[4][2];
+    for (long tw = t0; tw < t1; ++tw) {
+        dt_u32x4 w[NRB];
+        float4 bb[NRB][2];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) t[j] = *reinterpret_cast<const float4 *>(tr + (i1 + j) * Rp);
+        for (int rb = 0; rb < NRB; ++rb)
+            w[rb] = __builtin_nontemporal_load(reinterpret_cast<const dt_u32x4 *>(tb + tw * 64L * Rp + 1024 * rb));
 #pragma unroll
-        for (int rr = 0; rr < 4; ++rr) {
-            const float4 *pb = reinterpret_cast<const float4 *>(panelB + (((i1 >> 3) * Rp + 4 * q + rr) << 3));
-            bb[rr][0] = pb[0];
-            bb[rr][1] = pb[1];
-        }
+        for (int rb = 0; rb < NRB; ++rb)
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const float4 &x = t[j];
-            const float4 &b0 = bb[0][j >> 2], &b1 = bb[1][j >> 2], &b2 = bb[2][j >> 2], &b3 = bb[3][j >> 2];
-            const int jj = j & 3;
-            const float y0 = jj == 0 ? b0.x : jj == 1 ? b0.y : jj == 2 ? b0.z : b0.w;
-            const float y1 = jj == 0 ? b1.x : jj == 1 ? b1.y : jj == 2 ? b1.z : b1.w;
-            const float y2 = jj == 0 ? b2.x : jj == 1 ? b2.y : jj == 2 ? b2.z : b2.w;
-            const float y3 = jj == 0 ? b3.x : jj == 1 ? b3.y : jj == 2 ? b3.z : b3.w;
-            acc[0] = fmaf(x.x, y0, acc[0]);
-            acc[1] = fmaf(x.y, y1, acc[1]);
-            acc[2] = fmaf(x.z, y2, acc[2]);
-            acc[3] = fmaf(x.w, y3, acc[3]);
+            for (int c = 0; c < 2; ++c) {
+                const long i1b = 32 * tw + 16 * c + 4 * g;
+                bb[rb][c] = *reinterpret_cast<const float4 *>(panelB + (((i1b >> 3) * Rp + 16 * rb + lm) << 3) +
+                                                              4 * (g & 1));
+            }
+#pragma unroll
+        for (int rb = 0; rb < NRB; ++rb) {
+            const unsigned q[4] = {w[rb].x, w[rb].y, w[rb].z, w[rb].w};
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const float4 &y = bb[rb][c];
+                const unsigned p0 = q[2 * c], p1 = q[2 * c + 1];
+                acc[rb] = fmaf(__uint_as_float(p0 << 16), y.x, acc[rb]);
+                acc[rb] = fmaf(__uint_as_float(p0 & 0xffff0000u), y.y, acc[rb]);
+                acc[rb] = fmaf(__uint_as_float(p1 << 16), y.z, acc[rb]);
+                acc[rb] = fmaf(__uint_as_float(p1 & 0xffff0000u), y.w, acc[rb]);
+            }
         }
     }
-    *reinterpret_cast<float4 *>(partial + ((long)split * I2 + i2) * Rp + 4 * q) =
-        float4{acc[0], acc[1], acc[2], acc[3]};
+#pragma unroll
+    for (int rb = 0; rb < NRB; ++rb) {
+        acc[rb] += __shfl_xor(acc[rb], 16);
+        acc[rb] += __shfl_xor(acc[rb], 32);
+    }
+    if (g == 0) {
+#pragma unroll
+        for (int rb = 0; rb < NRB; ++rb) partial[((long)split * I2 + i2) * Rp + 16 * rb + lm] = acc[rb];
+    }
 }
 
-template <int NQ>
-__global__ __launch_bounds__(256) void k_m2(const float *__restrict__ T, const float *__restrict__ panelB,
-                                            float *__restrict__ partial, long I1, long I2, int Rp, int i1per,
+template <int NRB>
+__global__ __launch_bounds__(256) void k_m2(const uint16_t *__restrict__ T, const float *__restrict__ panelB,
+                                            float *__restrict__ partial, long I1, long I2, int Rp, int tper,
                                             int nb2) {
-    m2_block<NQ>(T, panelB, partial, I1, I2, Rp, i1per, nb2, blockIdx.x);
+    m2_block<NRB>(T, panelB, partial, I1, I2, Rp, tper, nb2, blockIdx.x);
 }
 
 // The pass over T with the mode-2 side work of the sweep in the same launch: blocks
@@ -379,15 +384,15 @@ __global__ __launch_bounds__(256) void k_m2(const float *__restrict__ T, const f
 // inverse (G0 * G1)^{-1} (as k_vinv); blocks ngb.. are k_m2's.  The Gram blocks come first in
 // dispatch order, so the inverse starts at once instead of waiting for a separate Gram
 // launch, a fork and a join; the solve after this kernel needs nothing else.
-template <int NQ, int NS>
-__global__ __launch_bounds__(256) void k_m2_vinv(const float *__restrict__ T, const float *__restrict__ panelB,
-                                                 float *__restrict__ partial, long I1, long I2, int Rp, int i1per,
+template <int NRB, int NS>
+__global__ __launch_bounds__(256) void k_m2_vinv(const uint16_t *__restrict__ T, const float *__restrict__ panelB,
+                                                 float *__restrict__ partial, long I1, long I2, int Rp, int tper,
                                                  int nb2, int ngb, const double *__restrict__ gram_part, int gnblk,
                                                  const double *G0, double *G1, int R, double *__restrict__ Vinv,
                                                  int *__restrict__ status, double *__restrict__ fb_ws,
                                                  int *__restrict__ done) {
     if ((int)blockIdx.x >= ngb) {
-        m2_block<NQ>(T, panelB, partial, I1, I2, Rp, i1per, nb2, (int)blockIdx.x - ngb);
+        m2_block<NRB>(T, panelB, partial, I1, I2, Rp, tper, nb2, (int)blockIdx.x - ngb);
         return;
     }
     __shared__ int last;
@@ -436,6 +441,7 @@ static int m1_by_nrb(const DtreePlan &d, const DtArgs *a, hipStream_t s) {
     switch (d.nrb) {
         case 1: return m1_by_nch<ST, 1>(d, a, s);
         case 2: return m1_by_nch<ST, 2>(d, a, s);
+        case 3: return m1_by_nch<ST, 3>(d, a, s);
         case 4: return m1_by_nch<ST, 4>(d, a, s);
         default: return record(TDX_ERR_UNSUPPORTED);
     }
@@ -455,7 +461,6 @@ int dtree_plan(DtreePlan &d, int order, const long *dims, int dtype, long R, boo
     d.R = (int)R;
     d.Rp = (int)(ceil_div(R, 16) * 16);
     d.nrb = d.Rp / 16;
-    if (d.Rp == 48) return TDX_OK;  // k_m2 maps 4 columns per lane onto 64 lanes: Rp / 4 | 64
     d.I0 = dims[0];
     d.I1 = dims[1];
     d.I2 = dims[2];
@@ -471,23 +476,22 @@ int dtree_plan(DtreePlan &d, int order, const long *dims, int dtype, long R, boo
     d.nsplit1 = (int)ceil_div(d.I2, (long)d.i2per);
     d.lds1 = dtree_lds_bytes(d.I0, d.nrb, d.i2per, d.Rp);
     if (d.lds1 > 160 * 1024) return TDX_OK;  // LDS per workgroup on gfx950
-    // k_m2: about 16 waves per CU over (I2 / GPW) i2 groups x nsplit2 i1 ranges
-    const long gpw = 64 / (d.Rp / 4);
-    const long groups = ceil_div(d.I2, gpw);
-    long n2 = ceil_div(16 * cus, groups);
-    if (n2 > d.I1 / 8) n2 = d.I1 / 8;
+    // k_m2: one wave per (i2, tile range), about 16 waves per CU: nsplit2 ranges of tper tiles
+    const long ntile = d.I1 / 32;
+    long n2 = ceil_div(16 * cus, d.I2);
+    if (n2 > ntile) n2 = ntile;
     if (n2 < 1) n2 = 1;
-    d.i1per = (int)(8 * ceil_div(d.I1 / 8, n2));
-    d.nsplit2 = (int)ceil_div(d.I1, (long)d.i1per);
-    d.nb2 = (int)ceil_div(d.I2, 4 * gpw);
+    d.tper = (int)ceil_div(ntile, n2);
+    d.nsplit2 = (int)ceil_div(ntile, (long)d.tper);
+    d.nb2 = (int)ceil_div(d.I2, 4L);
     const int rc = m1_call(d, nullptr, nullptr);
     if (rc) return rc;
     d.on = true;
     return TDX_OK;
 }
 
-int dtree_m1_launch(const DtreePlan &d, const void *X, const float *panelA, const float *C, float *T, float *partial,
-                    hipStream_t s) {
+int dtree_m1_launch(const DtreePlan &d, const void *X, const float *panelA, const float *C, uint16_t *T,
+                    float *partial, hipStream_t s) {
     if (!d.on) return record(TDX_ERR_UNSUPPORTED);
     if ((reinterpret_cast<uintptr_t>(X) & 15) != 0) return record(TDX_ERR_ARG);
     DtArgs a;
@@ -509,32 +513,35 @@ int dtree_m1_launch(const DtreePlan &d, const void *X, const float *panelA, cons
     return TDX_OK;
 }
 
-int dtree_m2_vinv_launch(const DtreePlan &d, const float *T, const float *panelB, float *partial,
+int dtree_m2_vinv_launch(const DtreePlan &d, const uint16_t *T, const float *panelB, float *partial,
                          const double *gram_part, int gnblk, const double *G0, double *G1, int R, double *Vinv,
                          int *status, double *fb_ws, int *done, hipStream_t s) {
     if (!d.on || R <= 16 || R > kMaxR) return record(TDX_ERR_UNSUPPORTED);
     const int ngb = (int)ceil_div((long)R * R, 256L);
     const int grid = ngb + d.nb2 * d.nsplit2;
-    const int nq = d.Rp / 4;
-#define TDX_M2V(NQ_, NS_)                                                                                         \
-    k_m2_vinv<NQ_, NS_><<<grid, 256, 0, s>>>(T, panelB, partial, d.I1, d.I2, d.Rp, d.i1per, d.nb2, ngb, gram_part, \
-                                             gnblk, G0, G1, R, Vinv, status, fb_ws, done)
-    // 16 < R <= 32: Rp 32; 32 < R <= 64 with the shared pass on: Rp 64 (Rp 48 turns it off)
-    if (nq == 8 && R <= 32) TDX_M2V(8, 8);
-    else if (nq == 16) TDX_M2V(16, 16);
+#define TDX_M2V(NRB_, NS_)                                                                                         \
+    k_m2_vinv<NRB_, NS_><<<grid, 256, 0, s>>>(T, panelB, partial, d.I1, d.I2, d.Rp, d.tper, d.nb2, ngb, gram_part, \
+                                              gnblk, G0, G1, R, Vinv, status, fb_ws, done)
+    // 16 < R <= 32: Rp 32 (NRB 2); 32 < R <= 64: Rp 48 or 64
+    if (d.nrb == 2) TDX_M2V(2, 8);
+    else if (d.nrb == 3) TDX_M2V(3, 16);
+    else if (d.nrb == 4) TDX_M2V(4, 16);
     else return record(TDX_ERR_UNSUPPORTED);
 #undef TDX_M2V
     TDX_LAUNCH_CHECK();
     return TDX_OK;
 }
 
-int dtree_m2_launch(const DtreePlan &d, const float *T, const float *panelB, float *partial, hipStream_t s) {
+int dtree_m2_launch(const DtreePlan &d, const uint16_t *T, const float *panelB, float *partial, hipStream_t s) {
     if (!d.on) return record(TDX_ERR_UNSUPPORTED);
     const int grid = d.nb2 * d.nsplit2;
-    const int nq = d.Rp / 4;
-    if (nq == 16) k_m2<16><<<grid, 256, 0, s>>>(T, panelB, partial, d.I1, d.I2, d.Rp, d.i1per, d.nb2);
-    else if (nq == 8) k_m2<8><<<grid, 256, 0, s>>>(T, panelB, partial, d.I1, d.I2, d.Rp, d.i1per, d.nb2);
-    else k_m2<4><<<grid, 256, 0, s>>>(T, panelB, partial, d.I1, d.I2, d.Rp, d.i1per, d.nb2);
+    switch (d.nrb) {
+        case 1: k_m2<1><<<grid, 256, 0, s>>>(T, panelB, partial, d.I1, d.I2, d.Rp, d.tper, d.nb2); break;
+        case 2: k_m2<2><<<grid, 256, 0, s>>>(T, panelB, partial, d.I1, d.I2, d.Rp, d.tper, d.nb2); break;
+        case 3: k_m2<3><<<grid, 256, 0, s>>>(T, panelB, partial, d.I1, d.I2, d.Rp, d.tper, d.nb2); break;
+        case 4: k_m2<4><<<grid, 256, 0, s>>>(T, panelB, partial, d.I1, d.I2, d.Rp, d.tper, d.nb2); break;
+        default: return record(TDX_ERR_UNSUPPORTED);
+    }
     TDX_LAUNCH_CHECK();
     return TDX_OK;
 }
@@ -566,11 +573,11 @@ int tdx_mttkrp_dtree(const void *X, int dtype, const float *scales, const long *
         panel[m] = (float *)ws;
     }
     void *T = nullptr, *part = nullptr;
-    st = workspace(WS_MISC0, sizeof(float) * d.t_floats(), &T);
+    st = workspace(WS_MISC0, d.t_bytes(), &T);
     if (st) return st;
     st = workspace(WS_PARTIAL, sizeof(float) * d.partial_floats(), &part);
     if (st) return st;
-    st = dtree_m1_launch(d, X, panel[0], factors[2], (float *)T, (float *)part, s);
+    st = dtree_m1_launch(d, X, panel[0], factors[2], (uint16_t *)T, (float *)part, s);
     if (st) return st;
     MttkrpPlan p1;
     p1.order = 3;
@@ -582,7 +589,7 @@ int tdx_mttkrp_dtree(const void *X, int dtype, const float *scales, const long *
     p1.nsplit = d.nsplit1;
     st = mttkrp_reduce(p1, (const float *)part, M1, s);
     if (st) return record(st);
-    st = dtree_m2_launch(d, (const float *)T, panel[1], (float *)part, s);
+    st = dtree_m2_launch(d, (const uint16_t *)T, panel[1], (float *)part, s);
     if (st) return st;
     MttkrpPlan p2 = p1;
     p2.mode = 2;
@@ -616,7 +623,7 @@ int tdx_bench_mttkrp_dtree(const void *X, int dtype, const float *scales, const 
         panel[m] = (float *)ws;
     }
     void *T = nullptr, *part = nullptr;
-    st = workspace(WS_MISC0, sizeof(float) * d.t_floats(), &T);
+    st = workspace(WS_MISC0, d.t_bytes(), &T);
     if (st) return st;
     st = workspace(WS_PARTIAL, sizeof(float) * d.partial_floats(), &part);
     if (st) return st;
@@ -625,12 +632,13 @@ int tdx_bench_mttkrp_dtree(const void *X, int dtype, const float *scales, const 
     };
     hipEvent_t e[3];
     for (int i = 0; i < 3; ++i) TDX_HIP(hipEventCreate(&e[i]));
-    st = dtree_m1_launch(d, X, panel[0], factors[2], (float *)T, (float *)part, s);
-    if (!st) st = dtree_m2_launch(d, (const float *)T, panel[1], (float *)part, s);
+    st = dtree_m1_launch(d, X, panel[0], factors[2], (uint16_t *)T, (float *)part, s);
+    if (!st) st = dtree_m2_launch(d, (const uint16_t *)T, panel[1], (float *)part, s);
     if (!st) st = chk(hipEventRecord(e[0], s), "hipEventRecord");
-    for (int i = 0; i < iters && !st; ++i) st = dtree_m1_launch(d, X, panel[0], factors[2], (float *)T, (float *)part, s);
+    for (int i = 0; i < iters && !st; ++i)
+        st = dtree_m1_launch(d, X, panel[0], factors[2], (uint16_t *)T, (float *)part, s);
     if (!st) st = chk(hipEventRecord(e[1], s), "hipEventRecord");
-    for (int i = 0; i < iters && !st; ++i) st = dtree_m2_launch(d, (const float *)T, panel[1], (float *)part, s);
+    for (int i = 0; i < iters && !st; ++i) st = dtree_m2_launch(d, (const uint16_t *)T, panel[1], (float *)part, s);
     if (!st) st = chk(hipEventRecord(e[2], s), "hipEventRecord");
     if (!st) st = chk(hipEventSynchronize(e[2]), "hipEventSynchronize");
     float t1 = 0.f, t2 = 0.f;

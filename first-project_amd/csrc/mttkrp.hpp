@@ -52,29 +52,38 @@ int mttkrp_lds_launch(const MttkrpPlan &p, const void *X, const float *const *pa
 // partials in the same pass over X, then the mode-2 partials from T
 struct DtreePlan {
     bool on = false;  // eligible (order 3, BF16/FP8 storage, I0 = 128 nch (BF16) / 256 nch (FP8)
-                      // with nch in {2, 4, 8, 16}, I1 % 32 == 0, Rp in {16, 32, 64}, A operands
-                      // fit in LDS)
+                      // with nch in {2, 4, 8, 16}, I1 % 32 == 0, R <= 64, A operands fit in LDS)
     int st = 0, nrb = 0, nch = 0, Rp = 0, R = 0;
     long I0 = 0, I1 = 0, I2 = 0;
     int blocks_i1 = 0, nsplit1 = 0, i2per = 0;  // k_ttm_m1 grid: blocks_i1 x nsplit1
     size_t lds1 = 0;
-    int nsplit2 = 0, i1per = 0, nb2 = 0;        // k_m2 grid: nb2 x nsplit2
-    size_t t_floats() const { return (size_t)I1 * I2 * Rp; }
+    int nsplit2 = 0, tper = 0, nb2 = 0;         // k_m2 grid: nb2 x nsplit2, tper T tiles per split
+    // T: BF16, I2 x (I1 / 32) tiles of 32 i1 x Rp, in the register order of the pass over X:
+    // element (i1, r) of slab i2 at dt_t_index
+    size_t t_elems() const { return (size_t)I1 * I2 * Rp; }
+    size_t t_bytes() const { return 2 * t_elems(); }
     size_t partial_floats() const {
         const size_t a = (size_t)nsplit1 * I1 * Rp, b = (size_t)nsplit2 * I2 * Rp;
         return a > b ? a : b;
     }
 };
+// Element (i1, r) of slab i2 of T: tile (i2, i1 / 32) holds 32 Rp values; in it row block
+// rb = r / 16 is 1 KB, lane l = r % 16 + 16 ((i1 % 16) / 4) of the pass over X holds 16 bytes
+// of it, ordered c = (i1 % 32) / 16, then v = i1 % 4 (the MFMA output order of that pass).
+__host__ __device__ inline long dt_t_index(long i1, long i2, int r, long I1, int Rp) {
+    return (i2 * (I1 / 32) + i1 / 32) * 32L * Rp + (r / 16) * 512L + ((r % 16) + 16 * ((i1 % 16) / 4)) * 8L +
+           ((i1 % 32) / 16) * 4 + i1 % 4;
+}
 // fills d (d.on = false when not enabled or when the shape or storage is not covered)
 int dtree_plan(DtreePlan &d, int order, const long *dims, int dtype, long R, bool enabled);
 // partial[split][i1][Rp] (nsplit1 splits, rows_pad I1) and T; C column-major I2 x R
-int dtree_m1_launch(const DtreePlan &d, const void *X, const float *panelA, const float *C, float *T, float *partial,
-                    hipStream_t s);
+int dtree_m1_launch(const DtreePlan &d, const void *X, const float *panelA, const float *C, uint16_t *T,
+                    float *partial, hipStream_t s);
 // partial[split][i2][Rp] (nsplit2 splits, rows_pad I2) from T and the mode-1 panel
-int dtree_m2_launch(const DtreePlan &d, const float *T, const float *panelB, float *partial, hipStream_t s);
+int dtree_m2_launch(const DtreePlan &d, const uint16_t *T, const float *panelB, float *partial, hipStream_t s);
 // the pass over T with mode 1's Gram sum (gram_part, gnblk blocks -> G1) and the mode-2 inverse
 // (G0 * G1)^{-1} -> Vinv in the same launch (16 < R <= 64); done: a zeroed ticket
-int dtree_m2_vinv_launch(const DtreePlan &d, const float *T, const float *panelB, float *partial,
+int dtree_m2_vinv_launch(const DtreePlan &d, const uint16_t *T, const float *panelB, float *partial,
                          const double *gram_part, int gnblk, const double *G0, double *G1, int R, double *Vinv,
                          int *status, double *fb_ws, int *done, hipStream_t s);
 

@@ -118,7 +118,7 @@ __device__ inline int sweep_inverse_wave16(VAt &vat, double *Vinv, int R) {
 // The inverse for one workgroup of 256 threads (the body of k_vinv).  G: the current Grams
 // (G[mode] unused).
 template <int NS>
-__device__ void vinv_block(const double *const *G, int order, int mode, int R, double *Vinv, int *status,
+__device__ __forceinline__ void vinv_block(const double *const *G, int order, int mode, int R, double *Vinv, int *status,
                            double *fb_ws) {
     __shared__ double Wb[128];
     __shared__ double dg[kMaxR];

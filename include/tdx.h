@@ -127,10 +127,10 @@ int tdx_bench_mttkrp(const void *X, int dtype, const float *scales, int order, c
 /* new, no counterpart in the reference (which runs one MTTKRP per mode, src/cp_als.cu:52,69,88):
  * the shared pass of an order-3 sweep.  M1 = X(1) (C (.) A) (I1 x rank) and M2 = X(2) (B (.) A)
  * (I2 x rank), both column-major like tdx_mttkrp_ex modes 1 and 2, from one pass over X that
- * writes T = X x_0 A^T and a pass over T.  BF16 or FP8-E4M3 storage (scales as for
- * tdx_mttkrp_ex); requires I0 = 128 k (BF16) or 256 k (FP8) with k in {2, 4, 8, 16}, I1 % 32 == 0,
- * rank <= 64
- * with rank rounded up to 16 in {16, 32, 64}; otherwise TDX_ERR_UNSUPPORTED. */
+ * writes T = X x_0 A^T (stored in BF16, rounded to nearest even: M2 carries that rounding) and a
+ * pass over T.  BF16 or FP8-E4M3 storage (scales as for tdx_mttkrp_ex); requires I0 = 128 k
+ * (BF16) or 256 k (FP8) with k in {2, 4, 8, 16}, I1 % 32 == 0 and rank <= 64; otherwise
+ * TDX_ERR_UNSUPPORTED. */
 int tdx_mttkrp_dtree(const void *X, int dtype, const float *scales, const long *dims,
                      const float *const *factors, long rank, float *M1, float *M2);
 /* new: mean launch duration (ms, hipEvents on the library stream, `iters` launches each) of the

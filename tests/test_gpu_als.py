@@ -168,9 +168,13 @@ def test_mixed_sweeps_reach_oracle_with_and_without_shared_pass(storage, dims, s
 
 def test_shared_pass_fast_fit_is_the_iterates_error():
     """BF16 storage with the shared pass: A is BF16-exact, so the normal equations, the Grams
-    and the pass over X all describe the same iterate and the per-sweep fast-fit identity
+    and the pass over X all describe the same iterate, and the per-sweep fast-fit identity
     (FP64) matches the direct relative error of the returned factors against the FP32 tensor
-    (FP64 oracle) to 5e-5, at a transient and on the plateau."""
+    (FP64 oracle) to 5e-5 plus the effect of storing T in BF16 on <M2, C>, at a transient and
+    on the plateau.  Rounding to nearest leaves each T entry off by a fraction of itself within
+    +-2^-9, close to uniform and independent over entries, so the fit moves by about
+    2^-9 / sqrt(3) sqrt(sum (T B C)^2) / (||X||^2 relerr) (RMS; T = X x_0 A^T of the returned
+    A); the test allows six times that."""
     dims, R = (512, 64, 48), 16
     X, _ = make_cp_tensor(dims, R, eta=1e-3, kind="normal")
     U0 = init_factors(dims, R, seed=5)
@@ -182,15 +186,20 @@ def test_shared_pass_fast_fit_is_the_iterates_error():
         st, rep, fit, ms = tdx.cp_als_ex(X, dims, R, Ud, o)
         U = [host(u).reshape((d, R), order="F").astype(np.float64) for u, d in zip(Ud, dims)]
         true = O.cp_relerr(Xh, dims, U)
-        assert abs(fit[-1] - true) <= 5e-5, (k, fit[-1], true)
+        X3 = Xh.reshape(dims, order="F").astype(np.float64)
+        T = np.einsum("abc,ar->bcr", X3, U[0])
+        rms = 2.0 ** -9 / np.sqrt(3.0) * np.sqrt(np.einsum("bcr,br,cr->", T * T, U[1] * U[1], U[2] * U[2]))
+        bound = 6.0 * rms / (np.sum(X3 * X3) * true)
+        assert bound <= 1e-3, bound
+        assert abs(fit[-1] - true) <= 5e-5 + bound, (k, fit[-1], true, bound)
         # the factor A holds BF16 values only
         a = U[0].astype(np.float32).ravel()
         assert np.array_equal(a, (O.bf16_bits(a).astype(np.uint32) << 16).view(np.float32))
 
 
-@pytest.mark.parametrize("R", [24, 56])
+@pytest.mark.parametrize("R", [24, 40, 56])
 def test_shared_pass_mode2_inverse_and_gram(R):
-    """Ranks 17..64 with the shared pass: mode 1's Gram is summed and mode 2's inverse formed
+    """Ranks 17..64 (padded to 32, 48, 64) with the shared pass: mode 1's Gram is summed and mode 2's inverse formed
     inside the pass over T.  The per-sweep fast fit (which uses that Gram) equals the direct
     relative error of the returned factors (FP64 oracle) to 5e-5, and the sweeps follow the ones
     without the shared pass (a fused MTTKRP per mode, separate inverse) to within BF16 rounding."""

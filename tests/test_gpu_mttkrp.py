@@ -178,8 +178,10 @@ def run_dtree(Xdev, dtype, scales, dims, U):
 
 @pytest.mark.parametrize("dims,R", [((256, 64, 40), 16), ((256, 160, 9), 64), ((512, 32, 70), 24)])
 def test_shared_pass_integer_data_bit_exact(dims, R):
-    """Small integers: BF16 storage and BF16 A operands are exact, every partial sum is an
-    integer below 2^24, so M1 and M2 equal the oracle's MTTKRPs bit for bit."""
+    """Small integers: BF16 storage and BF16 A operands are exact and every partial sum is an
+    integer below 2^24, so M1 equals the oracle's MTTKRP bit for bit, and M2 equals, bit for
+    bit, the contraction of T = X x_0 A^T rounded to BF16 (to nearest even: T is stored in BF16)
+    with B; on these inputs T exceeds 256 in magnitude, so the rounding is exercised."""
     import torch
     rng = np.random.default_rng(sum(dims) + 3 * R)
     X = rng.integers(-3, 4, size=int(np.prod(dims))).astype(np.float32)
@@ -189,14 +191,22 @@ def test_shared_pass_integer_data_bit_exact(dims, R):
     M1, M2 = run_dtree(q, tdx.TDX_BF16, None, dims, U)
     U64 = [u.astype(np.float64) for u in U]
     assert np.array_equal(M1, O.mttkrp(X, dims, U64, 1).astype(np.float32))
-    assert np.array_equal(M2, O.mttkrp(X, dims, U64, 2).astype(np.float32))
+    X3 = X.reshape(dims, order="F").astype(np.float64)
+    T = np.einsum("abc,ar->bcr", X3, U64[0])  # exact integers below 2^24
+    assert np.max(np.abs(T)) > 256
+    Tb = (O.bf16_bits(T.astype(np.float32)).astype(np.uint32) << 16).view(np.float32).reshape(T.shape)
+    M2_ref = np.einsum("bcr,br->cr", Tb.astype(np.float64), U64[1])
+    assert np.array_equal(M2, M2_ref.astype(np.float32))
+    # and within BF16 rounding of the unrounded MTTKRP
+    assert relnorm(M2, O.mttkrp(X, dims, U64, 2)) <= 3e-3
 
 
 @pytest.mark.parametrize("dims,R", [((256, 96, 48), 16), ((512, 64, 40), 32), ((256, 128, 24), 64), ((256, 32, 5), 10),
-                                    ((2048, 32, 6), 32)])
+                                    ((2048, 32, 6), 32), ((256, 64, 40), 48), ((512, 96, 17), 40)])
 def test_shared_pass_bf16_storage(dims, R):
-    """Within 3e-3 normwise of the oracle on the stored values (BF16 rounding of A), and
-    within the same bar of the fused per-mode kernel."""
+    """Within 3e-3 normwise of the oracle on the stored values (BF16 rounding of A and, for M2,
+    of T), and within 6e-3 of the fused per-mode kernel, which meets the same 3e-3 bar with its
+    own rounding (of the Khatri-Rao block): the two bars add."""
     import torch
     rng = np.random.default_rng(len(dims) * 17 + R)
     X = rng.standard_normal(int(np.prod(dims))).astype(np.float32)
@@ -209,7 +219,7 @@ def test_shared_pass_bf16_storage(dims, R):
     for mode, got in ((1, M1), (2, M2)):
         ref = O.mttkrp(Xb, dims, U64, mode)
         assert relnorm(got, ref) <= 3e-3, (dims, R, mode, relnorm(got, ref))
-        assert relnorm(got, run_mttkrp(q, tdx.TDX_BF16, None, dims, U, mode)) <= 3e-3
+        assert relnorm(got, run_mttkrp(q, tdx.TDX_BF16, None, dims, U, mode)) <= 6e-3
 
 
 @pytest.mark.parametrize("dims,R", [((512, 64, 40), 16), ((512, 96, 33), 64), ((1024, 32, 24), 32)])
@@ -232,10 +242,10 @@ def test_shared_pass_fp8_storage(dims, R):
 
 
 @pytest.mark.parametrize("dims,R,dtype", [((128, 64, 40), 16, 1), ((384, 64, 40), 16, 1), ((256, 48, 40), 16, 1),
-                                          ((256, 64, 40), 16, 2), ((256, 64, 40), 48, 1), ((256, 64, 40), 80, 1)])
+                                          ((256, 64, 40), 16, 2), ((256, 64, 40), 80, 1)])
 def test_shared_pass_rejects_uncovered_shapes(dims, R, dtype):
-    """I0 not 128 k (BF16) / 256 k (FP8) with k in {2, 4, 8, 16}, I1 not a multiple of 32,
-    padded rank 48 or rank > 64: TDX_ERR_UNSUPPORTED, nothing launched."""
+    """I0 not 128 k (BF16) / 256 k (FP8) with k in {2, 4, 8, 16}, I1 not a multiple of 32 or
+    rank > 64: TDX_ERR_UNSUPPORTED, nothing launched."""
     import torch
     dt = tdx.TDX_BF16 if dtype == 1 else tdx.TDX_FP8_E4M3
     n = int(np.prod(dims))
