@@ -13,7 +13,7 @@
 // the reference computes X(n) (KR product) per mode (src/cp_als.cu:46-52, 66-69, 85-88).
 //
 // k_ttm_m1 (BF16 or FP8 storage):
-//   * a workgroup (4 waves) owns 128 consecutive i1 and a range of i2; A_new (the mode-0
+//   * a workgroup (8 or 4 waves, dt_waves) owns 32 i1 per wave and a range of i2; A_new (the mode-0
 //     panel, pre-scaled by the FP8 slice scales) is converted once into BF16 MFMA operands
 //     in LDS (I0 Rp 2 bytes, 128 KB at I0 = 1024, Rp = 64), in the exact per-lane order the
 //     MFMA reads (one conflict-free 16-byte read per lane per operand);
@@ -25,8 +25,9 @@
 //     lines (8 columns), and one DPP lane exchange (lane ^ 8) turns the pair of loads into
 //     the operand order, where a load of 16 columns x 64 bytes each (half lines) streamed at
 //     ~4.3 TB/s against ~6.2-6.7 for whole lines (513 -> 460 us at 1024^3 BF16 R 64);
-//   * chunks of 4 (BF16) / 8 (FP8) k-steps (8 KB of X per wave) rotate through four
-//     register buffers across the i2 loop, three chunks in flight while one is multiplied
+//   * chunks of 4 (BF16) / 8 (FP8) k-steps (8 KB of X per wave) rotate through two (8 waves)
+//     or four (4 waves) register buffers across the i2 loop, the others in flight while one is
+//     multiplied
 //     (nontemporal loads; every load is issued unconditionally, the chunks past the end
 //     re-read a valid one), and after the last chunk of a column i2 the tile is multiplied
 //     into the mode-1 partial by C[i2][:] (from LDS, FP32) and stored to T after the next
@@ -103,10 +104,17 @@ inline size_t dtree_lds_bytes(long I0, int nrb, int i2per, int Rp) {
     return (size_t)(I0 / 32) * nrb * 1024 + (size_t)i2per * Rp * 4;
 }
 
+// waves per workgroup of the pass over X by row blocks: 8 waves (two per SIMD, two chunk
+// buffers, 190-212 VGPRs) sharing the A operands in LDS, or 4 waves with four buffers.
+// Measured at 1024^3 (same box): 8 waves 3% faster at Rp 64, 1% at Rp 16, equal at Rp 48,
+// 3% slower at Rp 32.
+__host__ __device__ constexpr int dt_waves(int nrb) { return (nrb == 2 || nrb == 3) ? 4 : 8; }
+
 template <int ST, int NRB, int NCH>
-__global__ __launch_bounds__(256, 1) void k_ttm_m1(DtArgs a) {
+__global__ __launch_bounds__(64 * dt_waves(NRB), 1) void k_ttm_m1(DtArgs a) {
+    constexpr int kDtWaves = dt_waves(NRB), kDtThreads = 64 * kDtWaves;
     constexpr int ES = DtTile<ST>::ES, KPL = DtTile<ST>::KPL, CH = DtTile<ST>::CH, RUNS = CH / KPL;
-    constexpr int NB = NCH < 4 ? NCH : 4;  // chunk buffers; NCH % NB == 0
+    constexpr int NB = kDtWaves == 8 ? 2 : (NCH < 4 ? NCH : 4);  // chunk buffers; NCH % NB == 0
     static_assert(NCH >= 2 && NCH % NB == 0, "NCH: chunks per column");
     extern __shared__ __align__(16) unsigned char dsm[];
     uint4 *aop = reinterpret_cast<uint4 *>(dsm);  // (I0/32) * NRB * 64 operands of 16 bytes
@@ -120,15 +128,15 @@ __global__ __launch_bounds__(256, 1) void k_ttm_m1(DtArgs a) {
     const int len = (int)(i2e - i2s);  // >= 1: the grid has ceil(I2 / i2per) splits
     // A operand (k-step ks, row block rb, lane l): rows r = 16 rb + l % 16, the 8 i0 of lane
     // group l / 16 within the run ks / KPL, step ks % KPL
-    // (nent is a multiple of 256; 4 entries per thread in flight, the ones past nent repeat
+    // (4 entries per thread in flight, the ones past nent repeat
     // the last entry's loads and are not stored)
     const long nent = nks * NRB * 64;
-    for (long e0 = tid; e0 < nent; e0 += 4 * 256) {
+    for (long e0 = tid; e0 < nent; e0 += 4 * kDtThreads) {
         float4 p[4][2];
         long dst[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const long e = (e0 + 256L * u < nent) ? e0 + 256L * u : nent - 1;
+            const long e = (e0 + (long)kDtThreads * u < nent) ? e0 + (long)kDtThreads * u : nent - 1;
             const int l = (int)(e & 63);
             const long t = e >> 6;
             const int rb = (int)(t % NRB);
@@ -142,17 +150,17 @@ __global__ __launch_bounds__(256, 1) void k_ttm_m1(DtArgs a) {
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-            if (e0 + 256L * u < nent)
+            if (e0 + (long)kDtThreads * u < nent)
                 aop[dst[u]] = uint4{dt_pack(p[u][0].x, p[u][0].y), dt_pack(p[u][0].z, p[u][0].w),
                                     dt_pack(p[u][1].x, p[u][1].y), dt_pack(p[u][1].z, p[u][1].w)};
     }
     // C block: consecutive threads read consecutive i2 of one column (coalesced)
-    for (int e = tid; e < len * a.Rp; e += 256) {
+    for (int e = tid; e < len * a.Rp; e += kDtThreads) {
         const int r = e / len, il = e % len;
         cs[il * a.Rp + r] = r < a.R ? a.C[(i2s + il) + (long)r * a.I2] : 0.0f;
     }
     __syncthreads();  // the only barrier
-    const long i1w = (long)bi * 128 + 32 * wave;  // first column of this wave's 32
+    const long i1w = (long)bi * (32 * kDtWaves) + 32 * wave;  // first column of this wave's 32
     if (i1w >= a.I1) return;                      // I1 % 32 == 0: whole tiles only
 
     const unsigned char *X = static_cast<const unsigned char *>(a.X);
@@ -259,7 +267,7 @@ __global__ __launch_bounds__(256, 1) void k_ttm_m1(DtArgs a) {
     // every column).  Each wave walks a column's chunks from its own rotation (the order of
     // the sum over i0 is free).  Chunks past the end re-read the last column's first chunk
     // (a valid address, never multiplied).
-    const int rot = (int)((blockIdx.x * 4u + (unsigned)wave) & (NCH - 1));
+    const int rot = (int)((blockIdx.x * (unsigned)kDtWaves + (unsigned)wave) & (NCH - 1));
     long qi = 0, i2_i = 0;
     int kc_i = 0;
     auto next_off = [&]() {
@@ -421,7 +429,7 @@ static int m1_kernel(const DtreePlan &d, const DtArgs *a, hipStream_t s) {
                                     (int)d.lds1));
         return TDX_OK;
     }
-    k_ttm_m1<ST, NRB, NCH><<<d.blocks_i1 * d.nsplit1, 256, d.lds1, s>>>(*a);
+    k_ttm_m1<ST, NRB, NCH><<<d.blocks_i1 * d.nsplit1, 64 * dt_waves(NRB), d.lds1, s>>>(*a);
     return TDX_OK;
 }
 
@@ -468,7 +476,7 @@ int dtree_plan(DtreePlan &d, int order, const long *dims, int dtype, long R, boo
     d.nch = (int)(d.I0 / (32 * CH));  // chunks per column: 2, 4, 8 or 16
     if (d.nch != 2 && d.nch != 4 && d.nch != 8 && d.nch != 16) return TDX_OK;
     const long cus = device_cus();
-    d.blocks_i1 = (int)ceil_div(d.I1, 128);
+    d.blocks_i1 = (int)ceil_div(d.I1, 32L * dt_waves(d.nrb));
     long ns = cus / d.blocks_i1;
     if (ns < 1) ns = 1;
     if (ns > d.I2) ns = d.I2;
