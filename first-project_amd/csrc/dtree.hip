@@ -6,7 +6,7 @@
 //   M1[i1][r]    = sum_{i2} T[i2][i1][r] C[i2][r]          (C is current during mode 1)
 //   M2[i2][r]    = sum_{i1} T[i2][i1][r] B_new[i1][r]
 // so one pass over X (k_ttm_m1: T and the mode-1 partials in the same pass) and one pass over
-// T (k_m2, I1 I2 Rp floats instead of I0 I1 I2 elements) replace two passes over X.  This is
+// T (k_m2, I1 I2 Rp BF16 values instead of I0 I1 I2 elements) replace two passes over X.  This is
 // the dimension-tree MTTKRP of Phan, Tichavsky and Cichocki ("Fast alternating LS
 // algorithms for high order CANDECOMP/PARAFAC tensor factorizations", IEEE TSP 61(19),
 // 2013), restricted to order 3.  The arithmetic is that of the fused MTTKRP reassociated:
