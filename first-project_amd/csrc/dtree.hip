@@ -38,10 +38,12 @@
 //     traffic, half the issue of the stores;
 //   * the mode-1 partial of the workgroup's i2 range goes to partial[split][i1][Rp], the
 //     layout k_solve sums (in split order, FP64).
-// k_m2: a wave per (i2, range of tiles); per tile a lane reads one 16-byte piece of T per row
-//   block (8 BF16: four consecutive i1 of two 16-column halves, one r) and the matching rows
-//   of B_new (one 16-byte panel load per half), FP32 FMAs, the four lane groups added at the
-//   end; partial[split][i2][Rp].
+// k_m2: a wave per (kM2Slabs slabs i2, range of tiles); per tile a lane reads one 16-byte piece
+//   of T per slab and row block (8 BF16: four consecutive i1 of two 16-column halves, one r)
+//   and the matching rows of B_new once for the wave's slabs (one 16-byte panel load per
+//   half), FP32 FMAs, the four lane groups added at the end; partial[split][i2][Rp].  With one
+//   slab per wave two of every three load instructions were panel loads: 41.7 us at 1024^3
+//   R64 against 26.7 with two slabs (four slabs: the same time, 130 VGPRs in k_m2_vinv).
 #include "mttkrp.hpp"
 #include "vinv.hpp"
 
@@ -319,33 +321,47 @@ __global__ __launch_bounds__(64 * dt_waves(NRB), 1) void k_ttm_m1(DtArgs a) {
                 a.partial[((long)split * a.I1 + i1w + 16 * c + 4 * g + v) * a.Rp + 16 * rb + lm] = m1[rb][c][v];
 }
 
-// M2 partials: wave w of block b takes i2 = 4 b + w and the T tiles [split tper, ...) of that
-// slab; lane (lm, g) holds, per tile and row block rb, the BF16 values of i1 = 32 tile + 16 c +
-// 4 g + v (c = 0, 1; v = 0..3) and r = 16 rb + lm (one 16-byte load, the layout the pass over X
-// stores) and multiplies them by B_new[i1][r] (one 16-byte panel load per c: four consecutive
-// rows of one column); the four lane groups g are added by lane exchanges at the end.
-// blk is the block's index among the pass's blocks.
+// slabs of T per wave in the pass over T: each panel load serves that many slabs
+static constexpr int kM2Slabs = 2;
+
+// M2 partials: wave w of block b takes the kM2Slabs slabs i2 = kM2Slabs (4 b + w) + j and the T
+// tiles [split tper, ...) of each; lane (lm, g) holds, per slab, tile and row block rb, the BF16
+// values of i1 = 32 tile + 16 c + 4 g + v (c = 0, 1; v = 0..3) and r = 16 rb + lm (one 16-byte
+// load, the layout the pass over X stores) and multiplies them by B_new[i1][r] (one 16-byte
+// panel load per c, four consecutive rows of one column, shared by the wave's slabs); the four
+// lane groups g are added by lane exchanges at the end.  Slabs past I2 re-read slab I2 - 1 and
+// store nothing.  blk is the block's index among the pass's blocks.
 template <int NRB>
 __device__ __forceinline__ void m2_block(const uint16_t *__restrict__ T, const float *__restrict__ panelB,
                                          float *__restrict__ partial, long I1, long I2, int Rp, int tper, int nb2,
                                          int blk) {
+    constexpr int J = kM2Slabs;
     const int tid = threadIdx.x, lane = tid & 63, lm = lane & 15, g = lane >> 4, wave = tid >> 6;
     const int split = blk / nb2, b = blk % nb2;
-    const long i2 = 4L * b + wave;
-    if (i2 >= I2) return;  // no barriers in this function
+    const long i2b = (long)J * (4L * b + wave);
+    if (i2b >= I2) return;  // no barriers in this function
     const long ntile = I1 / 32;
     const long t0 = (long)split * tper, t1 = (t0 + tper < ntile) ? t0 + tper : ntile;
-    const unsigned char *tb = reinterpret_cast<const unsigned char *>(T) + i2 * ntile * 64L * Rp + 16 * lane;
-    float acc[NRB];
+    const unsigned char *tb[J];
 #pragma unroll
-    for (int rb = 0; rb < NRB; ++rb) acc[rb] = 0.0f;
-#pragma unroll 2
+    for (int j = 0; j < J; ++j) {
+        const long i2 = (i2b + j < I2) ? i2b + j : I2 - 1;
+        tb[j] = reinterpret_cast<const unsigned char *>(T) + i2 * ntile * 64L * Rp + 16 * lane;
+    }
+    float acc[J][NRB];
+#pragma unroll
+    for (int j = 0; j < J; ++j)
+#pragma unroll
+        for (int rb = 0; rb < NRB; ++rb) acc[j][rb] = 0.0f;
     for (long tw = t0; tw < t1; ++tw) {
-        dt_u32x4 w[NRB];
+        dt_u32x4 w[J][NRB];
         float4 bb[NRB][2];
 #pragma unroll
-        for (int rb = 0; rb < NRB; ++rb)
-            w[rb] = __builtin_nontemporal_load(reinterpret_cast<const dt_u32x4 *>(tb + tw * 64L * Rp + 1024 * rb));
+        for (int j = 0; j < J; ++j)
+#pragma unroll
+            for (int rb = 0; rb < NRB; ++rb)
+                w[j][rb] = __builtin_nontemporal_load(
+                    reinterpret_cast<const dt_u32x4 *>(tb[j] + tw * 64L * Rp + 1024 * rb));
 #pragma unroll
         for (int rb = 0; rb < NRB; ++rb)
 #pragma unroll
@@ -355,27 +371,37 @@ __device__ __forceinline__ void m2_block(const uint16_t *__restrict__ T, const f
                                                               4 * (g & 1));
             }
 #pragma unroll
-        for (int rb = 0; rb < NRB; ++rb) {
-            const unsigned q[4] = {w[rb].x, w[rb].y, w[rb].z, w[rb].w};
+        for (int j = 0; j < J; ++j)
 #pragma unroll
-            for (int c = 0; c < 2; ++c) {
-                const float4 &y = bb[rb][c];
-                const unsigned p0 = q[2 * c], p1 = q[2 * c + 1];
-                acc[rb] = fmaf(__uint_as_float(p0 << 16), y.x, acc[rb]);
-                acc[rb] = fmaf(__uint_as_float(p0 & 0xffff0000u), y.y, acc[rb]);
-                acc[rb] = fmaf(__uint_as_float(p1 << 16), y.z, acc[rb]);
-                acc[rb] = fmaf(__uint_as_float(p1 & 0xffff0000u), y.w, acc[rb]);
+            for (int rb = 0; rb < NRB; ++rb) {
+                const unsigned q[4] = {w[j][rb].x, w[j][rb].y, w[j][rb].z, w[j][rb].w};
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    const float4 &y = bb[rb][c];
+                    const unsigned p0 = q[2 * c], p1 = q[2 * c + 1];
+                    float &s = acc[j][rb];
+                    s = fmaf(__uint_as_float(p0 << 16), y.x, s);
+                    s = fmaf(__uint_as_float(p0 & 0xffff0000u), y.y, s);
+                    s = fmaf(__uint_as_float(p1 << 16), y.z, s);
+                    s = fmaf(__uint_as_float(p1 & 0xffff0000u), y.w, s);
+                }
             }
-        }
     }
 #pragma unroll
-    for (int rb = 0; rb < NRB; ++rb) {
-        acc[rb] += __shfl_xor(acc[rb], 16);
-        acc[rb] += __shfl_xor(acc[rb], 32);
-    }
+    for (int j = 0; j < J; ++j)
+#pragma unroll
+        for (int rb = 0; rb < NRB; ++rb) {
+            acc[j][rb] += __shfl_xor(acc[j][rb], 16);
+            acc[j][rb] += __shfl_xor(acc[j][rb], 32);
+        }
     if (g == 0) {
 #pragma unroll
-        for (int rb = 0; rb < NRB; ++rb) partial[((long)split * I2 + i2) * Rp + 16 * rb + lm] = acc[rb];
+        for (int j = 0; j < J; ++j)
+            if (i2b + j < I2) {
+#pragma unroll
+                for (int rb = 0; rb < NRB; ++rb)
+                    partial[((long)split * I2 + i2b + j) * Rp + 16 * rb + lm] = acc[j][rb];
+            }
     }
 }
 
@@ -484,14 +510,15 @@ int dtree_plan(DtreePlan &d, int order, const long *dims, int dtype, long R, boo
     d.nsplit1 = (int)ceil_div(d.I2, (long)d.i2per);
     d.lds1 = dtree_lds_bytes(d.I0, d.nrb, d.i2per, d.Rp);
     if (d.lds1 > 160 * 1024) return TDX_OK;  // LDS per workgroup on gfx950
-    // k_m2: one wave per (i2, tile range), about 16 waves per CU: nsplit2 ranges of tper tiles
+    // k_m2: one wave per (kM2Slabs slabs, tile range), about 16 waves per CU: nsplit2 ranges
+    // of tper tiles
     const long ntile = d.I1 / 32;
-    long n2 = ceil_div(16 * cus, d.I2);
+    long n2 = ceil_div(16 * cus * kM2Slabs, d.I2);
     if (n2 > ntile) n2 = ntile;
     if (n2 < 1) n2 = 1;
     d.tper = (int)ceil_div(ntile, n2);
     d.nsplit2 = (int)ceil_div(ntile, (long)d.tper);
-    d.nb2 = (int)ceil_div(d.I2, 4L);
+    d.nb2 = (int)ceil_div(d.I2, 4L * kM2Slabs);
     const int rc = m1_call(d, nullptr, nullptr);
     if (rc) return rc;
     d.on = true;
