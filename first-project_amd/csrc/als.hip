@@ -210,7 +210,7 @@ __global__ __launch_bounds__(256) void k_gram_psum(const double *__restrict__ gr
                                                    double *__restrict__ Gout) {
     const int e = blockIdx.x * 256 + threadIdx.x;
     if (e >= RR) return;
-    Gout[e] = ordered_sum(gram_part + e, RR, nblk);
+    Gout[e] = ordered_sum<32>(gram_part + e, RR, nblk);
 }
 
 // The fast-fit relative error of Tensorlab cpd_als.m:281-283,
@@ -643,7 +643,6 @@ struct CpPlan {
     double *vinv_ws = nullptr;  // k_vinv fallback workspace, 2 R^2 doubles
     double *hpart = nullptr;    // k_gram_psum_fit: per-block partials (R^2 / 256 doubles)
     int *fit_done = nullptr;    // k_gram_psum_fit: ticket (0 between launches)
-    int *m2_done = nullptr;     // k_m2_vinv: ticket (0 between launches)
     bool shared_pass = true;    // modes 1 and 2 from one pass over X where eligible
     uint16_t *T = nullptr;      // shared-pass intermediate X x_0 A^T (BF16, I2 x I1 x Rp)
     size_t t_bytes = 0;
@@ -695,12 +694,12 @@ struct CpPlan {
             panel[m] = nullptr; G[m] = nullptr; Vinv[m] = nullptr;
         }
         void *ps[] = {partial, msum, ip_part, gram_part, fit_hist, counter, status, normx2, scratch, vinv_ws, T,
-                      hpart, fit_done, m2_done};
+                      hpart, fit_done};
         for (void *p : ps)
             if (p) (void)hipFree(p);
         partial = nullptr; msum = nullptr; ip_part = nullptr; gram_part = nullptr; fit_hist = nullptr;
         counter = nullptr; status = nullptr; normx2 = nullptr; scratch = nullptr; vinv_ws = nullptr; T = nullptr;
-        hpart = nullptr; fit_done = nullptr; m2_done = nullptr;
+        hpart = nullptr; fit_done = nullptr;
         t_bytes = 0;
         if (events) {
             for (int m = 0; m < 4; ++m) {
@@ -768,8 +767,6 @@ static int plan_setup(CpPlan &P, int order, const long *dims, long R, float *con
     TDX_HIP(hipMalloc(&P.hpart, sizeof(double) * ceil_div(R * R, 256L)));
     TDX_HIP(hipMalloc(&P.fit_done, sizeof(int)));
     TDX_HIP(hipMemset(P.fit_done, 0, sizeof(int)));
-    TDX_HIP(hipMalloc(&P.m2_done, sizeof(int)));
-    TDX_HIP(hipMemset(P.m2_done, 0, sizeof(int)));
     if (P.ls_mode) {
         auto alloc = [&](void **p, size_t bytes) -> int {
             TDX_HIP(hipMalloc(p, bytes));
@@ -936,8 +933,8 @@ static int enqueue_sweep(CpPlan &P, const Storage &S, hipStream_t s, bool fit) {
     // mode 2 from T.  The inverse (4 KB of LDS) fits beside either kernel's workgroups.
     // The fork and join between the streams cost ~15 us of replay time per mode, more
     // than the inverse itself at R <= 16 (~7 us), so small ranks keep it in line.
-    // Order 3 with the shared pass, R > 16: mode 1's Gram sum and mode 2's inverse run inside
-    // the pass over T (k_m2_vinv), so mode 2 has no inverse launch of its own.
+    // Order 3 with the shared pass, R > 16: mode 2's inverse runs inside the pass over T
+    // (k_m2_vinv), after mode 1's Gram sum in line, so mode 2 has no inverse launch of its own.
     const bool fuse2 = S.dt.on && P.order == 3 && R > 16;
     auto beside_of = [&](int n) {
         if (fuse2 && n == 2) return false;
@@ -991,9 +988,8 @@ static int enqueue_sweep(CpPlan &P, const Storage &S, hipStream_t s, bool fit) {
             rows_pad = S.dt.I1;
         } else if (dt && n == 2) {
             if (fuse2)
-                st = dtree_m2_vinv_launch(S.dt, P.T, P.panel[1], P.partial, P.gram_part,
-                                          (int)ceil_div(P.dims[1], kSolveRows), P.G[0], P.G[1], R, P.Vinv[2],
-                                          P.status, P.vinv_ws, P.m2_done, s);
+                st = dtree_m2_vinv_launch(S.dt, P.T, P.panel[1], P.partial, P.G[0], P.G[1], R, P.Vinv[2],
+                                          P.status, P.vinv_ws, s);
             else
                 st = dtree_m2_launch(S.dt, P.T, P.panel[1], P.partial, s);
             nsplit = S.dt.nsplit2;
@@ -1040,8 +1036,6 @@ static int enqueue_sweep(CpPlan &P, const Storage &S, hipStream_t s, bool fit) {
                 TDX_HIP(hipEventRecord(P.ev_fork[n + 1], s));  // the branch follows mode n + 1's kernel
                 forked[n + 1] = true;
                 prev_nblk = nblk;
-            } else if (fuse2 && n == 1) {
-                // G_1 is summed by the pass over T (k_m2_vinv)
             } else if (R <= 16 && n + 1 < P.order && !beside_of(n + 1)) {
                 pending_gram = n;  // the next inverse sums it (one launch fewer per mode)
                 pending_nblk = nblk;

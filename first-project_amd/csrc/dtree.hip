@@ -412,34 +412,23 @@ __global__ __launch_bounds__(256) void k_m2(const uint16_t *__restrict__ T, cons
     m2_block<NRB>(T, panelB, partial, I1, I2, Rp, tper, nb2, blockIdx.x);
 }
 
-// The pass over T with the mode-2 side work of the sweep in the same launch: blocks
-// [0, ngb) sum mode 1's Gram partials into G1 (each entry in block order, as k_gram_psum),
-// and the last of them to finish (atomic ticket, reset by that block) computes the mode-2
-// inverse (G0 * G1)^{-1} (as k_vinv); blocks ngb.. are k_m2's.  The Gram blocks come first in
-// dispatch order, so the inverse starts at once instead of waiting for a separate Gram
-// launch, a fork and a join; the solve after this kernel needs nothing else.
+// The pass over T with the mode-2 inverse (G0 * G1)^{-1} in the same launch: block 0
+// computes it (as k_vinv), blocks 1.. are k_m2's.  The inverse block comes first in dispatch
+// order, so it runs beside the pass instead of after it, without a fork and a join; the solve
+// after this kernel needs nothing else.  G1 is summed by the launch before this one: summed
+// here by leading blocks, with an atomic ticket choosing the block that inverts, the sum and
+// the ticket took ~17 us at 1024^3 R64 (memory round trips behind the pass's loads) before
+// the inverse could start.
 template <int NRB, int NS>
 __global__ __launch_bounds__(256) void k_m2_vinv(const uint16_t *__restrict__ T, const float *__restrict__ panelB,
                                                  float *__restrict__ partial, long I1, long I2, int Rp, int tper,
-                                                 int nb2, int ngb, const double *__restrict__ gram_part, int gnblk,
-                                                 const double *G0, double *G1, int R, double *__restrict__ Vinv,
-                                                 int *__restrict__ status, double *__restrict__ fb_ws,
-                                                 int *__restrict__ done) {
-    if ((int)blockIdx.x >= ngb) {
-        m2_block<NRB>(T, panelB, partial, I1, I2, Rp, tper, nb2, (int)blockIdx.x - ngb);
+                                                 int nb2, const double *G0, const double *G1, int R,
+                                                 double *__restrict__ Vinv, int *__restrict__ status,
+                                                 double *__restrict__ fb_ws) {
+    if (blockIdx.x > 0) {
+        m2_block<NRB>(T, panelB, partial, I1, I2, Rp, tper, nb2, (int)blockIdx.x - 1);
         return;
     }
-    __shared__ int last;
-    const int RR = R * R;
-    const int e = (int)blockIdx.x * 256 + (int)threadIdx.x;
-    if (e < RR) G1[e] = ordered_sum(gram_part + e, RR, gnblk);
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) last = (atomicAdd(done, 1) == ngb - 1);
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    if (threadIdx.x == 0) *done = 0;
     const double *G[4] = {G0, G1, G0, G0};
     vinv_block<NS>(G, 3, 2, R, Vinv, status, fb_ws);
 }
@@ -549,14 +538,13 @@ int dtree_m1_launch(const DtreePlan &d, const void *X, const float *panelA, cons
 }
 
 int dtree_m2_vinv_launch(const DtreePlan &d, const uint16_t *T, const float *panelB, float *partial,
-                         const double *gram_part, int gnblk, const double *G0, double *G1, int R, double *Vinv,
-                         int *status, double *fb_ws, int *done, hipStream_t s) {
+                         const double *G0, const double *G1, int R, double *Vinv, int *status, double *fb_ws,
+                         hipStream_t s) {
     if (!d.on || R <= 16 || R > kMaxR) return record(TDX_ERR_UNSUPPORTED);
-    const int ngb = (int)ceil_div((long)R * R, 256L);
-    const int grid = ngb + d.nb2 * d.nsplit2;
-#define TDX_M2V(NRB_, NS_)                                                                                         \
-    k_m2_vinv<NRB_, NS_><<<grid, 256, 0, s>>>(T, panelB, partial, d.I1, d.I2, d.Rp, d.tper, d.nb2, ngb, gram_part, \
-                                              gnblk, G0, G1, R, Vinv, status, fb_ws, done)
+    const int grid = 1 + d.nb2 * d.nsplit2;
+#define TDX_M2V(NRB_, NS_)                                                                                     \
+    k_m2_vinv<NRB_, NS_><<<grid, 256, 0, s>>>(T, panelB, partial, d.I1, d.I2, d.Rp, d.tper, d.nb2, G0, G1, R, \
+                                              Vinv, status, fb_ws)
     // 16 < R <= 32: Rp 32 (NRB 2); 32 < R <= 64: Rp 48 or 64
     if (d.nrb == 2) TDX_M2V(2, 8);
     else if (d.nrb == 3) TDX_M2V(3, 16);

@@ -81,11 +81,11 @@ int dtree_m1_launch(const DtreePlan &d, const void *X, const float *panelA, cons
                     float *partial, hipStream_t s);
 // partial[split][i2][Rp] (nsplit2 splits, rows_pad I2) from T and the mode-1 panel
 int dtree_m2_launch(const DtreePlan &d, const uint16_t *T, const float *panelB, float *partial, hipStream_t s);
-// the pass over T with mode 1's Gram sum (gram_part, gnblk blocks -> G1) and the mode-2 inverse
-// (G0 * G1)^{-1} -> Vinv in the same launch (16 < R <= 64); done: a zeroed ticket
+// the pass over T with the mode-2 inverse (G0 * G1)^{-1} -> Vinv in the same launch
+// (16 < R <= 64)
 int dtree_m2_vinv_launch(const DtreePlan &d, const uint16_t *T, const float *panelB, float *partial,
-                         const double *gram_part, int gnblk, const double *G0, double *G1, int R, double *Vinv,
-                         int *status, double *fb_ws, int *done, hipStream_t s);
+                         const double *G0, const double *G1, int R, double *Vinv, int *status, double *fb_ws,
+                         hipStream_t s);
 
 // panel packing (optionally scaling row i by scale[i])
 int pack_panel(const float *U, long I, long R, int Rp, const float *scale, float *panel, hipStream_t s);
