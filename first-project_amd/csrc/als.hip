@@ -210,7 +210,7 @@ __global__ __launch_bounds__(256) void k_gram_psum(const double *__restrict__ gr
                                                    double *__restrict__ Gout) {
     const int e = blockIdx.x * 256 + threadIdx.x;
     if (e >= RR) return;
-    Gout[e] = ordered_sum<32>(gram_part + e, RR, nblk);
+    Gout[e] = ordered_sum(gram_part + e, RR, nblk);
 }
 
 // The fast-fit relative error of Tensorlab cpd_als.m:281-283,

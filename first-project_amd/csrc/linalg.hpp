@@ -206,115 +206,65 @@ __device__ inline int block_spd_inverse(const double *V, double *L, double *W, d
     return 1;
 }
 
-// lane `lane` (wave-uniform) of a double
-__device__ __forceinline__ double readlane_f64(double v, int lane) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)b, lane), hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
-    return __hiloint2double(hi, lo);
-}
-
 // Vinv = V^{-1} for a symmetric R x R V (R <= 4 NS <= 64) by the symmetric sweep operator
-// taken four pivots at a time, for a workgroup of exactly 256 threads.  Sweeping the pivots
-// K = {4j, .., 4j + 3} one after another equals sweeping the block at once: with P = A[K][K],
-//   A[K][K] <- -P^{-1},  A[i][K] <- A[i][K] P^{-1},  A[i][c] <- A[i][c] - A[i][K] P^{-1} A[K][c]
-// (i, c outside K).  Lane i of wave w holds row i (rows >= R are zero) of the columns
-// c = 16 (s / 4) + 4 w + s % 4 in register slot s, so the four columns of block j are slots
-// 4 (j / 4) .. 4 (j / 4) + 3 of wave j % 4, the block's owner.  Per block the owner sweeps its
-// four columns pivot by pivot in registers (the pivot entries by readlane, the operations
-// per element of block_spd_inverse's sweep) and publishes the swept panel S and the
-// panel as it was before, A0; after one barrier every wave applies the block to its other
-// columns with a four-term update,
-//   A[i][c] <- alpha A[i][c] - sum_q S[i][q] A0[c][q]      (alpha 0 on the block's rows, else 1;
-// A0[c][q] = A[c][4j + q] stands for A[4j + q][c]: A is symmetric under the sweep), which
-// equals the four rank-one steps up to rounding.  One barrier per block: 16 at R = 64 where
-// the pivot-by-pivot form has 64, each behind an LDS round trip and a division (~28 us
-// against ~22 for the 64 x 64 inverse beside the pass over T at 1024^3).  The pivot test is
-// block_spd_inverse's (relative to V's diagonal).  Returns 1 (every thread) on a
-// failed pivot, Vinv not written.  LDS: pan and old 512 doubles each, flags 2 ints, diag R
-// doubles (written here from the lanes that hold the diagonal).
+// of block_spd_inverse (same pivots, same operations per element, so the same result bit
+// for bit), for a workgroup of exactly 256 threads.  vat(i, c) returns V[i][c] (read once
+// per element, at the start); diag[k] = V[k][k] (any memory visible to the workgroup).
+// Layout: lane i of wave w holds row i (rows >= R are zero) of the columns c = w + 4 s in
+// register slot s, so the slot holding pivot column k is slot k/4 of wave k%4 (a uniform
+// register index, s_set_gpr_idx) and only that wave stores the column (one 8-byte LDS write
+// per lane, a uniform branch).  The column
+// goes to an LDS half Wb[(k & 1) * 64 ..], permuted so that the values a wave needs (rows
+// w, w+4, ...) are contiguous: the entry of row c at (c % 4) * 16 + c / 4 (16-byte broadcast
+// reads).  One barrier per pivot (the halves alternate).  Returns 1 (every thread) when a
+// relative pivot is below 1e-13 or not finite; Vinv is then not written.  Wb: 128 doubles.
+// diag (R doubles, LDS) receives V[k][k] from the lanes that hold it.
 template <int NS, typename VAt>
-__device__ inline int sweep_inverse_block4(VAt &vat, double *diag, double *pan, double *old, int *flags,
-                                           double *Vinv, int R) {
+__device__ inline int sweep_inverse_rows(VAt &vat, double *diag, double *Wb, double *Vinv, int R, int *fail) {
     static_assert(NS == 4 || NS == 8 || NS == 16, "NS: slots per lane");
-    constexpr int NM = NS / 4;  // blocks per wave
     const int tid = threadIdx.x, i = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     double a[NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s) {  // every load issued, then one wait (see VGather)
-        const int c = 16 * (s >> 2) + 4 * w + (s & 3);
+        const int c = w + 4 * s;
         vat.load(s, i, c, i < R && c < R);
     }
     vat.finish(a);
 #pragma unroll
     for (int s = 0; s < NS; ++s)
-        if (16 * (s >> 2) + 4 * w + (s & 3) == i && i < R) diag[i] = a[s];
+        if (w + 4 * s == i && i < R) diag[i] = a[s];
+    const int pi = (i & 3) * 16 + (i >> 2);  // this lane's row in the permuted column
+    if (tid == 0) *fail = 0;
     __syncthreads();
-    const int nb = (R + 3) >> 2;
     int bad = 0;
-    for (int j = 0; j < nb; ++j) {
-        double *pj = pan + (j & 1) * 256, *oj = old + (j & 1) * 256;  // [row][q]
-        const int m = j >> 2;                                           // the block's slot group
-        if (w == (j & 3)) {
-            double p[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                p[q] = a[q];
-#pragma unroll
-                for (int mm = 1; mm < NM; ++mm) p[q] = (m == mm) ? a[4 * mm + q] : p[q];
-            }
-#pragma unroll
-            for (int q = 0; q < 4; ++q) oj[4 * i + q] = p[q];
-            int fb = 0;
-#pragma unroll
-            for (int q0 = 0; q0 < 4; ++q0) {
-                const int k = 4 * j + q0;
-                if (k < R && !fb) {
-                    const double d = readlane_f64(p[q0], k);
-                    if (!(d > 1e-13 * diag[k]) || !isfinite(d)) {
-                        fb = 1;
-                    } else {
-                        const double dinv = 1.0 / d;
-                        const double t = p[q0] * dinv;
-                        const bool rk = i == k;
-                        const double alpha = rk ? 0.0 : 1.0, beta = rk ? dinv : -t;
-                        double wr[4];  // A[4j + q][k], standing for A[k][4j + q]
-#pragma unroll
-                        for (int q = 0; q < 4; ++q) wr[q] = readlane_f64(p[q0], 4 * j + q);
-#pragma unroll
-                        for (int q = 0; q < 4; ++q) p[q] = fma(beta, wr[q], alpha * p[q]);
-                        p[q0] = rk ? -dinv : t;
-                    }
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < 4; ++q) pj[4 * i + q] = p[q];
-            if (i == 0) flags[j & 1] = fb;
-        }
+    for (int k = 0; k < 4 * NS; ++k) {
+        if (k >= R) break;
+        double *Wk = Wb + (k & 1) * 64;
+        if (w == (k & 3)) Wk[pi] = a[k >> 2];
         __syncthreads();
-        if (flags[j & 1]) {  // uniform; this flag is next written after the following barrier
+        const double d = Wk[(k & 3) * 16 + (k >> 2)];
+        if (!(d > 1e-13 * diag[k]) || !isfinite(d)) {
             bad = 1;
-            break;
+            break;  // uniform: every thread read the same pivot
         }
-        double S[4];
+        const double dinv = 1.0 / d;
+        const double t = Wk[pi] * dinv;
+        const bool rk = i == k;
+        const double alpha = rk ? 0.0 : 1.0, beta = rk ? dinv : -t;
+        const double *wr = Wk + w * 16;  // A[w + 4 s][k], s = 0 .. NS-1
 #pragma unroll
-        for (int q = 0; q < 4; ++q) S[q] = pj[4 * i + q];
-        const double alpha = ((i >> 2) == j) ? 0.0 : 1.0;
-        const bool owner = w == (j & 3);
-#pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            const int c = 16 * (s >> 2) + 4 * w + (s & 3);
-            const double *o = oj + 4 * c;  // A0[c][0..3], the same for the whole wave
-            double u = alpha * a[s];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) u = fma(-S[q], o[q], u);
-            a[s] = (owner && (s >> 2) == m) ? S[s & 3] : u;
-        }
+        for (int s = 0; s < NS; ++s) a[s] = fma(beta, wr[s], alpha * a[s]);
+        if (w == (k & 3)) a[k >> 2] = rk ? -dinv : t;
     }
-    if (bad) return 1;
+    if (bad) {
+        if (tid == 0) *fail = 1;
+        __syncthreads();
+        return 1;
+    }
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-        const int c = 16 * (s >> 2) + 4 * w + (s & 3);
+        const int c = w + 4 * s;
         if (i < R && c < R) Vinv[i + c * R] = -a[s];
     }
     __syncthreads();

@@ -8,31 +8,26 @@ namespace tdx {
 
 static constexpr int kMaxR = 64;  // rank limit of the LDS-resident solve and of the inverse
 
-// p[0] + p[stride] + ... + p[(n - 1) stride], added in index order, NF loads in flight (the
-// ones past n re-read a valid entry and are not added).  The result does not depend on NF;
-// a larger NF costs 2 NF VGPRs and saves a memory round trip per NF entries (several us per
-// round while a streaming kernel loads the memory system).
-template <int NF = 16>
+// p[0] + p[stride] + ... + p[(n - 1) stride], added in index order, 16 loads in flight (the
+// ones past n re-read a valid entry and are not added)
 __device__ __forceinline__ double ordered_sum(const double *p, long stride, int n) {
     double s = 0.0;
-    for (int b = 0; b < n; b += NF) {
-        double v[NF];
+    for (int b = 0; b < n; b += 16) {
+        double v[16];
 #pragma unroll
-        for (int u = 0; u < NF; ++u) v[u] = p[(long)(b + u < n ? b + u : n - 1) * stride];
-        __builtin_amdgcn_sched_barrier(0);  // every load above is issued before the first add
+        for (int u = 0; u < 16; ++u) v[u] = p[(long)(b + u < n ? b + u : n - 1) * stride];
 #pragma unroll
-        for (int u = 0; u < NF; ++u)
+        for (int u = 0; u < 16; ++u)
             if (b + u < n) s += v[u];
     }
     return s;
 }
 
-// V = hadamard_{m != mode} G_m; Vinv = V^{-1} by the symmetric sweep operator (FP64, in
-// registers: sweep_inverse_wave16 for R <= 16, else sweep_inverse_block4); if a pivot fails,
-// a pseudo-inverse from the Jacobi eigendecomposition (eigenvalues below 1e-12 * max dropped)
-// and status |= 1.
+// V = hadamard_{m != mode} G_m; Vinv = V^{-1} by the symmetric sweep operator (FP64, the
+// register-row form of sweep_inverse_rows); if a pivot fails, a pseudo-inverse from the
+// Jacobi eigendecomposition (eigenvalues below 1e-12 * max dropped) and status |= 1.
 // NS: register slots per lane, R <= 4 NS (4, 8 or 16 for R <= 16, 32, 64).
-// About 9 KB of LDS, so that the workgroup fits on a CU beside an MTTKRP or TTM workgroup
+// About 4 KB of LDS, so that the workgroup fits on a CU beside an MTTKRP or TTM workgroup
 // (up to 136 KB): V is read from the Grams, and the rare fallback works in fb_ws (global,
 // 2 R^2 doubles: a copy of V and the eigenvectors).
 // V[i][c] = product of the other modes' Grams in mode order, ((G_a G_b) G_c), the value of
@@ -125,10 +120,11 @@ __device__ inline int sweep_inverse_wave16(VAt &vat, double *Vinv, int R) {
 template <int NS>
 __device__ __forceinline__ void vinv_block(const double *const *G, int order, int mode, int R, double *Vinv, int *status,
                            double *fb_ws) {
-    __shared__ double wk[1024];  // the blocked sweep's panels, then the fallback's work arrays
+    __shared__ double Wb[128];
     __shared__ double dg[kMaxR];
-    __shared__ int fail[2];
-    double *sh = wk, *cs = wk + 256;  // 256 and 4 (kMaxR / 2 + 1) doubles
+    __shared__ double sh[256];
+    __shared__ double cs[4 * (kMaxR / 2 + 1)];
+    __shared__ int fail;
     // V[i][c] = product of the other modes' Grams in mode order, ((G_a G_b) G_c), the value
     // of multiplying them into 1.0 one mode after another.  Straight-line loads (the third is
     // read and not used at order 3), so a caller's unrolled fill issues every load before its
@@ -142,13 +138,13 @@ __device__ __forceinline__ void vinv_block(const double *const *G, int order, in
     if constexpr (NS == 4) {
         if (threadIdx.x < 64) {
             const int b = sweep_inverse_wave16(vat, Vinv, R);
-            if (threadIdx.x == 0) fail[0] = b;
+            if (threadIdx.x == 0) fail = b;
         }
         __syncthreads();
-        bad = fail[0];
+        bad = fail;
     } else {
-        // (writes the diagonal into dg; a failed pivot is seen by every thread after a barrier)
-        bad = sweep_inverse_block4<NS>(vat, dg, wk, wk + 512, fail, Vinv, R);
+        // (sweep_inverse_rows writes the diagonal into dg and starts with a barrier)
+        bad = sweep_inverse_rows<NS>(vat, dg, Wb, Vinv, R, &fail);
     }
     if (bad) {
         double *L = fb_ws, *Q = fb_ws + R * R;
