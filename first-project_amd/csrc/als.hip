@@ -837,7 +837,7 @@ static int storage_setup(CpPlan &P, Storage &S, const void *X, int dtype, const 
     S.dtype = dtype;
     S.scales = scales;
     for (int m = 0; m < P.order; ++m) {
-        int st = mttkrp_plan(S.mp[m], P.order, P.dims, m, dtype, P.R, X);
+        int st = mttkrp_plan(S.mp[m], P.order, P.dims, m, dtype, P.R, X, 0, split);
         if (st) return st;
         if (split) {
             if (dtype != TDX_BF16 || !S.mp[m].lds) {

@@ -33,9 +33,10 @@ struct MttkrpPlan {
     size_t partial_floats() const { return (size_t)nsplit * rows_pad * Rp; }
 };
 
-// chooses kernel variant and split; returns TDX_OK or an error
+// chooses kernel variant and split; returns TDX_OK or an error.  split: the plan is for the
+// hi + lo Khatri-Rao block (BF16 storage; the caller sets p.split when the LDS kernel is chosen)
 int mttkrp_plan(MttkrpPlan &p, int order, const long *dims, int mode, int dtype, long rank,
-                const void *X, int target_wgs = 0);
+                const void *X, int target_wgs = 0, bool split = false);
 
 // Launches the split-K MTTKRP writing partial[split][row][Rp] (FP32).
 // panels[m] are factor panels (panels[mode] unused); for FP8 storage panels[0] must be
